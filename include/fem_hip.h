@@ -1,0 +1,159 @@
+/*
+ * C-ABI of the MI355X SolverFEM substep library (libfem_hip.so).
+ *
+ * Plain pointers and sizes only.  "dev" pointers are HIP device pointers
+ * (e.g. torch.Tensor.data_ptr() of cuda tensors), "host" pointers are CPU
+ * memory.  Vectors of nodes are AoS float32 [P][3]; 3x3 blocks are row-major
+ * float32 [9].  Streams are hipStream_t passed as void*.  Every function
+ * returns FEM_OK (0) or an error code; fem_last_error() describes the last
+ * failure of the calling thread.
+ *
+ * Entry points that run per substep (fem_restore_fixed, fem_assemble,
+ * fem_step, fem_eval_forces) allocate nothing, never synchronize and are
+ * safe to record into a HIP graph.
+ *
+ * Reference: nvidia-beer/newton-solver-fem, src/newton_solver_fem/_src/
+ * solvers/fem/ (paths below are relative to that directory).
+ */
+#ifndef FEM_HIP_H
+#define FEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_system fem_system;
+
+enum {
+  FEM_OK = 0,
+  FEM_ERR_ARG = 1,         /* invalid argument (shape, range, pointer)        */
+  FEM_ERR_HIP = 2,         /* a HIP runtime call failed                       */
+  FEM_ERR_UNSUPPORTED = 3, /* mesh outside the library's limits               */
+  FEM_ERR_STATE = 4        /* call order misuse (e.g. buffers not bound)      */
+};
+
+/* Force terms for fem_eval_forces (solver.py:1357-1556 evaluators). */
+enum {
+  FEM_TERM_TET = 1,
+  FEM_TERM_HEX = 2,
+  FEM_TERM_GROUND = 4,
+  FEM_TERM_GRAVITY = 8,
+  FEM_TERM_DAMPING = 16
+};
+
+/* Device-side solve status bits (fem_solve_stats.error_flags). */
+enum { FEM_STATUS_BARRIER_TIMEOUT = 1 };
+
+/* Model arrays, all HOST pointers, copied at creation.  Mirrors the Model /
+ * LinearFEMData fields SolverFEM.__init__ consumes (solver.py:212-341,
+ * builder.py:12-20): particle q/mass/inv_mass/radius/flags, tet
+ * indices/poses/materials, hex indices/inv_J0/det_J0_w/materials, and the
+ * sorted unique fixed_particle_indices (solver.py:244-249).  Independent
+ * meshes may be packed back to back: instance_offsets[k]..[k+1] are the
+ * particles of instance k (instance_count 0 or 1 means one instance). */
+typedef struct fem_model_desc {
+  int32_t particle_count;
+  const float* particle_q;        /* [P][3] construction positions (anchors) */
+  const float* particle_mass;     /* [P] */
+  const float* particle_inv_mass; /* [P] */
+  const float* particle_radius;   /* [P] */
+  const int32_t* particle_flags;  /* [P], bit 0 = ACTIVE */
+  int32_t tet_count;
+  const int32_t* tet_indices;     /* [T][4] */
+  const float* tet_poses;         /* [T][9] inverse rest shape matrix */
+  const float* tet_materials;     /* [T][3] mu, lambda, k_damp */
+  int32_t hex_count;
+  const int32_t* hex_indices;     /* [H][8] VTK order */
+  const float* hex_inv_J0;        /* [H][8][9] */
+  const float* hex_det_J0_w;      /* [H][8] */
+  const float* hex_materials;     /* [H][3] */
+  int32_t fixed_count;
+  const int32_t* fixed_indices;   /* [fixed_count], sorted, unique */
+  int32_t instance_count;
+  const int32_t* instance_offsets; /* [instance_count+1] or NULL */
+} fem_model_desc;
+
+/* Physical and solver parameters (SolverFEM.__init__ arguments,
+ * solver.py:212-240; model.gravity). */
+typedef struct fem_params {
+  float min_stretch, max_stretch;
+  int32_t has_ground;
+  float ground[4];
+  float ground_ke, ground_kd, ground_kf, ground_mu;
+  float linear_damping;
+  float gravity[3];
+  int32_t enable_tet_activations;
+  float tolerance;          /* linear_solver_tolerance (relative) */
+  int32_t maxiter;          /* linear_solver_maxiter */
+  int32_t early_exit;       /* 1: stop at the tolerance; 0: full budget, residual-masked updates */
+} fem_params;
+
+/* Replaces SolverFEM.__init__'s pattern construction (solver.py:312-341:
+ * triplet layout + bsr_set_from_triplets) and sparse_update.map_triplet_blocks
+ * (sparse_update.py:9-30): builds the BSR pattern of A = M + h^2 K and the
+ * per-node element incidence/slot lists on the host, uploads model data. */
+int fem_create(const fem_model_desc* desc, fem_system** out);
+int fem_destroy(fem_system* sys);
+const char* fem_last_error(void);
+
+/* Number of stored 3x3 blocks (A_bsr.nnz) and the largest row length. */
+int fem_pattern_size(const fem_system* sys, int64_t* nnz, int32_t* max_row_blocks);
+
+/* Binds caller-owned device buffers that stay valid for the system's life:
+ * offsets [P+1] int32, columns [nnz] int32 (both filled here, synchronously),
+ * values [nnz][9], inv_diag [P][9] (block-Jacobi inverse, SolverFEM
+ * _pc_inv_diag) and increment [P][3] (SolverFEM.dv warm start). */
+int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float* values_dev, float* inv_diag_dev,
+             float* increment_dev);
+
+int fem_set_params(fem_system* sys, const fem_params* params);
+
+/* Launch shapes: lanes per block row in the element/assembly kernels and in
+ * the CG SpMV (4..64, power of two), and the CG workgroup count (0 = auto). */
+int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t cg_lanes, int32_t cg_workgroups);
+int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* cg_lanes, int32_t* cg_workgroups,
+                   int32_t* cg_rows_per_thread);
+
+/* FixedParticles.restore / restore_positions (fixed_particles.py:11-21,
+ * 152-168): q[i] = anchor[i] (and qd[i] = 0 when qd_dev is not NULL). */
+int fem_restore_fixed(fem_system* sys, float* q_dev, float* qd_dev, void* stream);
+
+/* refresh_tangent's numeric assembly (solver.py:769-857 ->
+ * build_system_matrix_tet_kernel kernels.py:635-850,
+ * build_system_matrix_hex_kernel :1609-1747, mass diagonal :141-153,
+ * accumulate_triplet_blocks sparse_update.py:33-59,
+ * invert_block_diagonal_kernel kernels.py:1836-1849): writes A's values at
+ * positions q and the block-Jacobi inverse.  q is read only; callers restore
+ * fixed positions first (fem_restore_fixed). */
+int fem_assemble(fem_system* sys, const float* q_dev, float dt, void* stream);
+
+/* SolverFEM.step fast path (solver.py:942-1002, implicit_integration
+ * :1254-1351): restores fixed q/qd in the input state, writes the projected
+ * right-hand side h*sum(f) + h*ext - (A-M)v into f_out (state_in.particle_f),
+ * solves (PAP+I-P) dv = P b with block-Jacobi PCG (relative tolerance,
+ * atol 0, warm start from the bound increment) and integrates
+ * v' = v + dv, q' = q + h v' into q_out/qd_out.  ext_dev and act_dev may be
+ * NULL. */
+int fem_step(fem_system* sys, float* q_in_dev, float* qd_in_dev, float* f_out_dev, const float* ext_dev,
+             const float* act_dev, float* q_out_dev, float* qd_out_dev, float dt, void* stream);
+
+/* Sum of the selected unscaled force terms (FEM_TERM_* mask) at (q, qd):
+ * eval_tetrahedra + eval_tet_activations (kernels.py:484-632),
+ * eval_hexahedra (:1481-1601), eval_particle_ground_contacts (:1132-1195),
+ * eval_gravity_from_array (:1273-1286), eval_linear_damping_kernel (:1289-1307). */
+int fem_eval_forces(fem_system* sys, int32_t terms, const float* q_dev, const float* qd_dev, const float* act_dev,
+                    float* out_dev, void* stream);
+
+/* Last solve's per-instance statistics (synchronizes the stream): CG
+ * iterations performed, final squared residual, squared absolute tolerance,
+ * and FEM_STATUS_* error flags.  Arrays have instance_count entries. */
+int fem_solve_stats(fem_system* sys, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,
+                    int32_t* error_flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM_HIP_H */

@@ -1,0 +1,1249 @@
+// MI355X (gfx950) kernels and C-ABI of the SolverFEM substep.
+//
+// Kernels (one row = one node = one 3x3 block row of A):
+//   k_prepare    fixed-node restore of the input state, per-step scalar reset
+//   k_assemble   tangent assembly: G lanes per row walk the row's element
+//                incidences, add the element blocks into an LDS copy of the
+//                row (no global atomics), then stream the row out coalesced
+//                and invert its diagonal block (block-Jacobi)
+//   k_rhs        G lanes per row: elastic + damping forces from the row's
+//                incidences, node forces, h-scaling, elastic predictor
+//                (block row of A), fixed-row projection, per-instance max|b|
+//   k_cg         persistent block-Jacobi PCG: every workgroup owns a
+//                contiguous row range for the whole solve (x, r, z, p, q and
+//                M^-1 of its rows stay in registers), SpMV with G lanes per
+//                row, two instance-wide barriers per iteration, device-side
+//                residual test, fused update_state + fixed restore epilogue
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fem_hip.h"
+#include "fem_math.hpp"
+#include "fem_pattern.hpp"
+
+using namespace fem;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(FEM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Workgroup b of n runs on XCD b % 8; give each XCD a contiguous 1/8 of the
+// logical blocks so that neighbouring rows share that XCD's L2.
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+  if (n < 8 || (n & 7)) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
+struct RowArgs {
+  int P;
+  const int* offsets;
+  const int* cols;
+  const int* diag_rel;
+  const int* tinc_ptr;
+  const int* tinc;
+  const uint32_t* tslots;
+  const int* hinc_ptr;
+  const int* hinc;
+  const uint32_t* hslots;
+  const int* tet_idx;
+  const float* tet_pose;
+  const float* tet_mat;
+  const int* hex_idx;
+  const float* hex_invJ;
+  const float* hex_detw;
+  const float* hex_mat;
+  const float* mass;
+  const float* inv_mass;
+  const float* radius;
+  const int* flags;
+  const unsigned char* row_fixed;
+  const int* row_inst;
+  float mn, mx;
+  int has_ground;
+  float gn[4];
+  float ke, kd, kf, gmu;
+  float lin_damp;
+  float grav[3];
+  float gmag;
+  int use_act;
+};
+
+// ---------------------------------------------------------------------------
+// k_prepare: FixedParticles.restore(state_in) and per-step resets.
+// ---------------------------------------------------------------------------
+__global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float* __restrict__ anchor, float* q,
+                          float* qd, unsigned* rhs_max, unsigned* bar, int ninst, int* err) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nfixed) {
+    int i = fixed[t];
+    st3(q, i, ld3(anchor, i));
+    if (qd) st3(qd, i, v3{0.f, 0.f, 0.f});
+  }
+  if (t < ninst) {
+    rhs_max[t] = 0u;
+    bar[32 * t] = 0u;
+  }
+  if (t == 0 && err) *err = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Hex8 helpers operating on one incidence (element e, local corner la).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __restrict__ x,
+                                             const float* __restrict__ v, int e, int la) {
+  int nd[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) nd[b] = a.hex_idx[8 * (size_t)e + b];
+  v3 xs[8], vs[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    xs[b] = ld3(x, nd[b]);
+    vs[b] = ld3(v, nd[b]);
+  }
+  float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1], k_damp = a.hex_mat[3 * e + 2];
+  float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
+  v3 f = {0.f, 0.f, 0.f};
+  for (int g = 0; g < 8; ++g) {
+    v3 dN[8];
+    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, dN);
+    float F[9], dF[9];
+    hex_F(xs, dN, F);
+    clamp_stretch(F, a.mn, a.mx);
+    hex_F(vs, dN, dF);
+    float ic = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
+    float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
+    float cof[9];
+    cofactor(F, cof);
+    float J = fmaxf(det3(F), 0.01f);
+    float vol = k_lambda * (J - alpha);
+    float P[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P[k] = F[k] * sdev + dF[k] * k_damp + cof[k] * vol;
+    float w = a.hex_detw[8 * (size_t)e + g];
+    v3 dNa = (la == 0) ? dN[0] : (la == 1) ? dN[1] : (la == 2) ? dN[2] : (la == 3) ? dN[3]
+           : (la == 4) ? dN[4] : (la == 5) ? dN[5] : (la == 6) ? dN[6] : dN[7];
+    f = f - mv(P, dNa) * w;
+  }
+  return f;
+}
+
+// Row `la` of the Hex8 tangent: K_ab, b = 0..7, summed over Gauss points, x h^2.
+__device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* __restrict__ x, int e, int la,
+                                                float h2, float K[8][9]) {
+  int nd[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) nd[b] = a.hex_idx[8 * (size_t)e + b];
+  v3 xs[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) xs[b] = ld3(x, nd[b]);
+  float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1];
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) K[b][k] = 0.f;
+  for (int g = 0; g < 8; ++g) {
+    v3 dN[8];
+    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, dN);
+    float F[9];
+    hex_F(xs, dN, F);
+    clamp_stretch(F, a.mn, a.mx);
+    float ic = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
+    float ic1 = fmaxf(ic + 1.f, 1e-6f);
+    float s = 1.f - 1.f / ic1;
+    float coef = k_mu * 2.f / (ic1 * ic1);
+    float cof[9];
+    cofactor(F, cof);
+    float w = a.hex_detw[8 * (size_t)e + g];
+    v3 dNa = (la == 0) ? dN[0] : (la == 1) ? dN[1] : (la == 2) ? dN[2] : (la == 3) ? dN[3]
+           : (la == 4) ? dN[4] : (la == 5) ? dN[5] : (la == 6) ? dN[6] : dN[7];
+    v3 fa = mv(F, dNa), ja = mv(cof, dNa);
+    const float fa3[3] = {fa.x, fa.y, fa.z}, ja3[3] = {ja.x, ja.y, ja.z};
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      v3 fb = mv(F, dN[b]), jb = mv(cof, dN[b]);
+      float dab = dot(dNa, dN[b]);
+      const float fb3[3] = {fb.x, fb.y, fb.z}, jb3[3] = {jb.x, jb.y, jb.z};
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float kdev = ((r == c) ? k_mu * s * dab : 0.f) + fa3[r] * fb3[c] * coef;
+          float kvol = ja3[r] * jb3[c] * k_lambda;
+          K[b][3 * r + c] = K[b][3 * r + c] + (kdev + kvol) * w;
+        }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) K[b][k] = K[b][k] * h2;
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// k_assemble: A = M + h^2 K at positions x, and M^-1 = inv(diag blocks).
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const float* __restrict__ x, float h2,
+                                                    float* __restrict__ vals, float* __restrict__ minv, int maxnnz) {
+  extern __shared__ float lds[];
+  constexpr int NG = BLOCK / G;
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int i = blk * NG + grp;
+  const bool active = i < a.P;
+  float* acc = lds + (size_t)grp * maxnnz * 9;
+  int k0 = 0, nnz = 0, drel = -1;
+  float m = 0.f;
+  if (active) {
+    k0 = a.offsets[i];
+    nnz = a.offsets[i + 1] - k0;
+    drel = a.diag_rel[i];
+    m = a.mass[i];
+  }
+  for (int f = lane; f < nnz * 9; f += G) {
+    int blkid = f / 9, c = f - 9 * blkid;
+    acc[f] = (blkid == drel && (c == 0 || c == 4 || c == 8)) ? m : 0.f;
+  }
+  __syncthreads();
+  if (active) {
+    const int t1 = a.tinc_ptr[i + 1];
+    for (int t = a.tinc_ptr[i] + lane; t < t1; t += G) {
+      const int code = a.tinc[t];
+      const int e = code >> 2, la = code & 3;
+      const uint32_t sl = a.tslots[t];
+      const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+      const v3 xs[4] = {ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w)};
+      TetGeo g;
+      tet_geo(xs, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+      float K[4][9];
+      tet_tangent_row(g, a.tet_mat[3 * e], a.tet_mat[3 * e + 1], h2, la, K);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        float* dst = acc + 9 * ((sl >> (8 * b)) & 255u);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) atomicAdd(dst + c, K[b][c]);
+      }
+    }
+    const int h1 = a.hinc_ptr[i + 1];
+    for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
+      const int code = a.hinc[t];
+      const int e = code >> 3, la = code & 7;
+      const uint32_t s0 = a.hslots[2 * (size_t)t], s1 = a.hslots[2 * (size_t)t + 1];
+      float K[8][9];
+      hex_tangent_row(a, x, e, la, h2, K);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        uint32_t rel = (b < 4) ? ((s0 >> (8 * b)) & 255u) : ((s1 >> (8 * (b - 4))) & 255u);
+        float* dst = acc + 9 * rel;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) atomicAdd(dst + c, K[b][c]);
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    float* out = vals + (size_t)k0 * 9;
+    for (int f = lane; f < nnz * 9; f += G) out[f] = acc[f];
+    if (lane == 0) {
+      float D[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+      if (drel >= 0)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) D[c] = acc[9 * drel + c];
+      float inv[9];
+      inverse_diagonal_block(D, inv);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) minv[9 * (size_t)i + c] = inv[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs: step right-hand side (STEP) or plain force evaluation (EVAL).
+// ---------------------------------------------------------------------------
+enum { MODE_EVAL = 0, MODE_STEP = 1 };
+
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restrict__ x, const float* __restrict__ v,
+                                               const float* __restrict__ act, const float* __restrict__ ext,
+                                               const float* __restrict__ vals, float* __restrict__ out,
+                                               unsigned* rhs_max, int terms, int mode, float h) {
+  constexpr int NG = BLOCK / G;
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int i = blk * NG + grp;
+  const bool active = i < a.P;
+  v3 ft = {0.f, 0.f, 0.f}, fh = {0.f, 0.f, 0.f}, corr = {0.f, 0.f, 0.f};
+  if (active) {
+    if (terms & FEM_TERM_TET) {
+      const int t1 = a.tinc_ptr[i + 1];
+      for (int t = a.tinc_ptr[i] + lane; t < t1; t += G) {
+        const int code = a.tinc[t];
+        const int e = code >> 2, la = code & 3;
+        const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+        const v3 xs[4] = {ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w)};
+        const v3 vs[4] = {ld3(v, nd.x), ld3(v, nd.y), ld3(v, nd.z), ld3(v, nd.w)};
+        const float mu = a.tet_mat[3 * e], lam = a.tet_mat[3 * e + 1], kd = a.tet_mat[3 * e + 2];
+        TetGeo g;
+        tet_geo(xs, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+        ft = ft + tet_node_force(g, vs, mu, lam, kd, la);
+        if (a.use_act && act) ft = ft + tet_activation_force(xs, lam, act[e], la);
+      }
+    }
+    if (terms & FEM_TERM_HEX) {
+      const int h1 = a.hinc_ptr[i + 1];
+      for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
+        const int code = a.hinc[t];
+        fh = fh + hex_node_force(a, x, v, code >> 3, code & 7);
+      }
+    }
+    if (mode == MODE_STEP) {
+      const v3 vi = ld3(v, i);
+      const int k1 = a.offsets[i + 1];
+      for (int k = a.offsets[i] + lane; k < k1; k += G) {
+        const float* b = vals + 9 * (size_t)k;
+        v3 d = ld3(v, a.cols[k]) - vi;
+        corr = corr + v3{b[0] * d.x + b[1] * d.y + b[2] * d.z, b[3] * d.x + b[4] * d.y + b[5] * d.z,
+                         b[6] * d.x + b[7] * d.y + b[8] * d.z};
+      }
+    }
+  }
+  ft = {group_sum<G>(ft.x), group_sum<G>(ft.y), group_sum<G>(ft.z)};
+  fh = {group_sum<G>(fh.x), group_sum<G>(fh.y), group_sum<G>(fh.z)};
+  if (mode == MODE_STEP) corr = {group_sum<G>(corr.x), group_sum<G>(corr.y), group_sum<G>(corr.z)};
+  float mval = 0.f;
+  if (active && lane == 0) {
+    const bool on = (a.flags[i] & 1) != 0;
+    const v3 xi = ld3(x, i), vi = ld3(v, i);
+    v3 fg = {0.f, 0.f, 0.f}, fgr = {0.f, 0.f, 0.f}, fdm = {0.f, 0.f, 0.f};
+    if ((terms & FEM_TERM_GROUND) && a.has_ground && on) {
+      // eval_particle_ground_contacts kernels.py:1132-1195
+      const v3 n = {a.gn[0], a.gn[1], a.gn[2]};
+      const float c = fminf(dot(n, xi) + a.gn[3] - a.radius[i], 0.f);
+      const float vn = dot(n, vi);
+      const float jn = c * a.ke;
+      if (c < 0.f) {
+        const float jd = fminf(vn, 0.f) * a.kd;
+        const float fn = jn + jd;
+        v3 vt = vi - n * vn;
+        const float vs = sqrtf(dot(vt, vt));
+        if (vs > 0.f) vt = vt * (1.f / vs);
+        const float inv_m = a.inv_mass[i];
+        const float m = inv_m > 0.f ? 1.f / fmaxf(inv_m, 1e-9f) : 0.f;
+        const float fn_eff = fminf(fabsf(fn), m * a.gmag);
+        const float ftan = fminf(vs * a.kf, a.gmu * fn_eff);
+        fg = fg - n * fn - vt * ftan;
+      }
+    }
+    if ((terms & FEM_TERM_GRAVITY) && on) {
+      const float m = a.mass[i];
+      fgr = {a.grav[0] * m, a.grav[1] * m, a.grav[2] * m};
+    }
+    if ((terms & FEM_TERM_DAMPING) && on && a.lin_damp > 0.f) fdm = vi * (-a.lin_damp * a.mass[i]);
+    v3 f;
+    if (mode == MODE_STEP) {
+      // _accumulate_forces order solver.py:570-589, then h*ext (:993-999),
+      // then subtract_elastic_predictor implicit_predictor.py:9-23.
+      f = {0.f, 0.f, 0.f};
+      f = f + ft * h;
+      f = f + fh * h;
+      f = f + fg * h;
+      f = f + fgr * h;
+      f = f + fdm * h;
+      if (ext) f = f + ld3(ext, i) * h;
+      f = f - corr;
+      if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
+      mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
+    } else {
+      f = ft + fh + fg + fgr + fdm;
+    }
+    st3(out, i, f);
+  }
+  if (mode == MODE_STEP) {
+    // maximum_rhs_component kernels.py:1779-1782, per instance.
+    if (a.row_inst == nullptr) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mval = fmaxf(mval, __shfl_xor(mval, o));
+      if ((threadIdx.x & 63) == 0 && mval > 0.f) atomicMax(rhs_max, __float_as_uint(mval));
+    } else if (active && lane == 0 && mval > 0.f) {
+      atomicMax(rhs_max + a.row_inst[i], __float_as_uint(mval));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent PCG.
+// ---------------------------------------------------------------------------
+struct CgArgs {
+  int P;
+  const int* offsets;
+  const int* cols;
+  const float* vals;
+  const float* minv;
+  const unsigned char* row_fixed;
+  const float* rhs;
+  float* dv;
+  const float* q_in;
+  const float* qd_in;
+  float* q_out;
+  float* qd_out;
+  const float* anchor;
+  float h;
+  float* zg;
+  float* pg;
+  const int* chunk_rows;
+  const int* chunk_inst;
+  const int* inst_chunks;
+  const unsigned* rhs_max;
+  float tolerance;
+  int maxiter;
+  int early_exit;
+  double* part;
+  unsigned* bar;
+  int* stats;
+  int* err;
+};
+
+// Instance-wide barrier over `n` co-resident workgroups (sense by generation).
+// Returns false after a bounded wait so that a launch that is not fully
+// resident ends with an error flag instead of hanging.
+__device__ bool inst_barrier(unsigned* bar, int n, int* err, int* lds_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (n > 1) {
+      unsigned* count = bar;
+      unsigned* gen = bar + 16;
+      const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (arrived == (unsigned)(n - 1)) {
+        __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        long spins = 0;
+        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > BARRIER_SPIN_LIMIT ||
+              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    *lds_flag = ok;
+  }
+  __syncthreads();
+  return *lds_flag != 0;
+}
+
+// Sum of N floats over the workgroup (fp32 within waves, fp64 across waves,
+// fixed order); every thread receives the result.
+template <int N>
+__device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* red) {
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[n] += __shfl_xor(v[n], o);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int n = 0; n < N; ++n) red[w * N + n] = (double)v[n];
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < N; ++n) out[n] = ((red[n] + red[N + n]) + red[2 * N + n]) + red[3 * N + n];
+  __syncthreads();
+}
+
+// Sum of part[c*4 + slot[n]] over the instance's chunks, identical in every
+// workgroup (same data, same reduction tree).
+template <int N>
+__device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, const int (&slot)[N], double (&out)[N],
+                                         double* red) {
+  double acc[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) acc[n] = 0.0;
+  for (int c = cb + (int)threadIdx.x; c < ce; c += BLOCK)
+#pragma unroll
+    for (int n = 0; n < N; ++n) acc[n] += part[4 * (size_t)c + slot[n]];
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[n] += __shfl_xor(acc[n], o);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int n = 0; n < N; ++n) red[w * N + n] = acc[n];
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < N; ++n) out[n] = ((red[n] + red[N + n]) + red[2 * N + n]) + red[3 * N + n];
+  __syncthreads();
+}
+
+// q_lds[lr] = sum_j A_ij u_j for the workgroup's rows; u = s*dv (prologue,
+// fixed columns masked) or z + beta*p (iterations).  Fixed rows give 0.
+template <int G, bool PROLOGUE>
+__device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, float s, float beta, float* q_lds) {
+  constexpr int NG = BLOCK / G;
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int npass = (nr + NG - 1) / NG;
+  for (int ps = 0; ps < npass; ++ps) {
+    const int lr = ps * NG + grp;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    if (lr < nr) {
+      const int i = R0 + lr;
+      if (!a.row_fixed[i]) {
+        const int k1 = a.offsets[i + 1];
+        for (int k = a.offsets[i] + lane; k < k1; k += G) {
+          const int col = a.cols[k];
+          const float* b = a.vals + 9 * (size_t)k;
+          v3 u;
+          if (PROLOGUE) {
+            u = a.row_fixed[col] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, col) * s;
+          } else {
+            const v3 zz = ld3(a.zg, col), pp = ld3(a.pg, col);
+            u = {fmaf(beta, pp.x, zz.x), fmaf(beta, pp.y, zz.y), fmaf(beta, pp.z, zz.z)};
+          }
+          a0 += b[0] * u.x + b[1] * u.y + b[2] * u.z;
+          a1 += b[3] * u.x + b[4] * u.y + b[5] * u.z;
+          a2 += b[6] * u.x + b[7] * u.y + b[8] * u.z;
+        }
+      }
+    }
+    a0 = group_sum<G>(a0);
+    a1 = group_sum<G>(a1);
+    a2 = group_sum<G>(a2);
+    if (lane == 0 && lr < nr) {
+      q_lds[3 * lr] = a0;
+      q_lds[3 * lr + 1] = a1;
+      q_lds[3 * lr + 2] = a2;
+    }
+  }
+}
+
+// Warp's absolute floor eps32^(9/4) on the residual norm (tol given, atol=0).
+__device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-16; }
+
+template <int G, int K>
+__global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
+  __shared__ float q_lds[K * BLOCK * 3];
+  __shared__ double red[16];
+  __shared__ int bflag;
+  constexpr bool MREG = (K <= 4);
+  const int c = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int inst = a.chunk_inst[c];
+  const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
+  const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
+  unsigned* bar = a.bar + 32 * inst;
+  const float m = __uint_as_float(a.rhs_max[inst]);
+  const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
+  const bool zero_start = (m == 0.f);
+
+  float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3], Mi[MREG ? K : 1][9];
+  bool own[K], fx[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int lr = threadIdx.x + BLOCK * k;
+    own[k] = lr < nr;
+    fx[k] = own[k] ? (a.row_fixed[R0 + lr] != 0) : true;
+  }
+
+  if (!zero_start) cg_spmv<G, true>(a, R0, nr, s, 0.f, q_lds);
+  __syncthreads();
+  float pb[3] = {0.f, 0.f, 0.f};  // b.b, r.r, r.z
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int lr = threadIdx.x + BLOCK * k;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = qv[k][d] = 0.f;
+    if (own[k]) {
+      const int i = R0 + lr;
+      const v3 b = ld3(a.rhs, i) * s;
+      const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : ld3(a.dv, i) * s;
+      const v3 q0 = zero_start ? v3{0.f, 0.f, 0.f} : v3{q_lds[3 * lr], q_lds[3 * lr + 1], q_lds[3 * lr + 2]};
+      float Mk[9];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
+      if constexpr (MREG) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) Mi[k][e] = Mk[e];
+      }
+      const v3 rr0 = b - q0;
+      const v3 zz = mv(Mk, rr0);
+      x[k][0] = x0.x; x[k][1] = x0.y; x[k][2] = x0.z;
+      r[k][0] = rr0.x; r[k][1] = rr0.y; r[k][2] = rr0.z;
+      z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
+      p[k][0] = zz.x; p[k][1] = zz.y; p[k][2] = zz.z;
+      st3(a.zg, i, zz);
+      st3(a.pg, i, zz);
+      pb[0] += dot(b, b);
+      pb[1] += dot(rr0, rr0);
+      pb[2] += dot(rr0, zz);
+    }
+  }
+  double wsum[3];
+  wg_sum<3>(pb, wsum, red);
+  if (threadIdx.x == 0) {
+    a.part[4 * (size_t)c + 3] = wsum[0];
+    a.part[4 * (size_t)c + 1] = wsum[1];
+    a.part[4 * (size_t)c + 2] = wsum[2];
+  }
+  bool ok = inst_barrier(bar, nci, a.err, &bflag);
+  double tot[3];
+  {
+    const int sl[3] = {3, 1, 2};
+    inst_sum<3>(a.part, cb, ce, sl, tot, red);
+  }
+  const double bb = tot[0];
+  double rr = tot[1], rz = tot[2];
+  double atol = fmax(0.0, (double)a.tolerance * sqrt(bb));
+  atol = fmax(atol, residual_floor());
+  const double atol_sq = atol * atol;
+  float beta = 0.f;
+  int iters = 0;
+  for (int it = 0; it < a.maxiter && ok; ++it) {
+    const bool live = rr > atol_sq;
+    if (!live && a.early_exit) break;
+    // Phase A: q = A p_new with p_new = z + beta p (gathered on the fly).
+    cg_spmv<G, false>(a, R0, nr, s, beta, q_lds);
+    __syncthreads();
+    float pp[1] = {0.f};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (own[k]) {
+        const int lr = threadIdx.x + BLOCK * k;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          p[k][d] = fmaf(beta, p[k][d], z[k][d]);
+          qv[k][d] = q_lds[3 * lr + d];
+          pp[0] += p[k][d] * qv[k][d];
+        }
+      }
+    }
+    double w1[1];
+    wg_sum<1>(pp, w1, red);
+    if (threadIdx.x == 0) a.part[4 * (size_t)c] = w1[0];
+    ok = inst_barrier(bar, nci, a.err, &bflag);
+    if (!ok) break;
+    double pAp[1];
+    {
+      const int sl[1] = {0};
+      inst_sum<1>(a.part, cb, ce, sl, pAp, red);
+    }
+    const float alpha = (live && pAp[0] != 0.0) ? (float)(rz / pAp[0]) : 0.f;
+    // Phase B: x += alpha p, r -= alpha q, z = M^-1 r.
+    float pr[2] = {0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (own[k]) {
+        const int i = R0 + threadIdx.x + BLOCK * k;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          x[k][d] = fmaf(alpha, p[k][d], x[k][d]);
+          r[k][d] = fmaf(-alpha, qv[k][d], r[k][d]);
+        }
+        float Mk[9];
+        if constexpr (MREG) {
+#pragma unroll
+          for (int e = 0; e < 9; ++e) Mk[e] = Mi[k][e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
+        }
+        const v3 rv = {r[k][0], r[k][1], r[k][2]};
+        const v3 zz = mv(Mk, rv);
+        z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
+        pr[0] += dot(rv, rv);
+        pr[1] += dot(rv, zz);
+        st3(a.zg, i, zz);
+        st3(a.pg, i, v3{p[k][0], p[k][1], p[k][2]});
+      }
+    }
+    double w2[2];
+    wg_sum<2>(pr, w2, red);
+    if (threadIdx.x == 0) {
+      a.part[4 * (size_t)c + 1] = w2[0];
+      a.part[4 * (size_t)c + 2] = w2[1];
+    }
+    ok = inst_barrier(bar, nci, a.err, &bflag);
+    if (!ok) break;
+    double t2[2];
+    {
+      const int sl[2] = {1, 2};
+      inst_sum<2>(a.part, cb, ce, sl, t2, red);
+    }
+    const double rr_new = t2[0], rz_new = t2[1];
+    beta = (rr_new > atol_sq && rz != 0.0) ? (float)(rz_new / rz) : 0.f;
+    rz = rz_new;
+    rr = rr_new;
+    if (live) ++iters;
+  }
+  // restore_linear_increment kernels.py:1808-1811, update_state :115-128,
+  // FixedParticles.restore(state_out) solver.py:1350-1351.
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (own[k]) {
+      const int i = R0 + threadIdx.x + BLOCK * k;
+      const v3 dvi = v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
+      st3(a.dv, i, dvi);
+      v3 vel = ld3(a.qd_in, i) + dvi;
+      v3 qo = ld3(a.q_in, i) + vel * a.h;
+      if (fx[k]) {
+        qo = ld3(a.anchor, i);
+        vel = v3{0.f, 0.f, 0.f};
+      }
+      st3(a.q_out, i, qo);
+      st3(a.qd_out, i, vel);
+    }
+  }
+  if (c == cb && threadIdx.x == 0) {
+    a.stats[4 * inst] = iters;
+    a.stats[4 * inst + 1] = __float_as_int((float)rr);
+    a.stats[4 * inst + 2] = __float_as_int((float)atol_sq);
+  }
+}
+
+__global__ void k_restore(const int* __restrict__ fixed, int nfixed, const float* __restrict__ anchor, float* q,
+                          float* qd) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nfixed) {
+    int i = fixed[t];
+    st3(q, i, ld3(anchor, i));
+    if (qd) st3(qd, i, v3{0.f, 0.f, 0.f});
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch dispatch over the template parameters.
+// ---------------------------------------------------------------------------
+template <template <int> class F, typename... Args>
+int dispatch_g(int g, Args&&... args) {
+  switch (g) {
+    case 4: return F<4>::run(args...);
+    case 8: return F<8>::run(args...);
+    case 16: return F<16>::run(args...);
+    case 32: return F<32>::run(args...);
+    case 64: return F<64>::run(args...);
+  }
+  return fail(FEM_ERR_ARG, "lanes per row must be 4, 8, 16, 32 or 64");
+}
+
+int padded_blocks(int rows, int rows_per_block) {
+  int n = (rows + rows_per_block - 1) / rows_per_block;
+  if (n > 8) n = (n + 7) & ~7;
+  return std::max(n, 1);
+}
+
+template <int G>
+struct LaunchAssemble {
+  static int run(const RowArgs& ra, const float* x, float h2, float* vals, float* minv, int maxnnz,
+                 hipStream_t st) {
+    size_t lds = (size_t)(BLOCK / G) * maxnnz * 9 * sizeof(float);
+    if (lds > 160 * 1024) return fail(FEM_ERR_UNSUPPORTED, "assembly LDS exceeds 160 KiB; use more lanes per row");
+    static bool attr = false;
+    if (!attr) {
+      HIP_TRY(hipFuncSetAttribute((const void*)k_assemble<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+      attr = true;
+    }
+    int nb = padded_blocks(ra.P, BLOCK / G);
+    hipLaunchKernelGGL(k_assemble<G>, dim3(nb), dim3(BLOCK), lds, st, ra, x, h2, vals, minv, maxnnz);
+    HIP_TRY(hipGetLastError());
+    return FEM_OK;
+  }
+};
+
+template <int G>
+struct LaunchRhs {
+  static int run(const RowArgs& ra, const float* x, const float* v, const float* act, const float* ext,
+                 const float* vals, float* out, unsigned* rmax, int terms, int mode, float h, hipStream_t st) {
+    int nb = padded_blocks(ra.P, BLOCK / G);
+    hipLaunchKernelGGL(k_rhs<G>, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, ext, vals, out, rmax, terms, mode, h);
+    HIP_TRY(hipGetLastError());
+    return FEM_OK;
+  }
+};
+
+const void* cg_kernel_ptr(int g, int k) {
+#define FEM_CG_CASE(G_, K_) \
+  if (g == G_ && k == K_) return (const void*)k_cg<G_, K_>;
+#define FEM_CG_ROW(G_) FEM_CG_CASE(G_, 1) FEM_CG_CASE(G_, 2) FEM_CG_CASE(G_, 4) FEM_CG_CASE(G_, 8)
+  FEM_CG_ROW(4) FEM_CG_ROW(8) FEM_CG_ROW(16) FEM_CG_ROW(32) FEM_CG_ROW(64)
+#undef FEM_CG_ROW
+#undef FEM_CG_CASE
+  return nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// System object.
+// ---------------------------------------------------------------------------
+struct fem_system {
+  int P = 0, T = 0, H = 0, nfixed = 0, ninst = 1;
+  Pattern pt;
+  std::vector<int32_t> inst_off;
+  // device copies (library-owned)
+  int *d_offsets = nullptr, *d_cols = nullptr, *d_diag = nullptr;
+  int *d_tinc_ptr = nullptr, *d_tinc = nullptr, *d_hinc_ptr = nullptr, *d_hinc = nullptr;
+  uint32_t *d_tslots = nullptr, *d_hslots = nullptr;
+  int *d_tet_idx = nullptr, *d_hex_idx = nullptr;
+  float *d_tet_pose = nullptr, *d_tet_mat = nullptr, *d_hex_invJ = nullptr, *d_hex_detw = nullptr,
+        *d_hex_mat = nullptr;
+  float *d_mass = nullptr, *d_inv_mass = nullptr, *d_radius = nullptr, *d_anchor = nullptr;
+  int* d_flags = nullptr;
+  unsigned char* d_row_fixed = nullptr;
+  int* d_fixed = nullptr;
+  int* d_row_inst = nullptr;
+  // solve scratch
+  float *d_z = nullptr, *d_p = nullptr;
+  unsigned *d_rhs_max = nullptr, *d_bar = nullptr;
+  double* d_part = nullptr;
+  int *d_stats = nullptr, *d_err = nullptr;
+  int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr;
+  int nchunks = 0, cg_k = 1;
+  // bound
+  int* b_offsets = nullptr;
+  int* b_cols = nullptr;
+  float *b_vals = nullptr, *b_minv = nullptr, *b_dv = nullptr;
+  fem_params prm{};
+  int elem_g = 16, cg_g = 16, cg_wg_request = 0;
+  int max_resident = 0;
+  std::vector<void*> allocs;
+
+  template <typename T_>
+  int upload(T_** dst, const T_* src, size_t n) {
+    *dst = nullptr;
+    if (n == 0) n = 1;
+    HIP_TRY(hipMalloc((void**)dst, n * sizeof(T_)));
+    allocs.push_back(*dst);
+    if (src) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T_), hipMemcpyHostToDevice));
+    else HIP_TRY(hipMemset(*dst, 0, n * sizeof(T_)));
+    return FEM_OK;
+  }
+  template <typename T_>
+  int alloc(T_** dst, size_t n) {
+    return upload<T_>(dst, nullptr, n);
+  }
+
+  RowArgs row_args() const {
+    RowArgs a;
+    a.P = P;
+    a.offsets = d_offsets;
+    a.cols = d_cols;
+    a.diag_rel = d_diag;
+    a.tinc_ptr = d_tinc_ptr;
+    a.tinc = d_tinc;
+    a.tslots = d_tslots;
+    a.hinc_ptr = d_hinc_ptr;
+    a.hinc = d_hinc;
+    a.hslots = d_hslots;
+    a.tet_idx = d_tet_idx;
+    a.tet_pose = d_tet_pose;
+    a.tet_mat = d_tet_mat;
+    a.hex_idx = d_hex_idx;
+    a.hex_invJ = d_hex_invJ;
+    a.hex_detw = d_hex_detw;
+    a.hex_mat = d_hex_mat;
+    a.mass = d_mass;
+    a.inv_mass = d_inv_mass;
+    a.radius = d_radius;
+    a.flags = d_flags;
+    a.row_fixed = d_row_fixed;
+    a.row_inst = ninst > 1 ? d_row_inst : nullptr;
+    a.mn = prm.min_stretch;
+    a.mx = prm.max_stretch;
+    a.has_ground = prm.has_ground;
+    for (int k = 0; k < 4; ++k) a.gn[k] = prm.ground[k];
+    a.ke = prm.ground_ke;
+    a.kd = prm.ground_kd;
+    a.kf = prm.ground_kf;
+    a.gmu = prm.ground_mu;
+    a.lin_damp = prm.linear_damping;
+    for (int k = 0; k < 3; ++k) a.grav[k] = prm.gravity[k];
+    a.gmag = sqrtf(prm.gravity[0] * prm.gravity[0] + prm.gravity[1] * prm.gravity[1] +
+                   prm.gravity[2] * prm.gravity[2]);
+    a.use_act = prm.enable_tet_activations;
+    return a;
+  }
+
+  // Row partition of the persistent CG: per instance, contiguous chunks.
+  int plan_cg() {
+    int dev = 0, ncu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long max_rows_chunk = 8L * BLOCK;
+    std::vector<int> per(ninst, 1);
+    long need = 0;
+    for (int k = 0; k < ninst; ++k) {
+      long rows = inst_off[k + 1] - inst_off[k];
+      per[k] = (int)std::max(1L, (rows + max_rows_chunk - 1) / max_rows_chunk);
+      need += per[k];
+    }
+    // Default chunk count: about 2 rows per thread for large meshes, one
+    // workgroup for small ones; cg_wg_request overrides.
+    long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, (long)P / (2 * BLOCK));
+    target = std::max(target, need);
+    // Distribute extra chunks proportionally to instance cost.
+    if (target > need) {
+      double total = 0;
+      for (int k = 0; k < ninst; ++k) total += (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]);
+      long extra = target - need;
+      for (int k = 0; k < ninst && total > 0; ++k) {
+        double share = (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]) / total;
+        long rows = inst_off[k + 1] - inst_off[k];
+        long add = (long)std::floor(share * extra);
+        per[k] = (int)std::min<long>(std::max<long>(per[k], per[k] + add), std::max<long>(rows, 1));
+      }
+    }
+    long n = 0;
+    for (int k = 0; k < ninst; ++k) n += per[k];
+    if (n > 8 && (n & 7)) {  // pad to a multiple of 8 (XCD swizzle), on the largest instance
+      int big = (int)(std::max_element(per.begin(), per.end()) - per.begin());
+      long rows = inst_off[big + 1] - inst_off[big];
+      long pad = 8 - (n & 7);
+      if (per[big] + pad <= rows) {
+        per[big] += (int)pad;
+        n += pad;
+      }
+    }
+    std::vector<int32_t> chunk_rows, chunk_inst, inst_chunks(1, 0);
+    int maxrows = 0;
+    for (int k = 0; k < ninst; ++k) {
+      std::vector<int32_t> b;
+      split_rows(pt.offsets, inst_off[k], inst_off[k + 1], per[k], b);
+      for (int j = 0; j < per[k]; ++j) {
+        chunk_rows.push_back(b[j]);
+        chunk_inst.push_back(k);
+        maxrows = std::max(maxrows, b[j + 1] - b[j]);
+      }
+      inst_chunks.push_back((int)chunk_rows.size());
+    }
+    chunk_rows.push_back(P);
+    nchunks = (int)chunk_inst.size();
+    cg_k = maxrows <= BLOCK ? 1 : maxrows <= 2 * BLOCK ? 2 : maxrows <= 4 * BLOCK ? 4 : 8;
+    if (maxrows > 8 * BLOCK) return fail(FEM_ERR_UNSUPPORTED, "CG chunk exceeds 2048 rows");
+    const void* kp = cg_kernel_ptr(cg_g, cg_k);
+    if (!kp) return fail(FEM_ERR_ARG, "unsupported CG lanes per row");
+    int nb = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, BLOCK, 0));
+    max_resident = nb * ncu;
+    if (nchunks > max_resident) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups, device holds %d", nchunks, max_resident);
+      return fail(FEM_ERR_UNSUPPORTED, buf);
+    }
+    for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part}) {
+      if (p) {
+        hipFree(p);
+        allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+      }
+    }
+    int rc;
+    if ((rc = upload(&d_chunk_rows, chunk_rows.data(), chunk_rows.size()))) return rc;
+    if ((rc = upload(&d_chunk_inst, chunk_inst.data(), chunk_inst.size()))) return rc;
+    if ((rc = upload(&d_inst_chunks, inst_chunks.data(), inst_chunks.size()))) return rc;
+    if ((rc = alloc(&d_part, 4 * (size_t)nchunks))) return rc;
+    return FEM_OK;
+  }
+
+  ~fem_system() {
+    for (void* p : allocs) hipFree(p);
+  }
+};
+
+namespace {
+int pick_lanes(double avg) {
+  int g = 4;
+  while (g < 64 && g < avg) g *= 2;
+  return g;
+}
+}  // namespace
+
+extern "C" {
+
+const char* fem_last_error(void) { return g_error.c_str(); }
+
+int fem_create(const fem_model_desc* d, fem_system** out) {
+  if (!d || !out) return fail(FEM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->particle_count < 0 || d->tet_count < 0 || d->hex_count < 0 || d->fixed_count < 0)
+    return fail(FEM_ERR_ARG, "negative count");
+  auto* s = new fem_system();
+  auto cleanup = [&](int rc) {
+    delete s;
+    return rc;
+  };
+  s->P = d->particle_count;
+  s->T = d->tet_count;
+  s->H = d->hex_count;
+  s->nfixed = d->fixed_count;
+  const int P = s->P;
+  if (d->instance_count > 1) {
+    s->ninst = d->instance_count;
+    s->inst_off.assign(d->instance_offsets, d->instance_offsets + s->ninst + 1);
+    if (s->inst_off[0] != 0 || s->inst_off[s->ninst] != P) return cleanup(fail(FEM_ERR_ARG, "instance_offsets"));
+    for (int k = 0; k < s->ninst; ++k)
+      if (s->inst_off[k + 1] <= s->inst_off[k]) return cleanup(fail(FEM_ERR_ARG, "empty or unsorted instance"));
+  } else {
+    s->ninst = 1;
+    s->inst_off = {0, P};
+  }
+  std::string err;
+  int rc = build_pattern(P, s->T, d->tet_indices, d->tet_materials, s->H, d->hex_indices, s->pt, err);
+  if (rc) return cleanup(fail(rc, err));
+  // Elements must not couple different instances.
+  if (s->ninst > 1) {
+    std::vector<int32_t> inst_of(P);
+    for (int k = 0; k < s->ninst; ++k)
+      for (int i = s->inst_off[k]; i < s->inst_off[k + 1]; ++i) inst_of[i] = k;
+    for (int i = 0; i < P; ++i)
+      for (int64_t k = s->pt.offsets[i]; k < s->pt.offsets[i + 1]; ++k)
+        if (inst_of[s->pt.columns[k]] != inst_of[i]) return cleanup(fail(FEM_ERR_ARG, "element spans two instances"));
+    if ((rc = s->upload(&s->d_row_inst, inst_of.data(), P))) return cleanup(rc);
+  }
+  std::vector<int32_t> off32(P + 1), tptr(P + 1), hptr(P + 1);
+  for (int i = 0; i <= P; ++i) {
+    off32[i] = (int32_t)s->pt.offsets[i];
+    tptr[i] = (int32_t)s->pt.tinc_ptr[i];
+    hptr[i] = (int32_t)s->pt.hinc_ptr[i];
+  }
+  std::vector<unsigned char> fixed_mask(std::max(P, 1), 0);
+  for (int k = 0; k < s->nfixed; ++k) {
+    int i = d->fixed_indices[k];
+    if (i < 0 || i >= P) return cleanup(fail(FEM_ERR_ARG, "fixed index out of range"));
+    fixed_mask[i] = 1;
+  }
+  if ((rc = s->upload(&s->d_offsets, off32.data(), off32.size())) ||
+      (rc = s->upload(&s->d_cols, s->pt.columns.data(), s->pt.columns.size())) ||
+      (rc = s->upload(&s->d_diag, s->pt.diag_rel.data(), s->pt.diag_rel.size())) ||
+      (rc = s->upload(&s->d_tinc_ptr, tptr.data(), tptr.size())) ||
+      (rc = s->upload(&s->d_tinc, s->pt.tinc.data(), s->pt.tinc.size())) ||
+      (rc = s->upload(&s->d_tslots, s->pt.tslots.data(), s->pt.tslots.size())) ||
+      (rc = s->upload(&s->d_hinc_ptr, hptr.data(), hptr.size())) ||
+      (rc = s->upload(&s->d_hinc, s->pt.hinc.data(), s->pt.hinc.size())) ||
+      (rc = s->upload(&s->d_hslots, s->pt.hslots.data(), s->pt.hslots.size())) ||
+      (rc = s->upload(&s->d_tet_idx, d->tet_indices, 4 * (size_t)s->T)) ||
+      (rc = s->upload(&s->d_tet_pose, d->tet_poses, 9 * (size_t)s->T)) ||
+      (rc = s->upload(&s->d_tet_mat, d->tet_materials, 3 * (size_t)s->T)) ||
+      (rc = s->upload(&s->d_hex_idx, d->hex_indices, 8 * (size_t)s->H)) ||
+      (rc = s->upload(&s->d_hex_invJ, d->hex_inv_J0, 72 * (size_t)s->H)) ||
+      (rc = s->upload(&s->d_hex_detw, d->hex_det_J0_w, 8 * (size_t)s->H)) ||
+      (rc = s->upload(&s->d_hex_mat, d->hex_materials, 3 * (size_t)s->H)) ||
+      (rc = s->upload(&s->d_mass, d->particle_mass, P)) ||
+      (rc = s->upload(&s->d_inv_mass, d->particle_inv_mass, P)) ||
+      (rc = s->upload(&s->d_radius, d->particle_radius, P)) ||
+      (rc = s->upload(&s->d_flags, d->particle_flags, P)) ||
+      (rc = s->upload(&s->d_anchor, d->particle_q, 3 * (size_t)P)) ||
+      (rc = s->upload(&s->d_row_fixed, fixed_mask.data(), fixed_mask.size())) ||
+      (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
+      (rc = s->alloc(&s->d_z, 3 * (size_t)P)) || (rc = s->alloc(&s->d_p, 3 * (size_t)P)) ||
+      (rc = s->alloc(&s->d_rhs_max, s->ninst)) || (rc = s->alloc(&s->d_bar, 32 * (size_t)s->ninst)) ||
+      (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)))
+    return cleanup(rc);
+  double avg_inc = P ? (double)(s->pt.tinc_ptr[P] + s->pt.hinc_ptr[P]) / P : 1.0;
+  double avg_nnz = P ? (double)s->pt.offsets[P] / P : 1.0;
+  s->elem_g = std::min(16, pick_lanes(avg_inc));
+  s->cg_g = std::min(32, pick_lanes(avg_nnz));
+  while ((size_t)(BLOCK / s->elem_g) * s->pt.max_row_nnz * 36 > 64 * 1024 && s->elem_g < 64) s->elem_g *= 2;
+  s->prm.min_stretch = 0.05f;
+  s->prm.max_stretch = 20.f;
+  s->prm.tolerance = 1e-2f;
+  s->prm.maxiter = 50;
+  s->prm.early_exit = 1;
+  s->prm.gravity[2] = -9.81f;
+  if (P > 0 && (rc = s->plan_cg())) return cleanup(rc);
+  *out = s;
+  return FEM_OK;
+}
+
+int fem_destroy(fem_system* s) {
+  delete s;
+  return FEM_OK;
+}
+
+int fem_pattern_size(const fem_system* s, int64_t* nnz, int32_t* max_row_blocks) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (nnz) *nnz = s->pt.offsets[s->P];
+  if (max_row_blocks) *max_row_blocks = s->pt.max_row_nnz;
+  return FEM_OK;
+}
+
+int fem_bind(fem_system* s, int32_t* offsets, int32_t* columns, float* values, float* inv_diag, float* increment) {
+  if (!s || !offsets || !columns || !values || !inv_diag || !increment) return fail(FEM_ERR_ARG, "null buffer");
+  s->b_offsets = offsets;
+  s->b_cols = columns;
+  s->b_vals = values;
+  s->b_minv = inv_diag;
+  s->b_dv = increment;
+  std::vector<int32_t> off32(s->P + 1);
+  for (int i = 0; i <= s->P; ++i) off32[i] = (int32_t)s->pt.offsets[i];
+  HIP_TRY(hipMemcpy(offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+  if (!s->pt.columns.empty())
+    HIP_TRY(hipMemcpy(columns, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
+  return FEM_OK;
+}
+
+int fem_set_params(fem_system* s, const fem_params* p) {
+  if (!s || !p) return fail(FEM_ERR_ARG, "null argument");
+  if (!(p->tolerance > 0.f) || !std::isfinite(p->tolerance)) return fail(FEM_ERR_ARG, "tolerance must be finite and positive");
+  if (p->maxiter < 0) return fail(FEM_ERR_ARG, "maxiter must be nonnegative");
+  s->prm = *p;
+  return FEM_OK;
+}
+
+int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t cg_lanes, int32_t cg_workgroups) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  auto ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
+  if (elem_lanes && !ok(elem_lanes)) return fail(FEM_ERR_ARG, "elem_lanes must be a power of two in [4, 64]");
+  if (cg_lanes && !ok(cg_lanes)) return fail(FEM_ERR_ARG, "cg_lanes must be a power of two in [4, 64]");
+  if (cg_workgroups < 0) return fail(FEM_ERR_ARG, "cg_workgroups must be >= 0");
+  if (elem_lanes) s->elem_g = elem_lanes;
+  if (cg_lanes) s->cg_g = cg_lanes;
+  s->cg_wg_request = cg_workgroups;
+  return s->P > 0 ? s->plan_cg() : FEM_OK;
+}
+
+int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* cg_lanes, int32_t* cg_workgroups,
+                   int32_t* cg_rows_per_thread) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (elem_lanes) *elem_lanes = s->elem_g;
+  if (cg_lanes) *cg_lanes = s->cg_g;
+  if (cg_workgroups) *cg_workgroups = s->nchunks;
+  if (cg_rows_per_thread) *cg_rows_per_thread = s->cg_k;
+  return FEM_OK;
+}
+
+int fem_restore_fixed(fem_system* s, float* q, float* qd, void* stream) {
+  if (!s || !q) return fail(FEM_ERR_ARG, "null argument");
+  if (s->nfixed == 0) return FEM_OK;
+  int nb = (s->nfixed + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_restore, dim3(nb), dim3(BLOCK), 0, (hipStream_t)stream, s->d_fixed, s->nfixed, s->d_anchor,
+                     q, qd);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
+  if (!s || !q) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  RowArgs ra = s->row_args();
+  return dispatch_g<LaunchAssemble>(s->elem_g, ra, q, dt * dt, s->b_vals, s->b_minv, std::max(s->pt.max_row_nnz, 1),
+                                    (hipStream_t)stream);
+}
+
+int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* qd, const float* act, float* out,
+                    void* stream) {
+  if (!s || !q || !qd || !out) return fail(FEM_ERR_ARG, "null argument");
+  if (s->P == 0) return FEM_OK;
+  RowArgs ra = s->row_args();
+  return dispatch_g<LaunchRhs>(s->elem_g, ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
+                               s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
+}
+
+int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float* ext, const float* act, float* q_out,
+             float* qd_out, float dt, void* stream) {
+  if (!s || !q_in || !qd_in || !f_out || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  {
+    int n = std::max(std::max(s->nfixed, s->ninst), 1);
+    hipLaunchKernelGGL(k_prepare, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_fixed, s->nfixed,
+                       s->d_anchor, q_in, qd_in, s->d_rhs_max, s->d_bar, s->ninst, s->d_err);
+    HIP_TRY(hipGetLastError());
+  }
+  RowArgs ra = s->row_args();
+  const int terms = FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
+  int rc = dispatch_g<LaunchRhs>(s->elem_g, ra, (const float*)q_in, (const float*)qd_in, act, ext,
+                                 (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
+  if (rc) return rc;
+  CgArgs ca;
+  ca.P = s->P;
+  ca.offsets = s->d_offsets;
+  ca.cols = s->d_cols;
+  ca.vals = s->b_vals;
+  ca.minv = s->b_minv;
+  ca.row_fixed = s->d_row_fixed;
+  ca.rhs = f_out;
+  ca.dv = s->b_dv;
+  ca.q_in = q_in;
+  ca.qd_in = qd_in;
+  ca.q_out = q_out;
+  ca.qd_out = qd_out;
+  ca.anchor = s->d_anchor;
+  ca.h = dt;
+  ca.zg = s->d_z;
+  ca.pg = s->d_p;
+  ca.chunk_rows = s->d_chunk_rows;
+  ca.chunk_inst = s->d_chunk_inst;
+  ca.inst_chunks = s->d_inst_chunks;
+  ca.rhs_max = s->d_rhs_max;
+  ca.tolerance = s->prm.tolerance;
+  ca.maxiter = s->prm.maxiter;
+  ca.early_exit = s->prm.early_exit;
+  ca.part = s->d_part;
+  ca.bar = s->d_bar;
+  ca.stats = s->d_stats;
+  ca.err = s->d_err;
+  const void* kp = cg_kernel_ptr(s->cg_g, s->cg_k);
+  if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
+  void* args[] = {&ca};
+  HIP_TRY(hipLaunchKernel(kp, dim3(s->nchunks), dim3(BLOCK), args, 0, st));
+  return FEM_OK;
+}
+
+int fem_solve_stats(fem_system* s, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,
+                    int32_t* error_flags) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  std::vector<int32_t> st(4 * (size_t)s->ninst);
+  int err = 0;
+  HIP_TRY(hipMemcpy(st.data(), s->d_stats, st.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&err, s->d_err, 4, hipMemcpyDeviceToHost));
+  for (int k = 0; k < s->ninst; ++k) {
+    if (iterations) iterations[k] = st[4 * k];
+    if (residual_sq) std::memcpy(&residual_sq[k], &st[4 * k + 1], 4);
+    if (atol_sq) std::memcpy(&atol_sq[k], &st[4 * k + 2], 4);
+    if (error_flags) error_flags[k] = err;
+  }
+  return FEM_OK;
+}
+
+}  // extern "C"

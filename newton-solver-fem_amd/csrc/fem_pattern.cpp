@@ -1,0 +1,167 @@
+// Host-side construction of the BSR pattern of A = M + h^2 K and of the
+// per-node element incidence lists used by the row-centric HIP kernels.
+//
+// The pattern equals what the reference obtains from bsr_set_from_triplets
+// over the triplet layout [P diagonal | 16 per Tet4 | 64 per Hex8]
+// (solver.py:312-341, kernels.py:141-153, :635-850, :1609-1747): per row the
+// sorted unique column indices of every valid triplet.  Tets whose three
+// material coefficients are all zero write row/column -1 and are excluded
+// (kernels.py:679-692); every Hex8 and every particle diagonal is present.
+//
+// Incidences list, for each node i, the (element, local corner) pairs with
+// n_a == i, in ascending element order.  For each incidence the relative
+// slot (position inside row i) of every block K_ab is precomputed, replacing
+// the triplet->slot binary search of sparse_update.map_triplet_blocks
+// (sparse_update.py:9-30).
+#include "fem_pattern.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace fem {
+
+static bool tet_valid(const float* mat) { return !(mat[0] == 0.f && mat[1] == 0.f && mat[2] == 0.f); }
+
+int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx,
+                  Pattern& pt, std::string& err) {
+  pt = Pattern();
+  pt.P = P;
+  // Incidence counts.
+  pt.tinc_ptr.assign(P + 1, 0);
+  pt.hinc_ptr.assign(P + 1, 0);
+  for (int e = 0; e < T; ++e) {
+    if (!tet_valid(tet_mat + 3 * (size_t)e)) continue;
+    for (int a = 0; a < 4; ++a) {
+      int n = tet_idx[4 * (size_t)e + a];
+      if (n < 0 || n >= P) {
+        err = "tet_indices contains an out-of-range node";
+        return 1;
+      }
+      pt.tinc_ptr[n + 1]++;
+    }
+  }
+  for (int e = 0; e < H; ++e)
+    for (int a = 0; a < 8; ++a) {
+      int n = hex_idx[8 * (size_t)e + a];
+      if (n < 0 || n >= P) {
+        err = "hex_indices contains an out-of-range node";
+        return 1;
+      }
+      pt.hinc_ptr[n + 1]++;
+    }
+  for (int i = 0; i < P; ++i) {
+    pt.tinc_ptr[i + 1] += pt.tinc_ptr[i];
+    pt.hinc_ptr[i + 1] += pt.hinc_ptr[i];
+  }
+  pt.tinc.resize(pt.tinc_ptr[P]);
+  pt.hinc.resize(pt.hinc_ptr[P]);
+  {
+    std::vector<int64_t> fill(pt.tinc_ptr.begin(), pt.tinc_ptr.end() - 1);
+    for (int e = 0; e < T; ++e) {
+      if (!tet_valid(tet_mat + 3 * (size_t)e)) continue;
+      for (int a = 0; a < 4; ++a) pt.tinc[fill[tet_idx[4 * (size_t)e + a]]++] = (int32_t)(((uint32_t)e << 2) | a);
+    }
+    std::vector<int64_t> hfill(pt.hinc_ptr.begin(), pt.hinc_ptr.end() - 1);
+    for (int e = 0; e < H; ++e)
+      for (int a = 0; a < 8; ++a) pt.hinc[hfill[hex_idx[8 * (size_t)e + a]]++] = (int32_t)(((uint32_t)e << 3) | a);
+  }
+  if (T >= (1 << 29) || H >= (1 << 28)) {
+    err = "element count exceeds the packed incidence limit";
+    return 3;
+  }
+  // Rows: sorted unique columns.
+  pt.offsets.assign(P + 1, 0);
+  pt.columns.clear();
+  pt.columns.reserve((size_t)pt.tinc_ptr[P] * 3 + (size_t)pt.hinc_ptr[P] * 7 + P);
+  std::vector<int32_t> cand;
+  pt.diag_rel.assign(P, -1);
+  int maxnnz = 0;
+  for (int i = 0; i < P; ++i) {
+    cand.clear();
+    cand.push_back(i);
+    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
+      int e = pt.tinc[k] >> 2;
+      for (int b = 0; b < 4; ++b) cand.push_back(tet_idx[4 * (size_t)e + b]);
+    }
+    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
+      int e = pt.hinc[k] >> 3;
+      for (int b = 0; b < 8; ++b) cand.push_back(hex_idx[8 * (size_t)e + b]);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    int nnz = (int)cand.size();
+    maxnnz = std::max(maxnnz, nnz);
+    if (nnz > 255) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "row %d has %d blocks; the assembly kernels support at most 255", i, nnz);
+      err = buf;
+      return 3;
+    }
+    pt.diag_rel[i] = (int32_t)(std::lower_bound(cand.begin(), cand.end(), i) - cand.begin());
+    pt.columns.insert(pt.columns.end(), cand.begin(), cand.end());
+    pt.offsets[i + 1] = (int64_t)pt.columns.size();
+    if (pt.offsets[i + 1] >= ((int64_t)1 << 31)) {
+      err = "more than 2^31 blocks";
+      return 3;
+    }
+  }
+  pt.max_row_nnz = maxnnz;
+  // Relative slots of every incidence's blocks.
+  pt.tslots.resize(pt.tinc.size());
+  for (int i = 0; i < P; ++i) {
+    const int32_t* row = pt.columns.data() + pt.offsets[i];
+    int nnz = (int)(pt.offsets[i + 1] - pt.offsets[i]);
+    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
+      int e = pt.tinc[k] >> 2;
+      uint32_t packed = 0;
+      for (int b = 0; b < 4; ++b) {
+        int c = tet_idx[4 * (size_t)e + b];
+        uint32_t rel = (uint32_t)(std::lower_bound(row, row + nnz, c) - row);
+        packed |= rel << (8 * b);
+      }
+      pt.tslots[k] = packed;
+    }
+  }
+  pt.hslots.resize(2 * pt.hinc.size());
+  for (int i = 0; i < P; ++i) {
+    const int32_t* row = pt.columns.data() + pt.offsets[i];
+    int nnz = (int)(pt.offsets[i + 1] - pt.offsets[i]);
+    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
+      int e = pt.hinc[k] >> 3;
+      uint32_t lo = 0, hi = 0;
+      for (int b = 0; b < 8; ++b) {
+        int c = hex_idx[8 * (size_t)e + b];
+        uint32_t rel = (uint32_t)(std::lower_bound(row, row + nnz, c) - row);
+        if (b < 4) lo |= rel << (8 * b);
+        else hi |= rel << (8 * (b - 4));
+      }
+      pt.hslots[2 * k] = lo;
+      pt.hslots[2 * k + 1] = hi;
+    }
+  }
+  return 0;
+}
+
+// Balanced contiguous row ranges: `nchunks` pieces of [r0, r1) with roughly
+// equal cost = blocks + 8 * rows.
+void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds) {
+  bounds.clear();
+  bounds.push_back(r0);
+  auto cost = [&](int r) { return (double)(offsets[r] - offsets[r0]) + 8.0 * (r - r0); };
+  double total = cost(r1);
+  int r = r0;
+  for (int c = 1; c < nchunks; ++c) {
+    double target = total * c / nchunks;
+    int lo = r, hi = r1;
+    while (lo < hi) {
+      int mid = lo + (hi - lo) / 2;
+      if (cost(mid) < target) lo = mid + 1;
+      else hi = mid;
+    }
+    r = lo;
+    bounds.push_back(r);
+  }
+  bounds.push_back(r1);
+}
+
+}  // namespace fem

@@ -1,0 +1,29 @@
+// Host-side BSR pattern and element incidence construction.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fem {
+
+struct Pattern {
+  int P = 0;
+  std::vector<int64_t> offsets;   // [P+1]
+  std::vector<int32_t> columns;   // [nnz]
+  std::vector<int32_t> diag_rel;  // [P] position of (i,i) inside row i
+  std::vector<int64_t> tinc_ptr;  // [P+1]
+  std::vector<int32_t> tinc;      // (element << 2) | corner
+  std::vector<uint32_t> tslots;   // 4 relative slots, 8 bits each
+  std::vector<int64_t> hinc_ptr;  // [P+1]
+  std::vector<int32_t> hinc;      // (element << 3) | corner
+  std::vector<uint32_t> hslots;   // 8 relative slots, two words per incidence
+  int max_row_nnz = 0;
+};
+
+int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx,
+                  Pattern& pt, std::string& err);
+
+void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds);
+
+}  // namespace fem

@@ -1,0 +1,43 @@
+"""MI355X-native SolverFEM substep: hand-written HIP kernels behind a C-ABI.
+
+Public names mirror the reference package (``newton_solver_fem``):
+``SolverFEM`` (alias ``SolverLinearFEM``), ``FEMBuilder`` and
+``LinearFEMData``; the Newton-side containers it needs (``ModelBuilder``,
+``Model``, ``State``, ``Control``) are provided here.
+"""
+
+from .model import (
+    PARTICLE_FLAG_ACTIVE,
+    Control,
+    FEMBuilder,
+    LinearFEMData,
+    Model,
+    ModelBuilder,
+    State,
+    build_hex_model,
+    build_tet_model,
+    hex_reference_data,
+)
+
+__all__ = [
+    "PARTICLE_FLAG_ACTIVE",
+    "Control",
+    "FEMBuilder",
+    "LinearFEMData",
+    "Model",
+    "ModelBuilder",
+    "State",
+    "SolverFEM",
+    "SolverLinearFEM",
+    "build_hex_model",
+    "build_tet_model",
+    "hex_reference_data",
+]
+
+
+def __getattr__(name):
+    if name in ("SolverFEM", "SolverLinearFEM"):
+        from .solver import SolverFEM
+
+        return SolverFEM
+    raise AttributeError(name)

@@ -1,0 +1,317 @@
+"""Minimal model/state containers and builders over torch device tensors.
+
+The reference builds its models with Newton 1.7's ``ModelBuilder`` plus the
+package's ``FEMBuilder`` (``src/newton_solver_fem/_src/solvers/fem/builder.py``).
+Newton is not a dependency here, so this module supplies the subset of that
+surface the FEM substep consumes (``SURVEY`` boundary list): particles with
+mass/inverse mass/radius/flags, Tet4 elements with inverted rest matrices
+(``tet_poses``) and (mu, lambda, k_damp) materials, Hex8 reference quadrature
+data under ``model.fem``, gravity, and ``state()`` / ``control()`` factories.
+
+Host copies of every array are kept on the model (``model.host``) because the
+sparsity pattern and element incidence lists are built on the host once.
+"""
+
+from __future__ import annotations
+
+import warnings
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+PARTICLE_FLAG_ACTIVE = 1
+
+# Hex8 corner / Gauss-point sign table (reference builder.py:104-107).
+HEX_SIGNS = np.asarray(
+    [[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]],
+    dtype=np.float64,
+)
+
+
+def hex_reference_data(rest: np.ndarray, hexes: np.ndarray):
+    """Per hex and Gauss point: ``inv(J0)`` and ``|det J0|`` in FP64.
+
+    Restates ``FEMBuilder.finalize`` (reference builder.py:95-135):
+    ``J0 = sum_a x_a (outer) grad_xi N_a(xi_g)`` at the 2x2x2 Gauss points
+    ``signs/sqrt(3)`` with unit Gauss weights; raises if ``det J0 <= 1e-12``.
+    """
+    rest = np.asarray(rest, dtype=np.float64)
+    hexes = np.asarray(hexes, dtype=np.int64).reshape(-1, 8)
+    gp = HEX_SIGNS / np.sqrt(3.0)  # (8 points, 3)
+    s = HEX_SIGNS  # (8 corners, 3)
+    xi, eta, zeta = gp[:, 0:1], gp[:, 1:2], gp[:, 2:3]  # (8,1) each
+    xa, ya, za = s[None, :, 0], s[None, :, 1], s[None, :, 2]  # (1,8)
+    grad = np.stack(
+        [
+            xa * (1 + ya * eta) * (1 + za * zeta),
+            (1 + xa * xi) * ya * (1 + za * zeta),
+            (1 + xa * xi) * (1 + ya * eta) * za,
+        ],
+        axis=-1,
+    ) * 0.125  # (8 points, 8 corners, 3)
+    pts = rest[hexes]  # (H, 8 corners, 3)
+    jac = np.einsum("hai,gaj->hgij", pts, grad)  # (H, 8 points, 3, 3)
+    det = np.linalg.det(jac)
+    bad = np.argwhere(det <= 1.0e-12)
+    if len(bad):
+        e, g = bad[0]
+        raise ValueError(f"Degenerate or inverted Hex8 {e}, Gauss point {g}: {det[e, g]}")
+    return np.linalg.inv(jac), np.abs(det)
+
+
+@dataclass
+class LinearFEMData:
+    """Hex8 reference geometry and materials stored under ``model.fem``."""
+
+    hex_count: int
+    hex_indices: torch.Tensor
+    hex_materials: torch.Tensor
+    hex_inv_J0: torch.Tensor
+    hex_det_J0_w: torch.Tensor
+
+
+class State:
+    """Particle positions, velocities and the force workspace."""
+
+    def __init__(self, particle_q: torch.Tensor, particle_qd: torch.Tensor, particle_f: torch.Tensor):
+        self.particle_q = particle_q
+        self.particle_qd = particle_qd
+        self.particle_f = particle_f
+
+    def clear_forces(self):
+        self.particle_f.zero_()
+
+
+class Control:
+    def __init__(self, tet_activations: torch.Tensor):
+        self.tet_activations = tet_activations
+
+
+class Model:
+    """Finalized arrays on one torch device (``cuda`` for the HIP solver)."""
+
+    def __init__(self, device, host: dict, gravity):
+        self.device = torch.device(device)
+        self.host = host
+        self.gravity = tuple(float(g) for g in gravity)
+        self.particle_count = int(len(host["particle_q"]))
+        self.tet_count = int(len(host["tet_indices"]))
+        self.spring_count = 0
+        self.tri_count = 0
+        dev = self.device
+
+        def t(a, dtype):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+        self.particle_q = t(host["particle_q"], torch.float32).reshape(-1, 3)
+        self.particle_qd = t(host["particle_qd"], torch.float32).reshape(-1, 3)
+        self.particle_mass = t(host["particle_mass"], torch.float32)
+        self.particle_inv_mass = t(host["particle_inv_mass"], torch.float32)
+        self.particle_radius = t(host["particle_radius"], torch.float32)
+        self.particle_flags = t(host["particle_flags"], torch.int32)
+        self.tet_indices = t(host["tet_indices"], torch.int32).reshape(-1, 4)
+        self.tet_poses = t(host["tet_poses"], torch.float32).reshape(-1, 3, 3)
+        self.tet_materials = t(host["tet_materials"], torch.float32).reshape(-1, 3)
+        self.tet_activations = t(host["tet_activations"], torch.float32)
+        h = host["hex_indices"]
+        self.fem = LinearFEMData(
+            len(h),
+            t(h, torch.int32).reshape(-1, 8),
+            t(host["hex_materials"], torch.float32).reshape(-1, 3),
+            t(host["hex_inv_J0"], torch.float32).reshape(-1, 8, 3, 3),
+            t(host["hex_det_J0_w"], torch.float32).reshape(-1, 8),
+        )
+
+    def set_gravity(self, gravity):
+        self.gravity = tuple(float(g) for g in gravity)
+
+    def state(self, requires_grad: bool = False) -> State:
+        q = self.particle_q.clone()
+        qd = self.particle_qd.clone()
+        f = torch.zeros_like(q)
+        if requires_grad:
+            q.requires_grad_(True)
+            qd.requires_grad_(True)
+        return State(q, qd, f)
+
+    def control(self) -> Control:
+        return Control(self.tet_activations.clone())
+
+
+class ModelBuilder:
+    """Collects particles and Tet4 elements (Newton ``ModelBuilder`` subset)."""
+
+    default_particle_radius = 0.1
+
+    def __init__(self, gravity=-9.81, up_axis: int = 2):
+        if np.ndim(gravity) == 0:
+            g = [0.0, 0.0, 0.0]
+            g[up_axis] = float(gravity)
+            gravity = g
+        self.gravity = tuple(float(x) for x in gravity)
+        self._q, self._qd, self._m, self._r, self._flags = [], [], [], [], []
+        self._tet_idx, self._tet_pose, self._tet_mat, self._tet_act = [], [], [], []
+
+    @property
+    def particle_count(self) -> int:
+        return len(self._m)
+
+    @property
+    def particle_q(self):
+        return list(self._q)
+
+    @property
+    def tet_count(self) -> int:
+        return sum(len(a) for a in self._tet_idx)
+
+    def add_particle(self, pos, vel=(0.0, 0.0, 0.0), mass=1.0, radius=None, flags=PARTICLE_FLAG_ACTIVE) -> int:
+        self._q.append(np.asarray(pos, dtype=np.float64).reshape(3))
+        self._qd.append(np.asarray(vel, dtype=np.float64).reshape(3))
+        self._m.append(float(mass))
+        self._r.append(self.default_particle_radius if radius is None else float(radius))
+        self._flags.append(int(flags))
+        return len(self._m) - 1
+
+    def add_particles(self, pos, vel, mass, radius=None):
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.asarray(vel, dtype=np.float64).reshape(-1, 3)
+        mass = np.broadcast_to(np.asarray(mass, dtype=np.float64), (len(pos),))
+        rad = np.broadcast_to(
+            np.asarray(self.default_particle_radius if radius is None else radius, dtype=np.float64), (len(pos),)
+        )
+        self._q.extend(pos)
+        self._qd.extend(vel)
+        self._m.extend(mass.tolist())
+        self._r.extend(rad.tolist())
+        self._flags.extend([PARTICLE_FLAG_ACTIVE] * len(pos))
+
+    def add_tetrahedron(self, i, j, k, l, k_mu=1.0e3, k_lambda=1.0e3, k_damp=0.0) -> float:
+        """Append one tet; its pose is ``inv([x1-x0, x2-x0, x3-x0])`` in FP64.
+
+        Like Newton's builder, a non-positive rest volume skips the element.
+        """
+        return float(self.add_tetrahedra(np.array([[i, j, k, l]]), [[k_mu, k_lambda, k_damp]])[0])
+
+    def add_tetrahedra(self, indices, materials, activations=None) -> np.ndarray:
+        """Vectorized ``add_tetrahedron``; returns the rest volumes."""
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1, 4)
+        mats = np.broadcast_to(np.asarray(materials, dtype=np.float64), (len(idx), 3))
+        pts = np.asarray(self._q, dtype=np.float64)
+        dm = np.transpose(pts[idx[:, 1:]] - pts[idx[:, :1]], (0, 2, 1))  # columns = edges
+        det = np.linalg.det(dm)
+        volume = det / 6.0
+        ok = volume > 0.0
+        if not ok.all():
+            warnings.warn(f"skipping {int((~ok).sum())} inverted tetrahedral element(s)")
+        poses = np.linalg.inv(dm[ok])
+        self._tet_idx.append(idx[ok].astype(np.int32))
+        self._tet_pose.append(poses)
+        self._tet_mat.append(np.array(mats[ok]))
+        act = np.zeros(int(ok.sum())) if activations is None else np.asarray(activations, dtype=np.float64)[ok]
+        self._tet_act.append(act)
+        return volume
+
+    def _host_arrays(self) -> dict:
+        p = len(self._m)
+        mass = np.asarray(self._m, dtype=np.float64)
+        inv_mass = np.where(mass > 0.0, 1.0 / np.where(mass > 0.0, mass, 1.0), 0.0)
+        cat = (lambda lst, shape, dt: np.concatenate(lst).astype(dt) if lst else np.zeros(shape, dt))
+        return {
+            "particle_q": np.asarray(self._q, dtype=np.float32).reshape(p, 3),
+            "particle_qd": np.asarray(self._qd, dtype=np.float32).reshape(p, 3),
+            "particle_mass": mass.astype(np.float32),
+            "particle_inv_mass": inv_mass.astype(np.float32),
+            "particle_radius": np.asarray(self._r, dtype=np.float32),
+            "particle_flags": np.asarray(self._flags, dtype=np.int32),
+            "tet_indices": cat(self._tet_idx, (0, 4), np.int32),
+            "tet_poses": cat(self._tet_pose, (0, 3, 3), np.float32),
+            "tet_materials": cat(self._tet_mat, (0, 3), np.float32),
+            "tet_activations": cat(self._tet_act, (0,), np.float32),
+            "hex_indices": np.zeros((0, 8), np.int32),
+            "hex_materials": np.zeros((0, 3), np.float32),
+            "hex_inv_J0": np.zeros((0, 8, 3, 3), np.float32),
+            "hex_det_J0_w": np.zeros((0, 8), np.float32),
+        }
+
+    def finalize(self, device="cuda", requires_grad: bool = False) -> Model:
+        return Model(device, self._host_arrays(), self.gravity)
+
+
+class FEMBuilder(ModelBuilder):
+    """Adds Hex8 elements with FP64 reference quadrature (reference builder.py:21-135)."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.fem_hex_indices = []
+        self.fem_hex_materials = []
+
+    def add_soft_hex_mesh(self, hex_indices, k_mu=1.0e3, k_lambda=1.0e3, k_damp=0.0):
+        indices = np.asarray(hex_indices, dtype=np.int32)
+        if indices.ndim != 2 or indices.shape[1] != 8:
+            raise ValueError("hex_indices must have shape (H, 8)")
+        if indices.size and (indices.min() < 0 or indices.max() >= self.particle_count):
+            raise ValueError("Hex8 indices must reference existing builder particles")
+        materials = np.column_stack(
+            [np.broadcast_to(np.asarray(v, dtype=np.float32), (len(indices),)) for v in (k_mu, k_lambda, k_damp)]
+        )
+        if not np.isfinite(materials).all() or np.any(materials < 0):
+            raise ValueError("Hex8 materials must be finite and nonnegative")
+        self.fem_hex_indices.append(indices)
+        self.fem_hex_materials.append(materials)
+
+    def finalize(self, device="cuda", requires_grad: bool = False) -> Model:
+        host = self._host_arrays()
+        if self.fem_hex_indices:
+            hexes = np.concatenate(self.fem_hex_indices).reshape(-1, 8)
+            inv, det = hex_reference_data(np.asarray(self._q, dtype=np.float64), hexes)
+            host["hex_indices"] = hexes.astype(np.int32)
+            host["hex_materials"] = np.concatenate(self.fem_hex_materials).astype(np.float32).reshape(-1, 3)
+            host["hex_inv_J0"] = inv.astype(np.float32)
+            host["hex_det_J0_w"] = det.astype(np.float32)
+        return Model(device, host, self.gravity)
+
+
+def build_tet_model(scene, device="cuda", gravity=(0.0, 0.0, -9.81)) -> Model:
+    """Model from a :class:`meshes.TetScene` (particles then tets, no springs)."""
+    b = ModelBuilder(gravity=gravity)
+    n = len(scene.points)
+    b.add_particles(scene.points, np.zeros((n, 3)), scene.masses, radius=scene.radius)
+    b.add_tetrahedra(scene.tets, scene.materials, activations=scene.activations)
+    return b.finalize(device)
+
+
+def build_hex_model(scene, device="cuda", gravity=(0.0, 0.0, -9.81)) -> Model:
+    """Model from a :class:`meshes.HexScene`; masses ``rho * sum_g w_g N_a``."""
+    b = FEMBuilder(gravity=gravity)
+    pts = np.asarray(scene.points, dtype=np.float64)
+    hexes = np.asarray(scene.hexes, dtype=np.int64)
+    _, det = hex_reference_data(pts, hexes)
+    gp = HEX_SIGNS / np.sqrt(3.0)
+    shape = np.prod(1.0 + gp[:, None, :] * HEX_SIGNS[None, :, :], axis=2) / 8.0  # (8 points, 8 corners)
+    masses = np.zeros(len(pts))
+    np.add.at(masses, hexes.ravel(), (scene.density * det @ shape).ravel())
+    b.add_particles(pts, np.zeros_like(pts), masses, radius=scene.radius)
+    b.add_soft_hex_mesh(hexes, *scene.materials[0])
+    return b.finalize(device)
+
+
+def merge_tet_scenes(scenes):
+    """Concatenate independent Tet4 scenes; returns (scene, particle offsets)."""
+    from .meshes import TetScene
+
+    offs = [0]
+    pts, tets, mats, masses, fixed = [], [], [], [], []
+    for s in scenes:
+        base = offs[-1]
+        pts.append(s.points)
+        tets.append(s.tets + base)
+        mats.append(s.materials)
+        masses.append(s.masses)
+        fixed.append(np.asarray(s.fixed, dtype=np.int64) + base)
+        offs.append(base + len(s.points))
+    merged = TetScene(
+        np.concatenate(pts), np.concatenate(tets).astype(np.int32), np.concatenate(mats), np.concatenate(masses),
+        fixed=np.concatenate(fixed).astype(np.int32), radius=scenes[0].radius,
+    )
+    return merged, np.asarray(offs, dtype=np.int64)

@@ -1,0 +1,488 @@
+"""``SolverFEM``: the linearly implicit Tet4/Hex8 substep on MI355X.
+
+Host-side mirror of the reference's ``SolverFEM``
+(src/newton_solver_fem/_src/solvers/fem/solver.py:132-1701) for its
+substep path: same constructor arguments, ``step`` / ``refresh_tangent``
+signatures, side effects and error behaviour.  All numerics run in
+``libfem_hip.so`` (hand-written HIP for gfx950) through the C-ABI in
+``include/fem_hip.h``; this module only validates arguments, owns the
+persistent device buffers and forwards raw pointers plus the current
+torch stream.  There is no CPU or torch fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import Control, Model, State
+
+SUPPORTED_SOLVER = "cg"
+SUPPORTED_PRECONDITIONER = "block_diag"
+
+
+@dataclass
+class BSRMatrix:
+    """Block-sparse (3x3) matrix view over device tensors (warp BsrMatrix subset)."""
+
+    offsets: torch.Tensor  # [nrow+1] int32
+    columns: torch.Tensor  # [nnz] int32
+    values: torch.Tensor  # [nnz, 3, 3] float32
+    nrow: int
+    ncol: int
+
+    @property
+    def nnz(self) -> int:
+        return int(self.columns.numel())
+
+    def dense(self) -> np.ndarray:
+        """FP64 dense copy (small matrices; test helper)."""
+        off = self.offsets.cpu().numpy()
+        cols = self.columns.cpu().numpy()
+        vals = self.values.detach().cpu().numpy().astype(np.float64)
+        out = np.zeros((3 * self.nrow, 3 * self.ncol))
+        for r in range(self.nrow):
+            for k in range(off[r], off[r + 1]):
+                c = cols[k]
+                out[3 * r : 3 * r + 3, 3 * c : 3 * c + 3] += vals[k]
+        return out
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _check_vec3(name: str, t, count: int, device: torch.device):
+    if (
+        not isinstance(t, torch.Tensor)
+        or t.dtype != torch.float32
+        or tuple(t.shape) != (count, 3)
+        or t.device != device
+        or not t.is_contiguous()
+    ):
+        raise ValueError(f"{name} must be a contiguous float32 tensor of shape ({count}, 3) on {device}")
+
+
+class SolverFEM:
+    """Linearly implicit backward-Euler FEM solver (Tet4 + Hex8, SNH).
+
+    Constructor arguments follow the reference (solver.py:212-240).  The
+    implemented linear solver is block-Jacobi preconditioned CG
+    (``solver_type="cg"``, ``preconditioner_type="block_diag"``); other
+    Krylov methods / preconditioners, springs, triangles, particle
+    self-contact and soft contacts are outside this substep path and raise
+    ``NotImplementedError``.
+    """
+
+    def __init__(
+        self,
+        model: Model,
+        dt: float = 1.0 / 60.0,
+        mass: float = 1.0,
+        preconditioner_type: str = "diag",
+        solver_type: str = "bicgstab",
+        linear_solver_maxiter: int = 50,
+        min_stretch: float = 0.05,
+        max_stretch: float = 20.0,
+        ground_plane=None,
+        ground_ke: float = 1.0e5,
+        ground_kd: float = 1.0e2,
+        ground_kf: float = 1.0e3,
+        ground_mu: float = 0.5,
+        torque_stiffness: float = 0.0,
+        torque_damping: float = 0.0,
+        spring_rest_direction=None,
+        self_contact_ke: float = 1.0e3,
+        self_contact_kd: float = 1.0e2,
+        linear_damping: float = 0.0,
+        differentiable: bool = False,
+        enable_tet_activations: bool = False,
+        defer_differentiation_checks: bool = False,
+        tet_volume_weights=None,
+        fixed_particle_indices=None,
+        linear_solver_tolerance: float = 1.0e-2,
+        linear_solver_use_cuda_graph: bool = True,
+        instance_offsets=None,
+    ):
+        # Argument validation, reference solver.py:241-261.
+        if not math.isfinite(linear_solver_tolerance) or linear_solver_tolerance <= 0:
+            raise ValueError("linear_solver_tolerance must be finite and positive")
+        fixed = np.asarray([] if fixed_particle_indices is None else fixed_particle_indices)
+        if fixed.ndim != 1 or (fixed.size and fixed.dtype.kind not in "iu"):
+            raise ValueError("fixed_particle_indices must be a one-dimensional integer sequence")
+        if fixed.size and (fixed.min() < 0 or fixed.max() >= model.particle_count):
+            raise ValueError("fixed_particle_indices contains an out-of-range node")
+        fixed = np.unique(fixed).astype(np.int32)
+        if fixed.size and differentiable:
+            raise ValueError("fixed_particle_indices is not supported with differentiable=True")
+        if not (isinstance(dt, (int, float)) and math.isfinite(dt) and dt > 0):
+            raise ValueError("dt must be finite and positive")
+        if solver_type != SUPPORTED_SOLVER or preconditioner_type != SUPPORTED_PRECONDITIONER:
+            raise NotImplementedError(
+                "this substep implements solver_type='cg' with preconditioner_type='block_diag'; "
+                f"got solver_type={solver_type!r}, preconditioner_type={preconditioner_type!r}"
+            )
+        if model.spring_count or model.tri_count:
+            raise NotImplementedError("springs and triangle membranes are not part of this substep path")
+        if torque_stiffness > 0.0 and spring_rest_direction is not None:
+            raise NotImplementedError("spring bend torque is not part of this substep path")
+        if self_contact_ke > 0.0 and model.particle_count > 1 and float(np.max(model.host["particle_radius"], initial=0)) > 0:
+            raise NotImplementedError("particle self-contact is not part of this substep path; pass self_contact_ke=0")
+        if model.device.type != "cuda":
+            raise ValueError("SolverFEM runs on the HIP device: finalize the model with device='cuda'")
+        if tet_volume_weights is not None:
+            w = torch.as_tensor(tet_volume_weights)
+            if tuple(w.shape) != (model.tet_count,) or w.dtype != torch.float32:
+                raise ValueError("tet_volume_weights must be a float32 array of shape (tet_count,)")
+            if not torch.isfinite(w).all() or (w < 0).any():
+                raise ValueError("tet_volume_weights must be finite and nonnegative")
+            tet_volume_weights = w.to(model.device)
+
+        self.model = model
+        self.device = model.device
+        self.mass = float(mass)
+        self.solver_type = solver_type
+        self.preconditioner_type = preconditioner_type
+        self.min_stretch = float(min_stretch)
+        self.max_stretch = float(max_stretch)
+        self.linear_damping = float(linear_damping)
+        self.differentiable = bool(differentiable)
+        self.enable_tet_activations = bool(enable_tet_activations)
+        self.defer_differentiation_checks = bool(defer_differentiation_checks)
+        self.linear_solver_use_cuda_graph = bool(linear_solver_use_cuda_graph)
+        self.linear_solver_check_every = 0
+        self.last_cg_iters: int | None = None
+        self.last_differentiation_residuals = None
+        self.dt = float(dt)
+        self._ground_plane = None if ground_plane is None else tuple(float(g) for g in ground_plane)
+        self._ground = (float(ground_ke), float(ground_kd), float(ground_kf), float(ground_mu))
+        self._tolerance = float(linear_solver_tolerance)
+        self._maxiter = int(linear_solver_maxiter)
+        self._tet_volume_weights = tet_volume_weights
+        self.fixed_particle_indices = fixed
+        self.hex_count = int(model.fem.hex_count)
+        self._diff = None
+        self._revision = 0
+
+        self._lib = _lib.load()
+        host = model.host
+        P = model.particle_count
+        self._host_keep = {}
+
+        def fptr(name, arr):
+            a = np.ascontiguousarray(arr, dtype=np.float32)
+            self._host_keep[name] = a
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+        def iptr(name, arr):
+            a = np.ascontiguousarray(arr, dtype=np.int32)
+            self._host_keep[name] = a
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+        desc = _lib.ModelDesc()
+        desc.particle_count = P
+        desc.particle_q = fptr("q", host["particle_q"])
+        desc.particle_mass = fptr("m", host["particle_mass"])
+        desc.particle_inv_mass = fptr("im", host["particle_inv_mass"])
+        desc.particle_radius = fptr("r", host["particle_radius"])
+        desc.particle_flags = iptr("fl", host["particle_flags"])
+        desc.tet_count = model.tet_count
+        desc.tet_indices = iptr("ti", host["tet_indices"])
+        desc.tet_poses = fptr("tp", host["tet_poses"])
+        desc.tet_materials = fptr("tm", host["tet_materials"])
+        desc.hex_count = self.hex_count
+        desc.hex_indices = iptr("hi", host["hex_indices"])
+        desc.hex_inv_J0 = fptr("hj", host["hex_inv_J0"])
+        desc.hex_det_J0_w = fptr("hw", host["hex_det_J0_w"])
+        desc.hex_materials = fptr("hm", host["hex_materials"])
+        desc.fixed_count = int(fixed.size)
+        desc.fixed_indices = iptr("fx", fixed)
+        if instance_offsets is not None:
+            offs = np.asarray(instance_offsets, dtype=np.int32)
+            desc.instance_count = int(len(offs) - 1)
+            desc.instance_offsets = iptr("io", offs)
+            self.instance_count = int(len(offs) - 1)
+        else:
+            desc.instance_count = 1
+            self.instance_count = 1
+        handle = ctypes.c_void_p()
+        torch.cuda.set_device(self.device.index or 0)
+        _lib.check(self._lib.fem_create(ctypes.byref(desc), ctypes.byref(handle)), "fem_create")
+        self._sys = handle
+        nnz = ctypes.c_int64()
+        maxrow = ctypes.c_int32()
+        _lib.check(self._lib.fem_pattern_size(self._sys, ctypes.byref(nnz), ctypes.byref(maxrow)), "fem_pattern_size")
+        dev = self.device
+        offsets = torch.zeros(P + 1, dtype=torch.int32, device=dev)
+        columns = torch.zeros(max(int(nnz.value), 1), dtype=torch.int32, device=dev)
+        values = torch.zeros(max(int(nnz.value), 1), 3, 3, dtype=torch.float32, device=dev)
+        self._pc_inv_diag = torch.zeros(max(P, 1), 3, 3, dtype=torch.float32, device=dev)
+        self.dv = torch.zeros(max(P, 1), 3, dtype=torch.float32, device=dev)[:P]
+        _lib.check(
+            self._lib.fem_bind(self._sys, offsets.data_ptr(), columns.data_ptr(), values.data_ptr(),
+                               self._pc_inv_diag.data_ptr(), self.dv.data_ptr()),
+            "fem_bind",
+        )
+        n = int(nnz.value)
+        self.A_bsr = BSRMatrix(offsets, columns[:n], values[:n], P, P)
+        self._push_params()
+        # Construction-time rest tangent (solver.py:324-341).
+        _lib.check(self._lib.fem_assemble(self._sys, model.particle_q.data_ptr(), self.dt, _stream(dev)), "fem_assemble")
+        self._anchor = model.particle_q.clone()
+        self._initial_volume = self._compute_volume(model.particle_q)
+
+    # ------------------------------------------------------------------
+    def _push_params(self):
+        p = _lib.Params()
+        p.min_stretch = self.min_stretch
+        p.max_stretch = self.max_stretch
+        p.has_ground = 0 if self._ground_plane is None else 1
+        for k in range(4):
+            p.ground[k] = (0.0, 0.0, 1.0, 0.0)[k] if self._ground_plane is None else self._ground_plane[k]
+        p.ground_ke, p.ground_kd, p.ground_kf, p.ground_mu = self._ground
+        p.linear_damping = self.linear_damping
+        for k in range(3):
+            p.gravity[k] = self.model.gravity[k]
+        p.enable_tet_activations = 1 if self.enable_tet_activations else 0
+        p.tolerance = self._tolerance
+        p.maxiter = self._maxiter
+        p.early_exit = 1 if self.linear_solver_use_cuda_graph else 0
+        self._params = p
+        self._gravity_pushed = tuple(self.model.gravity)
+        _lib.check(self._lib.fem_set_params(self._sys, ctypes.byref(p)), "fem_set_params")
+
+    def __del__(self):
+        sys = getattr(self, "_sys", None)
+        if sys is not None and getattr(self, "_lib", None) is not None:
+            try:
+                torch.cuda.synchronize(self.device)
+                self._lib.fem_destroy(sys)
+            except Exception:
+                pass
+            self._sys = None
+
+    @property
+    def linear_solver_tolerance(self) -> float:
+        return self._tolerance
+
+    @linear_solver_tolerance.setter
+    def linear_solver_tolerance(self, value: float):
+        value = float(value)
+        if not math.isfinite(value) or value <= 0:
+            raise ValueError("linear_solver_tolerance must be finite and positive")
+        self._tolerance = value
+        self._push_params()
+
+    @property
+    def linear_solver_maxiter(self) -> int:
+        return self._maxiter
+
+    @linear_solver_maxiter.setter
+    def linear_solver_maxiter(self, value: int):
+        self._maxiter = int(value)
+        self._push_params()
+
+    @property
+    def velocity_increment(self) -> torch.Tensor:
+        """Persistent solved velocity increment (CG warm start)."""
+        return self.dv
+
+    def set_launch(self, elem_lanes: int = 0, cg_lanes: int = 0, cg_workgroups: int = 0):
+        """Override kernel launch shapes (0 keeps the current value / auto)."""
+        _lib.check(self._lib.fem_set_launch(self._sys, elem_lanes, cg_lanes, cg_workgroups), "fem_set_launch")
+
+    def launch_config(self) -> dict:
+        v = [ctypes.c_int32() for _ in range(4)]
+        _lib.check(self._lib.fem_get_launch(self._sys, *[ctypes.byref(x) for x in v]), "fem_get_launch")
+        return dict(zip(("elem_lanes", "cg_lanes", "cg_workgroups", "cg_rows_per_thread"), (x.value for x in v)))
+
+    # ------------------------------------------------------------------
+    def reset(self, state=None, world_mask=None, flags=None):
+        """Clear the reusable solve workspace (reference solver.py:436-443)."""
+        self.dv.zero_()
+
+    def notify_model_changed(self, flags=None):
+        """Re-read gravity from the model (MODEL_PROPERTIES)."""
+        self._push_params()
+
+    def refresh_preconditioner(self):
+        """The block-Jacobi inverse is rewritten with A by every assembly; nothing to do."""
+
+    def refresh_tangent(self, particle_q: torch.Tensor, dt: float | None = None) -> None:
+        """Assemble A = M + h^2 K at ``particle_q`` (reference solver.py:769-857).
+
+        Fixed nodes of ``particle_q`` are first reset to their anchors (as
+        the reference does).  A changed timestep clears the warm start.
+        """
+        dt = self.dt if dt is None else float(dt)
+        if not math.isfinite(dt) or dt <= 0:
+            raise ValueError("refresh_tangent dt must be finite and positive")
+        _check_vec3("particle_q", particle_q, self.model.particle_count, self.device)
+        if self._diff is not None and self._diff.recording():
+            raise RuntimeError("Reassembly inside a recorded rollout is unsupported; the cached matrix is fixed")
+        if dt != self.dt:
+            if self.differentiable:
+                raise ValueError("Differentiable tangent refresh requires the construction timestep")
+            self.dv.zero_()
+        st = _stream(self.device)
+        _lib.check(self._lib.fem_restore_fixed(self._sys, particle_q.data_ptr(), None, st), "fem_restore_fixed")
+        _lib.check(self._lib.fem_assemble(self._sys, particle_q.data_ptr(), float(dt), st), "fem_assemble")
+        self.dt = dt
+        self._revision += 1
+
+    def rebuild_element_blocks(self, particle_q, dt, *, update_preconditioner=True):
+        """Reassemble the deformation-dependent tangent (reference solver.py:506-508)."""
+        self.refresh_tangent(particle_q, dt)
+
+    def refresh_topology(self, particle_q, dt=None, *, element_sources=None):
+        """Same connectivity: equivalent to a tangent refresh and a cleared warm start."""
+        self.refresh_tangent(particle_q, dt)
+        self.dv.zero_()
+
+    def _sync_params(self):
+        if tuple(self.model.gravity) != self._gravity_pushed:
+            self._push_params()
+
+    def step(self, state_in: State, state_out: State, control: Control | None, contacts, dt: float, *,
+             external_forces: torch.Tensor | None = None, observables=None) -> State:
+        """One linearly implicit substep (reference solver.py:942-1002).
+
+        ``state_in.particle_f`` is overwritten with the projected right-hand
+        side; ``state_in`` positions/velocities of fixed nodes are reset to
+        their anchors; ``state_out`` receives the integrated state.
+        """
+        P = self.model.particle_count
+        if external_forces is not None:
+            _check_vec3("external_forces", external_forces, P, self.device)
+        if not (math.isfinite(dt) and dt > 0):
+            raise ValueError("dt must be finite and positive")
+        if self.differentiable:
+            from .differentiation import differentiable_step
+
+            return differentiable_step(self, state_in, state_out, control, contacts, dt, external_forces)
+        if contacts is not None and hasattr(contacts, "soft_contact_count"):
+            raise NotImplementedError("soft-rigid contacts are not part of this substep path")
+        for name, t in (("state_in.particle_q", state_in.particle_q), ("state_in.particle_qd", state_in.particle_qd),
+                        ("state_in.particle_f", state_in.particle_f), ("state_out.particle_q", state_out.particle_q),
+                        ("state_out.particle_qd", state_out.particle_qd)):
+            _check_vec3(name, t, P, self.device)
+        self._sync_params()
+        act = None
+        if self.enable_tet_activations:
+            ctrl = control if control is not None else self.model.control()
+            act = ctrl.tet_activations
+            if act.dtype != torch.float32 or act.numel() != self.model.tet_count or act.device != self.device:
+                raise ValueError("control.tet_activations must be a float32 (tet_count,) tensor on the model device")
+            act = act.contiguous()
+        st = _stream(self.device)
+        _lib.check(
+            self._lib.fem_step(self._sys, state_in.particle_q.data_ptr(), state_in.particle_qd.data_ptr(),
+                               state_in.particle_f.data_ptr(), _ptr(external_forces), _ptr(act),
+                               state_out.particle_q.data_ptr(), state_out.particle_qd.data_ptr(), float(dt), st),
+            "fem_step",
+        )
+        if self.linear_solver_check_every > 0:
+            stats = self.solve_stats()
+            self.last_cg_iters = int(stats["iterations"][0])
+        return state_out
+
+    def solve_stats(self) -> dict:
+        """Per-instance statistics of the last solve (synchronizes)."""
+        n = self.instance_count
+        it = (ctypes.c_int32 * n)()
+        rr = (ctypes.c_float * n)()
+        at = (ctypes.c_float * n)()
+        er = (ctypes.c_int32 * n)()
+        _lib.check(self._lib.fem_solve_stats(self._sys, _stream(self.device), it, rr, at, er), "fem_solve_stats")
+        if any(er):
+            raise _lib.FEMError("CG barrier timed out: the persistent launch was not fully resident")
+        return {"iterations": list(it), "residual_sq": list(rr), "atol_sq": list(at)}
+
+    # ------------------------------------------------------------------
+    # Force evaluators (reference solver.py:1357-1556), unscaled [N].
+    def _eval(self, terms: int, state: State, control: Control | None = None) -> torch.Tensor:
+        self._sync_params()
+        out = torch.empty_like(state.particle_q)
+        act = None
+        if self.enable_tet_activations and (terms & _lib.FEM_TERM_TET):
+            act = (control if control is not None else self.model.control()).tet_activations.contiguous()
+        _lib.check(
+            self._lib.fem_eval_forces(self._sys, terms, state.particle_q.data_ptr(), state.particle_qd.data_ptr(),
+                                      _ptr(act), out.data_ptr(), _stream(self.device)),
+            "fem_eval_forces",
+        )
+        return out
+
+    def eval_tetrahedral_forces(self, model, control, state):
+        return self._eval(_lib.FEM_TERM_TET, state, control)
+
+    def eval_hexahedral_forces(self, model, control, state):
+        return self._eval(_lib.FEM_TERM_HEX, state, control)
+
+    def eval_particle_ground_contact_forces(self, model, control, state):
+        return self._eval(_lib.FEM_TERM_GROUND, state)
+
+    def eval_gravity_forces(self, model, state=None):
+        s = state if state is not None else State(model.particle_q, model.particle_qd, None)
+        return self._eval(_lib.FEM_TERM_GRAVITY, s)
+
+    def eval_linear_damping_forces(self, model, state):
+        return self._eval(_lib.FEM_TERM_DAMPING, state)
+
+    # ------------------------------------------------------------------
+    # Volume telemetry (reference solver.py:1614-1701).
+    def _compute_volume(self, positions: torch.Tensor | None) -> float:
+        total = 0.0
+        m = self.model
+        if m.tet_count and positions is not None:
+            idx = m.tet_indices.long()
+            p = positions[idx]
+            e = p[:, 1:] - p[:, :1]
+            vol = torch.linalg.det(e.double()).abs() / 6.0
+            if self._tet_volume_weights is not None:
+                vol = vol * self._tet_volume_weights.double()
+            total += float(vol.sum())
+        if self.hex_count:
+            total += float(m.fem.hex_det_J0_w.double().sum())
+        return total
+
+    def compute_volume(self, state: State | None = None) -> float:
+        return self._compute_volume(None if state is None else state.particle_q)
+
+    def get_initial_volume(self, state=None) -> float:
+        return self._initial_volume if self._initial_volume > 1e-12 else 1.0
+
+    def get_volume_ratio(self, state=None) -> float:
+        initial = self.get_initial_volume(state)
+        current = self.compute_volume(state)
+        if not math.isfinite(current) or current <= 0.0:
+            return 1.0
+        ratio = current / initial
+        return ratio if math.isfinite(ratio) else 1.0
+
+    # Differentiation entry points (reference solver.py:445-469).
+    def prepare_differentiation(self):
+        if not self.differentiable:
+            raise RuntimeError("Construct the solver with differentiable=True")
+        from .differentiation import prepare
+
+        return prepare(self)
+
+    def check_differentiation(self, *, reset=True):
+        if not self.differentiable:
+            raise RuntimeError("Construct the solver with differentiable=True")
+        from .differentiation import check
+
+        return check(self, reset=reset)
+
+
+SolverLinearFEM = SolverFEM

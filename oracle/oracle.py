@@ -1,0 +1,231 @@
+"""ctypes wrapper of the CPU restatement in ``fem_oracle.cpp``.
+
+Test infrastructure only: imported by ``tests/``, ``__graft_entry__.smoke``
+and the CPU-baseline leg of ``bench.py`` as the checker / baseline, never by
+the shipped package.  ``bits=64`` is the FP64 oracle, ``bits=32`` mirrors the
+reference's fp32 CPU path (nvidia-beer/newton-solver-fem ``solver.py``
+``SolverFEM.step`` / ``refresh_tangent``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc_fem.so"
+
+TERM_TET, TERM_HEX, TERM_GROUND, TERM_GRAVITY, TERM_DAMPING = 1, 2, 4, 8, 16
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "fem_oracle.cpp").stat().st_mtime:
+        subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        d, i = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        _lib.orc_create.restype = ctypes.c_void_p
+        _lib.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, d, d, d, d, i, ctypes.c_int, i, d, d, ctypes.c_int, i,
+                                    d, d, d, ctypes.c_int, i]
+        _lib.orc_destroy.argtypes = [ctypes.c_void_p]
+        _lib.orc_set_params.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_construct.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        _lib.orc_refresh.argtypes = [ctypes.c_void_p, d, ctypes.c_double]
+        _lib.orc_nnz.argtypes = [ctypes.c_void_p]
+        _lib.orc_nnz.restype = ctypes.c_int
+        _lib.orc_get_matrix.argtypes = [ctypes.c_void_p, i, i, d, d]
+        _lib.orc_get_dv.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_set_dv.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_step.argtypes = [ctypes.c_void_p, d, d, d, ctypes.c_void_p, ctypes.c_void_p, d, d, ctypes.c_double]
+        _lib.orc_step.restype = ctypes.c_int
+        _lib.orc_last_stats.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_eval_terms.argtypes = [ctypes.c_void_p, ctypes.c_int, d, d, ctypes.c_void_p, d]
+        _lib.orc_clamp_stretch.argtypes = [ctypes.c_int, d, ctypes.c_double, ctypes.c_double, d]
+        _lib.orc_svd3.argtypes = [d, d, d, d]
+        _lib.orc_block_inverse.argtypes = [ctypes.c_int, d, d]
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def clamp_stretch(F, min_s, max_s, bits=64):
+    F, pf = _d(np.asarray(F).reshape(9))
+    out, po = _d(np.zeros(9))
+    lib().orc_clamp_stretch(bits, pf, float(min_s), float(max_s), po)
+    return out.reshape(3, 3)
+
+
+def svd3(F):
+    F, pf = _d(np.asarray(F).reshape(9))
+    U, pu = _d(np.zeros(9))
+    S, ps = _d(np.zeros(3))
+    V, pv = _d(np.zeros(9))
+    lib().orc_svd3(pf, pu, ps, pv)
+    return U.reshape(3, 3), S, V.reshape(3, 3)
+
+
+def block_inverse(B, bits=64):
+    B, pb = _d(np.asarray(B).reshape(9))
+    out, po = _d(np.zeros(9))
+    lib().orc_block_inverse(bits, pb, po)
+    return out.reshape(3, 3)
+
+
+class OracleSolver:
+    """CPU restatement of ``SolverFEM`` for one model.
+
+    ``host`` is the dict of host arrays a model carries (``model.host``).
+    Parameters mirror the reference constructor (solver.py:212-240).
+    """
+
+    def __init__(self, host: dict, *, bits=64, dt=1.0 / 60.0, gravity=(0.0, 0.0, -9.81), fixed=(),
+                 min_stretch=0.05, max_stretch=20.0, ground_plane=None, ground_ke=1.0e5, ground_kd=1.0e2,
+                 ground_kf=1.0e3, ground_mu=0.5, linear_damping=0.0, enable_tet_activations=False,
+                 tolerance=1.0e-2, maxiter=50):
+        L = lib()
+        self.bits = bits
+        self.P = len(host["particle_q"])
+        self.T = len(host["tet_indices"])
+        self.H = len(host["hex_indices"])
+        self._keep = []
+
+        def D(a):
+            arr, p = _d(a)
+            self._keep.append(arr)
+            return p
+
+        def I(a):
+            arr, p = _i(a)
+            self._keep.append(arr)
+            return p
+
+        fixed = np.unique(np.asarray(fixed, dtype=np.int64)).astype(np.int32)
+        self.fixed = fixed
+        self.h = L.orc_create(
+            bits, self.P, D(host["particle_q"]), D(host["particle_mass"]), D(host["particle_inv_mass"]),
+            D(host["particle_radius"]), I(host["particle_flags"]), self.T, I(host["tet_indices"]),
+            D(host["tet_poses"]), D(host["tet_materials"]), self.H, I(host["hex_indices"]), D(host["hex_inv_J0"]),
+            D(host["hex_det_J0_w"]), D(host["hex_materials"]), len(fixed), I(fixed),
+        )
+        self.params = dict(min_stretch=min_stretch, max_stretch=max_stretch, ground_plane=ground_plane,
+                           ground_ke=ground_ke, ground_kd=ground_kd, ground_kf=ground_kf, ground_mu=ground_mu,
+                           linear_damping=linear_damping, gravity=gravity,
+                           enable_tet_activations=enable_tet_activations, tolerance=tolerance, maxiter=maxiter)
+        self._push_params()
+        self.dt = float(dt)
+        L.orc_construct(self.h, float(dt))
+
+    def _push_params(self):
+        p = self.params
+        g = p["ground_plane"]
+        arr = np.array(
+            [p["min_stretch"], p["max_stretch"], 0.0 if g is None else 1.0, *(g if g is not None else (0, 0, 1, 0)),
+             p["ground_ke"], p["ground_kd"], p["ground_kf"], p["ground_mu"], p["linear_damping"], *p["gravity"],
+             1.0 if p["enable_tet_activations"] else 0.0, p["tolerance"], p["maxiter"]],
+            dtype=np.float64,
+        )
+        _, ptr = _d(arr)
+        self._params_arr = arr
+        lib().orc_set_params(self.h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def set(self, **kw):
+        self.params.update(kw)
+        self._push_params()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def refresh_tangent(self, q: np.ndarray, dt=None) -> np.ndarray:
+        dt = self.dt if dt is None else float(dt)
+        q = np.ascontiguousarray(q, dtype=np.float64).copy()
+        lib().orc_refresh(self.h, q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), dt)
+        return q
+
+    def matrix(self):
+        n = lib().orc_nnz(self.h)
+        off = np.zeros(self.P + 1, np.int32)
+        cols = np.zeros(n, np.int32)
+        vals = np.zeros((n, 3, 3))
+        minv = np.zeros((self.P, 3, 3))
+        lib().orc_get_matrix(self.h, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             minv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return off, cols, vals, minv
+
+    def dense(self) -> np.ndarray:
+        off, cols, vals, _ = self.matrix()
+        out = np.zeros((3 * self.P, 3 * self.P))
+        for r in range(self.P):
+            for k in range(off[r], off[r + 1]):
+                c = cols[k]
+                out[3 * r : 3 * r + 3, 3 * c : 3 * c + 3] += vals[k]
+        return out
+
+    @property
+    def dv(self) -> np.ndarray:
+        out = np.zeros((self.P, 3))
+        lib().orc_get_dv(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    @dv.setter
+    def dv(self, value):
+        v = np.ascontiguousarray(np.broadcast_to(value, (self.P, 3)), dtype=np.float64)
+        lib().orc_set_dv(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def step(self, q, qd, dt=None, external_forces=None, activations=None):
+        """One substep; returns dict(q_in, qd_in (after fixed restore), f, q, qd, iters)."""
+        dt = self.dt if dt is None else float(dt)
+        P = self.P
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(P, 3).copy()
+        qd = np.ascontiguousarray(qd, dtype=np.float64).reshape(P, 3).copy()
+        f = np.zeros((P, 3))
+        qo = np.zeros((P, 3))
+        qdo = np.zeros((P, 3))
+        ext = None if external_forces is None else np.ascontiguousarray(external_forces, dtype=np.float64)
+        act = None if activations is None else np.ascontiguousarray(activations, dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        it = lib().orc_step(self.h, q.ctypes.data_as(dp), qd.ctypes.data_as(dp), f.ctypes.data_as(dp),
+                            None if ext is None else ext.ctypes.data_as(ctypes.c_void_p),
+                            None if act is None else act.ctypes.data_as(ctypes.c_void_p),
+                            qo.ctypes.data_as(dp), qdo.ctypes.data_as(dp), dt)
+        return {"q_in": q, "qd_in": qd, "f": f, "q": qo, "qd": qdo, "iters": it}
+
+    def last_stats(self):
+        out = np.zeros(4)
+        lib().orc_last_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return {"iters": int(out[0]), "rr": out[1], "atol_sq": out[2], "scale": out[3]}
+
+    def eval_terms(self, mask, q, qd, activations=None):
+        P = self.P
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(P, 3)
+        qd = np.ascontiguousarray(qd, dtype=np.float64).reshape(P, 3)
+        out = np.zeros((P, 3))
+        act = None if activations is None else np.ascontiguousarray(activations, dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib().orc_eval_terms(self.h, int(mask), q.ctypes.data_as(dp), qd.ctypes.data_as(dp),
+                             None if act is None else act.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(dp))
+        return out
