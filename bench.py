@@ -1,0 +1,216 @@
+"""Substep throughput of the HIP SolverFEM on one MI355X.
+
+Workload (one "step" = one substep): the large Tet4 solid -- 60x60x56 cells,
+5 tets per cell (1,008,000 tets, 212,097 nodes, 2,693,569 BSR blocks),
+interior nodes jittered +-0.2 cell, per-element E log-uniform in
+[5e4, 5e5] Pa, nu = 0.45, rho = 1000, x = 0 face fixed, gravity.  Each
+substep refreshes the tangent (streaming assembly of A = M + h^2 K and the
+block-Jacobi inverse), evaluates forces + predictor, and runs exactly 50
+PCG iterations (device residual exit disabled; the updates are
+residual-masked so the result equals the tol-1e-2 solve).
+
+Prints one JSON line (see README of the measurement contract in DESIGN.md).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "newton-solver-fem_amd"))
+
+METRIC = "substeps/s at 1M Tet4 (fp32, %HBM roofline); batched vs looped steps/s; adjoint/fwd"
+HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 600.0
+CG_ITERS = 50
+
+
+def cg_algorithmic_bytes(blocks: int, nodes: int, iters: int) -> int:
+    """Per CG launch: (40 B per block + 148 B per node) per iteration.
+
+    SpMV reads 36 B of values + 4 B of column per block and 28 B per node
+    (offsets 4, p 12, q 12); the vector phase reads x, p, r, q, M^-1 and
+    writes x, r, p (120 B per node).
+    """
+    return iters * (40 * blocks + 148 * nodes)
+
+
+def build_large(device):
+    from newton_solver_fem_amd import build_tet_model, meshes
+
+    scene = meshes.large_tet_scene()
+    model = build_tet_model(scene, device=device)
+    return scene, model
+
+
+def cpu_baseline(scene, model, substeps: int):
+    """FP32 CPU restatement of the reference substep (single thread)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+
+    import oracle
+
+    o = oracle.OracleSolver(model.host, bits=32, dt=DT, gravity=model.gravity, fixed=scene.fixed, tolerance=1e-2,
+                            maxiter=CG_ITERS)
+    q = model.host["particle_q"].astype(np.float64)
+    v = np.zeros_like(q)
+    q = o.refresh_tangent(q)  # first refresh builds the slot map (setup, untimed)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        q = o.refresh_tangent(q)
+        out = o.step(q, v)
+        q, v = out["q"], out["qd"]
+    el = time.perf_counter() - t0
+    return {
+        "value": substeps / el,
+        "unit": "substeps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{substeps} substeps (tangent refresh + step, tol 1e-2, <=50 CG iterations) of the same "
+                  f"1,008,000-tet workload, fp32 C++ restatement, one thread, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 512 MiB device copy (scale reference for PMC byte counters)")
+    ap.add_argument("--cg-workgroups", type=int, default=0)
+    ap.add_argument("--cg-lanes", type=int, default=0)
+    ap.add_argument("--elem-lanes", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    scene, model = build_large(dev)
+    solver = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag",
+                       linear_solver_maxiter=CG_ITERS, linear_solver_tolerance=1e-2, self_contact_ke=0.0,
+                       fixed_particle_indices=scene.fixed, linear_solver_use_cuda_graph=False)
+    if args.cg_workgroups or args.cg_lanes or args.elem_lanes:
+        solver.set_launch(args.elem_lanes, args.cg_lanes, args.cg_workgroups)
+    lib, sysh = solver._lib, solver._sys
+    s0, s1 = model.state(), model.state()
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+
+    from newton_solver_fem_amd._lib import check
+
+    def substep(ev=None):
+        nonlocal s0, s1
+        if ev is not None:
+            ev[0].record(stream)
+        check(lib.fem_restore_fixed(sysh, s0.particle_q.data_ptr(), None, sptr), "restore")
+        check(lib.fem_assemble(sysh, s0.particle_q.data_ptr(), DT, sptr), "assemble")
+        if ev is not None:
+            ev[1].record(stream)
+        check(lib.fem_step_rhs(sysh, s0.particle_q.data_ptr(), s0.particle_qd.data_ptr(), s0.particle_f.data_ptr(),
+                               None, None, DT, sptr), "step_rhs")
+        if ev is not None:
+            ev[2].record(stream)
+        check(lib.fem_step_solve(sysh, s0.particle_q.data_ptr(), s0.particle_qd.data_ptr(), s0.particle_f.data_ptr(),
+                                 s1.particle_q.data_ptr(), s1.particle_qd.data_ptr(), DT, sptr), "step_solve")
+        if ev is not None:
+            ev[3].record(stream)
+        s0, s1 = s1, s0
+
+    for _ in range(args.warmup):
+        substep()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        substep(events[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    asm_ms = np.mean([e[0].elapsed_time(e[1]) for e in events])
+    rhs_ms = np.mean([e[1].elapsed_time(e[2]) for e in events])
+    cg_ms = np.mean([e[2].elapsed_time(e[3]) for e in events])
+    stats = solver.solve_stats()
+    if not np.isfinite(s0.particle_q.cpu().numpy()).all():
+        raise RuntimeError("non-finite state")
+
+    # Same workload with the device residual exit: iterations actually needed at tol 1e-2.
+    solver.linear_solver_use_cuda_graph = True
+    solver._push_params()
+    substep()
+    early_iters = solver.solve_stats()["iterations"][0]
+
+    if args.copy_probe:
+        a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    P, B = model.particle_count, solver.A_bsr.nnz
+    alg = cg_algorithmic_bytes(B, P, CG_ITERS)
+    achieved = alg / (cg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "cg_large_tet4_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "large Tet4 solid 60x60x56 cells x 5 tets, jittered, log-uniform E, x=0 face fixed; "
+                        "tangent refresh + forces/predictor + 50 PCG iterations per substep",
+            "tets": model.tet_count,
+            "nodes": P,
+            "bsr_blocks": B,
+            "cg_iterations": CG_ITERS,
+            "dt": DT,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cg (persistent PCG, 50 iterations)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes": alg,
+        },
+        "phases_ms": {"assemble": asm_ms, "forces_rhs": rhs_ms, "cg": cg_ms},
+        "cg_iterations_at_tol": early_iters,
+        "launch": solver.launch_config(),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, model, args.cpu_substeps)
+        out["cpu_baseline"]["host_cpus"] = os.cpu_count()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

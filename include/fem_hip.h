@@ -139,6 +139,16 @@ int fem_assemble(fem_system* sys, const float* q_dev, float dt, void* stream);
 int fem_step(fem_system* sys, float* q_in_dev, float* qd_in_dev, float* f_out_dev, const float* ext_dev,
              const float* act_dev, float* q_out_dev, float* qd_out_dev, float dt, void* stream);
 
+/* The two halves of fem_step, for callers that overlap or time them
+ * separately.  fem_step_rhs: fixed restore of the input state, forces and
+ * predictor -> f_out (solver.py:987-999, 1268-1299).  fem_step_solve: RHS
+ * normalization, PCG and integration (solver.py:1301-1351); f_in is the
+ * f_out of the preceding fem_step_rhs. */
+int fem_step_rhs(fem_system* sys, float* q_in_dev, float* qd_in_dev, float* f_out_dev, const float* ext_dev,
+                 const float* act_dev, float dt, void* stream);
+int fem_step_solve(fem_system* sys, const float* q_in_dev, const float* qd_in_dev, const float* f_in_dev,
+                   float* q_out_dev, float* qd_out_dev, float dt, void* stream);
+
 /* Sum of the selected unscaled force terms (FEM_TERM_* mask) at (q, qd):
  * eval_tetrahedra + eval_tet_activations (kernels.py:484-632),
  * eval_hexahedra (:1481-1601), eval_particle_ground_contacts (:1132-1195),

@@ -33,6 +33,10 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
+// Barrier words of one CG instance (uint offsets inside its BAR_WORDS region;
+// every counter on its own 64-byte line).
+constexpr int BAR_WORDS = 512;
+constexpr int BAR_TOP_COUNT = 0, BAR_TOP_GEN = 16, BAR_XCD = 32, BAR_REG = 320;
 
 thread_local std::string g_error;
 
@@ -92,18 +96,29 @@ struct RowArgs {
 // k_prepare: FixedParticles.restore(state_in) and per-step resets.
 // ---------------------------------------------------------------------------
 __global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float* __restrict__ anchor, float* q,
-                          float* qd, unsigned* rhs_max, unsigned* bar, int ninst, int* err) {
+                          float* qd, unsigned* rhs_max, int ninst) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nfixed) {
     int i = fixed[t];
     st3(q, i, ld3(anchor, i));
     if (qd) st3(qd, i, v3{0.f, 0.f, 0.f});
   }
+  if (t < ninst) rhs_max[t] = 0u;
+}
+
+// Per-solve reset of the CG barrier counters / XCC registration and the
+// device error word (generation words keep counting across launches).
+__global__ void k_cg_reset(unsigned* bar, int ninst, int* err) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ninst) {
-    rhs_max[t] = 0u;
-    bar[32 * t] = 0u;
+    unsigned* b = bar + BAR_WORDS * (size_t)t;
+    b[BAR_TOP_COUNT] = 0u;
+    for (int x = 0; x < 8; ++x) {
+      b[BAR_XCD + 32 * x] = 0u;
+      b[BAR_REG + x] = 0u;
+    }
   }
-  if (t == 0 && err) *err = 0;
+  if (t == 0) *err = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -424,6 +439,8 @@ struct CgArgs {
   const int* chunk_rows;
   const int* chunk_inst;
   const int* inst_chunks;
+  const int* chunk_tiles;  // [nchunks+1] first tile of each chunk
+  const int* tile_rows;    // [ntiles] first row of each tile (row-aligned, <= TILE blocks)
   const unsigned* rhs_max;
   float tolerance;
   int maxiter;
@@ -434,35 +451,106 @@ struct CgArgs {
   int* err;
 };
 
-// Instance-wide barrier over `n` co-resident workgroups (sense by generation).
-// Returns false after a bounded wait so that a launch that is not fully
+__device__ __forceinline__ unsigned* bar_xcount(unsigned* b, int x) { return b + BAR_XCD + 32 * x; }
+__device__ __forceinline__ unsigned* bar_xgen(unsigned* b, int x) { return b + BAR_XCD + 32 * x + 16; }
+
+__device__ __forceinline__ unsigned ld_acq(unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Spin until *gen != g; bounded so that a launch whose workgroups are not all
 // resident ends with an error flag instead of hanging.
-__device__ bool inst_barrier(unsigned* bar, int n, int* err, int* lds_flag) {
+__device__ __forceinline__ bool spin_gen(unsigned* gen, unsigned g, int* err) {
+  long spins = 0;
+  while (ld_acq(gen) == g) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > BARRIER_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+  return true;
+}
+
+// Flat barrier over n workgroups with full agent-scope release/acquire
+// (used once per launch, for the XCD registration).
+__device__ bool flat_barrier(unsigned* b, int n, int* err, int* lds_flag) {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
     if (n > 1) {
-      unsigned* count = bar;
-      unsigned* gen = bar + 16;
-      const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned* count = b + BAR_TOP_COUNT;
+      unsigned* gen = b + BAR_TOP_GEN;
+      const unsigned g = ld_acq(gen);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (arrived == (unsigned)(n - 1)) {
         __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        long spins = 0;
-        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > BARRIER_SPIN_LIMIT ||
-              __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = 0;
-            break;
-          }
-        }
+        ok = spin_gen(gen, g, err);
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    *lds_flag = ok;
+  }
+  __syncthreads();
+  return *lds_flag != 0;
+}
+
+struct XcdGroup {
+  int xcc;      // hardware XCC id of this workgroup
+  int nx;       // workgroups of this instance on that XCC
+  int ngroups;  // XCCs the instance spans
+};
+
+// Two-level barrier.  Workgroups of one XCC share its L2: each makes its own
+// stores complete (s_waitcnt) and arrives on the XCC counter.  The last
+// arriver of an XCC writes that L2 back, meets the other XCCs' last arrivers
+// on the top counter, invalidates and releases its XCC; the others then
+// acquire.  One L2 writeback per XCC per barrier instead of one per
+// workgroup, and no counter sees more arrivals than the workgroups of one XCC.
+__device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, int* err, int* lds_flag) {
+  // s_waitcnt is per wave and the workgroup barrier's release does not wait
+  // for global stores: every wave drains its own stores into L2 first.
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (n > 1) {
+      unsigned* xc = bar_xcount(b, xg.xcc);
+      unsigned* xgen = bar_xgen(b, xg.xcc);
+      const unsigned g = ld_acq(xgen);
+      const unsigned a = __hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a == (unsigned)(xg.nx - 1)) {
+        __hip_atomic_store(xc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (xg.ngroups > 1) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          unsigned* tc = b + BAR_TOP_COUNT;
+          unsigned* tg = b + BAR_TOP_GEN;
+          const unsigned gt = ld_acq(tg);
+          const unsigned at = __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (at == (unsigned)(xg.ngroups - 1)) {
+            __hip_atomic_store(tc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(tg, gt + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } else {
+            ok = spin_gen(tg, gt, err);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        asm volatile("" ::: "memory");
+        __hip_atomic_store(xgen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        ok = spin_gen(xgen, g, err);
+        // A workgroup-scope invalidate leaves this CU's L1 intact; the
+        // agent-scope acquire drops stale L1 (and clean L2) lines.
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
     }
     *lds_flag = ok;
   }
@@ -513,66 +601,114 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
   __syncthreads();
 }
 
-// q_lds[lr] = sum_j A_ij u_j for the workgroup's rows; u = s*dv (prologue,
-// fixed columns masked) or z + beta*p (iterations).  Fixed rows give 0.
-template <int G, bool PROLOGUE>
-__device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, float s, float beta, float* q_lds) {
-  constexpr int NG = BLOCK / G;
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  const int npass = (nr + NG - 1) / NG;
-  for (int ps = 0; ps < npass; ++ps) {
-    const int lr = ps * NG + grp;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    if (lr < nr) {
-      const int i = R0 + lr;
-      if (!a.row_fixed[i]) {
-        const int k1 = a.offsets[i + 1];
-        for (int k = a.offsets[i] + lane; k < k1; k += G) {
-          const int col = a.cols[k];
-          const float* b = a.vals + 9 * (size_t)k;
-          v3 u;
-          if (PROLOGUE) {
-            u = a.row_fixed[col] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, col) * s;
-          } else {
-            const v3 zz = ld3(a.zg, col), pp = ld3(a.pg, col);
-            u = {fmaf(beta, pp.x, zz.x), fmaf(beta, pp.y, zz.y), fmaf(beta, pp.z, zz.z)};
-          }
-          a0 += b[0] * u.x + b[1] * u.y + b[2] * u.z;
-          a1 += b[3] * u.x + b[4] * u.y + b[5] * u.z;
-          a2 += b[6] * u.x + b[7] * u.y + b[8] * u.z;
+constexpr int CG_J = 8;                   // blocks in flight per thread
+constexpr int CG_TILE = CG_J * BLOCK;     // blocks per row-aligned tile
+
+// q[k] = sum_j A_ij u_j for the thread's own rows i = R0 + t + 256 k.
+// Row-aligned tiles of <= CG_TILE blocks are streamed with CG_J coalesced
+// blocks per thread; block products go to LDS and each row is summed by its
+// owner thread.  u = s*dv with fixed columns masked (prologue) or
+// z + beta*p (iterations).  Fixed rows give q = 0.
+template <int K, bool PROLOGUE>
+__device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0, int t1, const int* off_l,
+                                        float s, float beta, float* part_l, const bool (&fx)[K], float (&q)[K][3]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) q[k][0] = q[k][1] = q[k][2] = 0.f;
+  for (int tile = t0; tile < t1; ++tile) {
+    const int ta = a.tile_rows[tile] - R0;
+    const int tb = (tile + 1 < t1 ? a.tile_rows[tile + 1] : R0 + nr) - R0;
+    const int kb = off_l[ta], ke = off_l[tb];
+    int col[CG_J];
+#pragma unroll
+    for (int j = 0; j < CG_J; ++j) {
+      const int k = kb + threadIdx.x + BLOCK * j;
+      col[j] = k < ke ? a.cols[k] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < CG_J; ++j) {
+      const int k = kb + threadIdx.x + BLOCK * j;
+      if (col[j] >= 0) {
+        const float* b = a.vals + 9 * (size_t)k;
+        v3 u;
+        if (PROLOGUE) {
+          u = a.row_fixed[col[j]] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, col[j]) * s;
+        } else {
+          const v3 zz = ld3(a.zg, col[j]), pp = ld3(a.pg, col[j]);
+          u = {fmaf(beta, pp.x, zz.x), fmaf(beta, pp.y, zz.y), fmaf(beta, pp.z, zz.z)};
         }
+        float* o = part_l + 3 * (k - kb);
+        o[0] = b[0] * u.x + b[1] * u.y + b[2] * u.z;
+        o[1] = b[3] * u.x + b[4] * u.y + b[5] * u.z;
+        o[2] = b[6] * u.x + b[7] * u.y + b[8] * u.z;
       }
     }
-    a0 = group_sum<G>(a0);
-    a1 = group_sum<G>(a1);
-    a2 = group_sum<G>(a2);
-    if (lane == 0 && lr < nr) {
-      q_lds[3 * lr] = a0;
-      q_lds[3 * lr + 1] = a1;
-      q_lds[3 * lr + 2] = a2;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int lr = threadIdx.x + BLOCK * k;
+      if (lr >= ta && lr < tb && !fx[k]) {
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        const int e = off_l[lr + 1] - kb;
+        for (int m = off_l[lr] - kb; m < e; ++m) {
+          s0 += part_l[3 * m];
+          s1 += part_l[3 * m + 1];
+          s2 += part_l[3 * m + 2];
+        }
+        q[k][0] = s0;
+        q[k][1] = s1;
+        q[k][2] = s2;
+      }
     }
+    __syncthreads();
   }
 }
 
 // Warp's absolute floor eps32^(9/4) on the residual norm (tol given, atol=0).
 __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-16; }
 
-template <int G, int K>
+template <int K>
 __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
-  __shared__ float q_lds[K * BLOCK * 3];
+  __shared__ float part_l[3 * CG_TILE];
+  __shared__ int off_l[K * BLOCK + 1];
   __shared__ double red[16];
   __shared__ int bflag;
-  constexpr bool MREG = (K <= 4);
+  __shared__ int xinfo[3];
   const int c = xcd_swizzle(blockIdx.x, gridDim.x);
   const int inst = a.chunk_inst[c];
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
-  unsigned* bar = a.bar + 32 * inst;
+  const int t0 = a.chunk_tiles[c], t1 = a.chunk_tiles[c + 1];
+  unsigned* bar = a.bar + BAR_WORDS * inst;
   const float m = __uint_as_float(a.rhs_max[inst]);
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
   const bool zero_start = (m == 0.f);
 
-  float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3], Mi[MREG ? K : 1][9];
+  for (int r = threadIdx.x; r <= nr; r += BLOCK) off_l[r] = a.offsets[R0 + r];
+
+  // Register on this workgroup's XCC and learn the instance's XCC layout.
+  XcdGroup xg{0, 1, 1};
+  bool ok = true;
+  if (nci > 1) {
+    if (threadIdx.x == 0) {
+      unsigned x;
+      asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+      x &= 7u;
+      __hip_atomic_fetch_add(bar + BAR_REG + x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      xinfo[0] = (int)x;
+    }
+    ok = flat_barrier(bar, nci, a.err, &bflag);
+    if (threadIdx.x == 0) {
+      int ng = 0;
+      for (int x = 0; x < 8; ++x) ng += ld_acq(bar + BAR_REG + x) != 0u;
+      xinfo[1] = (int)ld_acq(bar + BAR_REG + xinfo[0]);
+      xinfo[2] = ng;
+    }
+    __syncthreads();
+    xg = XcdGroup{xinfo[0], xinfo[1], xinfo[2]};
+  }
+  __syncthreads();
+
+  float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3];
   bool own[K], fx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -581,27 +717,25 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     fx[k] = own[k] ? (a.row_fixed[R0 + lr] != 0) : true;
   }
 
-  if (!zero_start) cg_spmv<G, true>(a, R0, nr, s, 0.f, q_lds);
-  __syncthreads();
+  if (!zero_start) {
+    cg_spmv<K, true>(a, R0, nr, t0, t1, off_l, s, 0.f, part_l, fx, qv);
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) qv[k][0] = qv[k][1] = qv[k][2] = 0.f;
+  }
   float pb[3] = {0.f, 0.f, 0.f};  // b.b, r.r, r.z
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int lr = threadIdx.x + BLOCK * k;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = qv[k][d] = 0.f;
+    for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = 0.f;
     if (own[k]) {
-      const int i = R0 + lr;
+      const int i = R0 + threadIdx.x + BLOCK * k;
       const v3 b = ld3(a.rhs, i) * s;
       const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : ld3(a.dv, i) * s;
-      const v3 q0 = zero_start ? v3{0.f, 0.f, 0.f} : v3{q_lds[3 * lr], q_lds[3 * lr + 1], q_lds[3 * lr + 2]};
       float Mk[9];
 #pragma unroll
       for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
-      if constexpr (MREG) {
-#pragma unroll
-        for (int e = 0; e < 9; ++e) Mi[k][e] = Mk[e];
-      }
-      const v3 rr0 = b - q0;
+      const v3 rr0 = b - v3{qv[k][0], qv[k][1], qv[k][2]};
       const v3 zz = mv(Mk, rr0);
       x[k][0] = x0.x; x[k][1] = x0.y; x[k][2] = x0.z;
       r[k][0] = rr0.x; r[k][1] = rr0.y; r[k][2] = rr0.z;
@@ -621,7 +755,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     a.part[4 * (size_t)c + 1] = wsum[1];
     a.part[4 * (size_t)c + 2] = wsum[2];
   }
-  bool ok = inst_barrier(bar, nci, a.err, &bflag);
+  ok = ok && hier_barrier(bar, xg, nci, a.err, &bflag);
   double tot[3];
   {
     const int sl[3] = {3, 1, 2};
@@ -638,17 +772,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     const bool live = rr > atol_sq;
     if (!live && a.early_exit) break;
     // Phase A: q = A p_new with p_new = z + beta p (gathered on the fly).
-    cg_spmv<G, false>(a, R0, nr, s, beta, q_lds);
-    __syncthreads();
+    cg_spmv<K, false>(a, R0, nr, t0, t1, off_l, s, beta, part_l, fx, qv);
     float pp[1] = {0.f};
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (own[k]) {
-        const int lr = threadIdx.x + BLOCK * k;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           p[k][d] = fmaf(beta, p[k][d], z[k][d]);
-          qv[k][d] = q_lds[3 * lr + d];
           pp[0] += p[k][d] * qv[k][d];
         }
       }
@@ -656,7 +787,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     double w1[1];
     wg_sum<1>(pp, w1, red);
     if (threadIdx.x == 0) a.part[4 * (size_t)c] = w1[0];
-    ok = inst_barrier(bar, nci, a.err, &bflag);
+    ok = hier_barrier(bar, xg, nci, a.err, &bflag);
     if (!ok) break;
     double pAp[1];
     {
@@ -676,13 +807,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
           r[k][d] = fmaf(-alpha, qv[k][d], r[k][d]);
         }
         float Mk[9];
-        if constexpr (MREG) {
 #pragma unroll
-          for (int e = 0; e < 9; ++e) Mk[e] = Mi[k][e];
-        } else {
-#pragma unroll
-          for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
-        }
+        for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
         const v3 rv = {r[k][0], r[k][1], r[k][2]};
         const v3 zz = mv(Mk, rv);
         z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
@@ -698,7 +824,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
       a.part[4 * (size_t)c + 1] = w2[0];
       a.part[4 * (size_t)c + 2] = w2[1];
     }
-    ok = inst_barrier(bar, nci, a.err, &bflag);
+    ok = hier_barrier(bar, xg, nci, a.err, &bflag);
     if (!ok) break;
     double t2[2];
     {
@@ -797,13 +923,13 @@ struct LaunchRhs {
   }
 };
 
-const void* cg_kernel_ptr(int g, int k) {
-#define FEM_CG_CASE(G_, K_) \
-  if (g == G_ && k == K_) return (const void*)k_cg<G_, K_>;
-#define FEM_CG_ROW(G_) FEM_CG_CASE(G_, 1) FEM_CG_CASE(G_, 2) FEM_CG_CASE(G_, 4) FEM_CG_CASE(G_, 8)
-  FEM_CG_ROW(4) FEM_CG_ROW(8) FEM_CG_ROW(16) FEM_CG_ROW(32) FEM_CG_ROW(64)
-#undef FEM_CG_ROW
-#undef FEM_CG_CASE
+const void* cg_kernel_ptr(int k) {
+  switch (k) {
+    case 1: return (const void*)k_cg<1>;
+    case 2: return (const void*)k_cg<2>;
+    case 4: return (const void*)k_cg<4>;
+    case 8: return (const void*)k_cg<8>;
+  }
   return nullptr;
 }
 
@@ -833,7 +959,8 @@ struct fem_system {
   unsigned *d_rhs_max = nullptr, *d_bar = nullptr;
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
-  int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr;
+  int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
+      *d_tile_rows = nullptr;
   int nchunks = 0, cg_k = 1;
   // bound
   int* b_offsets = nullptr;
@@ -900,88 +1027,130 @@ struct fem_system {
     return a;
   }
 
-  // Row partition of the persistent CG: per instance, contiguous chunks.
+  // Row partition of the persistent CG: per instance, contiguous chunks
+  // (at most 8*BLOCK rows each), row-aligned tiles inside every chunk.  All
+  // state changes are committed only after every check has passed.
   int plan_cg() {
     int dev = 0, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const long max_rows_chunk = 8L * BLOCK;
     std::vector<int> per(ninst, 1);
+    std::vector<double> cost(ninst);
     long need = 0;
+    double total = 0;
     for (int k = 0; k < ninst; ++k) {
       long rows = inst_off[k + 1] - inst_off[k];
       per[k] = (int)std::max(1L, (rows + max_rows_chunk - 1) / max_rows_chunk);
       need += per[k];
+      cost[k] = (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]) + 8.0 * rows;
+      total += cost[k];
     }
     // Default chunk count: about 2 rows per thread for large meshes, one
     // workgroup for small ones; cg_wg_request overrides.
     long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, (long)P / (2 * BLOCK));
     target = std::max(target, need);
-    // Distribute extra chunks proportionally to instance cost.
-    if (target > need) {
-      double total = 0;
-      for (int k = 0; k < ninst; ++k) total += (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]);
+    if (target > need && total > 0) {
       long extra = target - need;
-      for (int k = 0; k < ninst && total > 0; ++k) {
-        double share = (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]) / total;
+      for (int k = 0; k < ninst; ++k) {
         long rows = inst_off[k + 1] - inst_off[k];
-        long add = (long)std::floor(share * extra);
-        per[k] = (int)std::min<long>(std::max<long>(per[k], per[k] + add), std::max<long>(rows, 1));
+        long add = (long)std::floor(cost[k] / total * extra);
+        per[k] = (int)std::min<long>(per[k] + add, std::max<long>(rows, 1));
       }
     }
-    long n = 0;
-    for (int k = 0; k < ninst; ++k) n += per[k];
-    if (n > 8 && (n & 7)) {  // pad to a multiple of 8 (XCD swizzle), on the largest instance
-      int big = (int)(std::max_element(per.begin(), per.end()) - per.begin());
-      long rows = inst_off[big + 1] - inst_off[big];
-      long pad = 8 - (n & 7);
-      if (per[big] + pad <= rows) {
-        per[big] += (int)pad;
-        n += pad;
-      }
-    }
-    std::vector<int32_t> chunk_rows, chunk_inst, inst_chunks(1, 0);
+    auto count = [&]() {
+      long n = 0;
+      for (int v : per) n += v;
+      return n;
+    };
+    std::vector<int32_t> chunk_rows, chunk_inst, inst_chunks;
     int maxrows = 0;
-    for (int k = 0; k < ninst; ++k) {
-      std::vector<int32_t> b;
-      split_rows(pt.offsets, inst_off[k], inst_off[k + 1], per[k], b);
-      for (int j = 0; j < per[k]; ++j) {
-        chunk_rows.push_back(b[j]);
-        chunk_inst.push_back(k);
-        maxrows = std::max(maxrows, b[j + 1] - b[j]);
+    for (int attempt = 0;; ++attempt) {
+      long n = count();
+      if (n > 8 && (n & 7)) {  // pad to a multiple of 8 (XCD swizzle) on the largest instance
+        int big = (int)(std::max_element(cost.begin(), cost.end()) - cost.begin());
+        long rows = inst_off[big + 1] - inst_off[big];
+        long pad = 8 - (n & 7);
+        if (per[big] + pad <= rows) per[big] += (int)pad;
       }
-      inst_chunks.push_back((int)chunk_rows.size());
+      chunk_rows.clear();
+      chunk_inst.clear();
+      inst_chunks.assign(1, 0);
+      maxrows = 0;
+      int worst = -1;
+      for (int k = 0; k < ninst; ++k) {
+        std::vector<int32_t> bnd;
+        split_rows(pt.offsets, inst_off[k], inst_off[k + 1], per[k], bnd);
+        for (int j = 0; j < per[k]; ++j) {
+          chunk_rows.push_back(bnd[j]);
+          chunk_inst.push_back(k);
+          if (bnd[j + 1] - bnd[j] > maxrows) {
+            maxrows = bnd[j + 1] - bnd[j];
+            worst = k;
+          }
+        }
+        inst_chunks.push_back((int)chunk_rows.size());
+      }
+      if (maxrows <= max_rows_chunk) break;
+      if (attempt > 64) return fail(FEM_ERR_UNSUPPORTED, "cannot split the CG rows into chunks of <= 2048 rows");
+      per[worst] += 1;
     }
     chunk_rows.push_back(P);
-    nchunks = (int)chunk_inst.size();
-    cg_k = maxrows <= BLOCK ? 1 : maxrows <= 2 * BLOCK ? 2 : maxrows <= 4 * BLOCK ? 4 : 8;
-    if (maxrows > 8 * BLOCK) return fail(FEM_ERR_UNSUPPORTED, "CG chunk exceeds 2048 rows");
-    const void* kp = cg_kernel_ptr(cg_g, cg_k);
-    if (!kp) return fail(FEM_ERR_ARG, "unsupported CG lanes per row");
+    const int n_chunks = (int)chunk_inst.size();
+    // Row-aligned tiles of at most CG_TILE blocks inside every chunk.
+    std::vector<int32_t> chunk_tiles(1, 0), tile_rows;
+    for (int c = 0; c < n_chunks; ++c) {
+      int r = chunk_rows[c];
+      const int r1 = chunk_rows[c + 1];
+      while (r < r1) {
+        tile_rows.push_back(r);
+        int e = r;
+        while (e < r1 && pt.offsets[e + 1] - pt.offsets[r] <= CG_TILE) ++e;
+        if (e == r) return fail(FEM_ERR_UNSUPPORTED, "row longer than a CG tile");
+        r = e;
+      }
+      chunk_tiles.push_back((int)tile_rows.size());
+    }
+    const int k_rows = maxrows <= BLOCK ? 1 : maxrows <= 2 * BLOCK ? 2 : maxrows <= 4 * BLOCK ? 4 : 8;
+    const void* kp = cg_kernel_ptr(k_rows);
+    if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, BLOCK, 0));
-    max_resident = nb * ncu;
-    if (nchunks > max_resident) {
+    if (n_chunks > nb * ncu) {
       char buf[160];
-      std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups, device holds %d", nchunks, max_resident);
+      std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups, device holds %d", n_chunks, nb * ncu);
       return fail(FEM_ERR_UNSUPPORTED, buf);
     }
-    for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part}) {
+    int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr;
+    double* n_part = nullptr;
+    int rc;
+    if ((rc = upload(&n_rows, chunk_rows.data(), chunk_rows.size())) ||
+        (rc = upload(&n_inst, chunk_inst.data(), chunk_inst.size())) ||
+        (rc = upload(&n_ichunks, inst_chunks.data(), inst_chunks.size())) ||
+        (rc = upload(&n_tiles, chunk_tiles.data(), chunk_tiles.size())) ||
+        (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 4 * (size_t)n_chunks)))
+      return rc;
+    for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
+                    (void*)d_chunk_tiles, (void*)d_tile_rows}) {
       if (p) {
-        hipFree(p);
+        (void)hipFree(p);
         allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
       }
     }
-    int rc;
-    if ((rc = upload(&d_chunk_rows, chunk_rows.data(), chunk_rows.size()))) return rc;
-    if ((rc = upload(&d_chunk_inst, chunk_inst.data(), chunk_inst.size()))) return rc;
-    if ((rc = upload(&d_inst_chunks, inst_chunks.data(), inst_chunks.size()))) return rc;
-    if ((rc = alloc(&d_part, 4 * (size_t)nchunks))) return rc;
+    d_chunk_rows = n_rows;
+    d_chunk_inst = n_inst;
+    d_inst_chunks = n_ichunks;
+    d_chunk_tiles = n_tiles;
+    d_tile_rows = n_trows;
+    d_part = n_part;
+    nchunks = n_chunks;
+    cg_k = k_rows;
+    max_resident = nb * ncu;
     return FEM_OK;
   }
 
   ~fem_system() {
-    for (void* p : allocs) hipFree(p);
+    for (void* p : allocs) (void)hipFree(p);
   }
 };
 
@@ -1071,7 +1240,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_row_fixed, fixed_mask.data(), fixed_mask.size())) ||
       (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
       (rc = s->alloc(&s->d_z, 3 * (size_t)P)) || (rc = s->alloc(&s->d_p, 3 * (size_t)P)) ||
-      (rc = s->alloc(&s->d_rhs_max, s->ninst)) || (rc = s->alloc(&s->d_bar, 32 * (size_t)s->ninst)) ||
+      (rc = s->alloc(&s->d_rhs_max, s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
       (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)))
     return cleanup(rc);
   double avg_inc = P ? (double)(s->pt.tinc_ptr[P] + s->pt.hinc_ptr[P]) / P : 1.0;
@@ -1131,10 +1300,18 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t cg_lanes, int32_t 
   if (elem_lanes && !ok(elem_lanes)) return fail(FEM_ERR_ARG, "elem_lanes must be a power of two in [4, 64]");
   if (cg_lanes && !ok(cg_lanes)) return fail(FEM_ERR_ARG, "cg_lanes must be a power of two in [4, 64]");
   if (cg_workgroups < 0) return fail(FEM_ERR_ARG, "cg_workgroups must be >= 0");
+  const int old_request = s->cg_wg_request;
+  s->cg_wg_request = cg_workgroups;
+  if (s->P > 0) {
+    int rc = s->plan_cg();
+    if (rc) {
+      s->cg_wg_request = old_request;
+      return rc;
+    }
+  }
   if (elem_lanes) s->elem_g = elem_lanes;
   if (cg_lanes) s->cg_g = cg_lanes;
-  s->cg_wg_request = cg_workgroups;
-  return s->P > 0 ? s->plan_cg() : FEM_OK;
+  return FEM_OK;
 }
 
 int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* cg_lanes, int32_t* cg_workgroups,
@@ -1176,9 +1353,9 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
 
-int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float* ext, const float* act, float* q_out,
-             float* qd_out, float dt, void* stream) {
-  if (!s || !q_in || !qd_in || !f_out || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const float* ext, const float* act, float dt,
+                 void* stream) {
+  if (!s || !q_in || !qd_in || !f_out) return fail(FEM_ERR_ARG, "null argument");
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
@@ -1186,14 +1363,21 @@ int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float
   {
     int n = std::max(std::max(s->nfixed, s->ninst), 1);
     hipLaunchKernelGGL(k_prepare, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_fixed, s->nfixed,
-                       s->d_anchor, q_in, qd_in, s->d_rhs_max, s->d_bar, s->ninst, s->d_err);
+                       s->d_anchor, q_in, qd_in, s->d_rhs_max, s->ninst);
     HIP_TRY(hipGetLastError());
   }
   RowArgs ra = s->row_args();
   const int terms = FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
-  int rc = dispatch_g<LaunchRhs>(s->elem_g, ra, (const float*)q_in, (const float*)qd_in, act, ext,
-                                 (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
-  if (rc) return rc;
+  return dispatch_g<LaunchRhs>(s->elem_g, ra, (const float*)q_in, (const float*)qd_in, act, ext,
+                               (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
+}
+
+int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
+                   float* qd_out, float dt, void* stream) {
+  if (!s || !q_in || !qd_in || !f_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
   CgArgs ca;
   ca.P = s->P;
   ca.offsets = s->d_offsets;
@@ -1201,7 +1385,7 @@ int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float
   ca.vals = s->b_vals;
   ca.minv = s->b_minv;
   ca.row_fixed = s->d_row_fixed;
-  ca.rhs = f_out;
+  ca.rhs = f_in;
   ca.dv = s->b_dv;
   ca.q_in = q_in;
   ca.qd_in = qd_in;
@@ -1212,6 +1396,8 @@ int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float
   ca.zg = s->d_z;
   ca.pg = s->d_p;
   ca.chunk_rows = s->d_chunk_rows;
+  ca.chunk_tiles = s->d_chunk_tiles;
+  ca.tile_rows = s->d_tile_rows;
   ca.chunk_inst = s->d_chunk_inst;
   ca.inst_chunks = s->d_inst_chunks;
   ca.rhs_max = s->d_rhs_max;
@@ -1222,11 +1408,22 @@ int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float
   ca.bar = s->d_bar;
   ca.stats = s->d_stats;
   ca.err = s->d_err;
-  const void* kp = cg_kernel_ptr(s->cg_g, s->cg_k);
+  const void* kp = cg_kernel_ptr(s->cg_k);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
+  hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, s->d_bar,
+                     s->ninst, s->d_err);
+  HIP_TRY(hipGetLastError());
   void* args[] = {&ca};
-  HIP_TRY(hipLaunchKernel(kp, dim3(s->nchunks), dim3(BLOCK), args, 0, st));
+  HIP_TRY(hipLaunchKernel(kp, dim3(s->nchunks), dim3(BLOCK), args, 0, (hipStream_t)stream));
   return FEM_OK;
+}
+
+int fem_step(fem_system* s, float* q_in, float* qd_in, float* f_out, const float* ext, const float* act, float* q_out,
+             float* qd_out, float dt, void* stream) {
+  if (!q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+  int rc = fem_step_rhs(s, q_in, qd_in, f_out, ext, act, dt, stream);
+  if (rc) return rc;
+  return fem_step_solve(s, q_in, qd_in, f_out, q_out, qd_out, dt, stream);
 }
 
 int fem_solve_stats(fem_system* s, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,

@@ -74,6 +74,8 @@ SIGNATURES = {
     "fem_restore_fixed": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "fem_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]),
     "fem_step": (ctypes.c_int, [ctypes.c_void_p] * 8 + [ctypes.c_float, ctypes.c_void_p]),
+    "fem_step_rhs": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
+    "fem_step_solve": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_eval_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 5),
     "fem_solve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _ip, _fp, _fp, _ip]),
 }
