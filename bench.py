@@ -87,8 +87,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 512 MiB device copy (scale reference for PMC byte counters)")
+    ap.add_argument("--skip-tol-run", action="store_true",
+                    help="skip the extra early-exit substep (profiled runs keep only fixed-budget CG launches)")
     ap.add_argument("--cg-workgroups", type=int, default=0)
-    ap.add_argument("--cg-lanes", type=int, default=0)
+    ap.add_argument("--assemble-rows", type=int, default=0)
     ap.add_argument("--elem-lanes", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -107,8 +109,8 @@ def main():
     solver = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag",
                        linear_solver_maxiter=CG_ITERS, linear_solver_tolerance=1e-2, self_contact_ke=0.0,
                        fixed_particle_indices=scene.fixed, linear_solver_use_cuda_graph=False)
-    if args.cg_workgroups or args.cg_lanes or args.elem_lanes:
-        solver.set_launch(args.elem_lanes, args.cg_lanes, args.cg_workgroups)
+    if args.cg_workgroups or args.assemble_rows or args.elem_lanes:
+        solver.set_launch(args.elem_lanes, args.assemble_rows, args.cg_workgroups)
     lib, sysh = solver._lib, solver._sys
     s0, s1 = model.state(), model.state()
     stream = torch.cuda.current_stream(dev)
@@ -152,10 +154,12 @@ def main():
         raise RuntimeError("non-finite state")
 
     # Same workload with the device residual exit: iterations actually needed at tol 1e-2.
-    solver.linear_solver_use_cuda_graph = True
-    solver._push_params()
-    substep()
-    early_iters = solver.solve_stats()["iterations"][0]
+    early_iters = None
+    if not args.skip_tol_run:
+        solver.linear_solver_use_cuda_graph = True
+        solver._push_params()
+        substep()
+        early_iters = solver.solve_stats()["iterations"][0]
 
     if args.copy_probe:
         a = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)

@@ -110,10 +110,11 @@ int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float*
 
 int fem_set_params(fem_system* sys, const fem_params* params);
 
-/* Launch shapes: lanes per block row in the element/assembly kernels and in
- * the CG SpMV (4..64, power of two), and the CG workgroup count (0 = auto). */
-int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t cg_lanes, int32_t cg_workgroups);
-int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* cg_lanes, int32_t* cg_workgroups,
+/* Launch shapes (0 keeps the current value): lanes per block row of the
+ * force/RHS kernel and rows per workgroup of the assembly (powers of two in
+ * [4, 64]), and the persistent CG workgroup count (0 = automatic). */
+int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups);
+int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
                    int32_t* cg_rows_per_thread);
 
 /* FixedParticles.restore / restore_positions (fixed_particles.py:11-21,

@@ -37,6 +37,8 @@ constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
 // every counter on its own 64-byte line).
 constexpr int BAR_WORDS = 512;
 constexpr int BAR_TOP_COUNT = 0, BAR_TOP_GEN = 16, BAR_XCD = 32, BAR_REG = 320;
+// Per-instance words holding max|b| (reduced by the solve).
+constexpr int RHS_MAX_SLOTS = 64;
 
 thread_local std::string g_error;
 
@@ -69,6 +71,8 @@ struct RowArgs {
   const int* hinc_ptr;
   const int* hinc;
   const uint32_t* hslots;
+  const int* slot_ptr;
+  const uint16_t* contrib;
   const int* tet_idx;
   const float* tet_pose;
   const float* tet_mat;
@@ -103,7 +107,7 @@ __global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float
     st3(q, i, ld3(anchor, i));
     if (qd) st3(qd, i, v3{0.f, 0.f, 0.f});
   }
-  if (t < ninst) rhs_max[t] = 0u;
+  if (t < ninst * RHS_MAX_SLOTS) rhs_max[t] = 0u;
 }
 
 // Per-solve reset of the CG barrier counters / XCC registration and the
@@ -124,41 +128,45 @@ __global__ void k_cg_reset(unsigned* bar, int ninst, int* err) {
 // ---------------------------------------------------------------------------
 // Hex8 helpers operating on one incidence (element e, local corner la).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __restrict__ x,
-                                             const float* __restrict__ v, int e, int la) {
-  int nd[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) nd[b] = a.hex_idx[8 * (size_t)e + b];
-  v3 xs[8], vs[8];
+__device__ __forceinline__ void hex_gather(const int* __restrict__ idx, const float* __restrict__ x, float px[8],
+                                           float py[8], float pz[8]) {
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    xs[b] = ld3(x, nd[b]);
-    vs[b] = ld3(v, nd[b]);
+    const int n = idx[b];
+    px[b] = x[3 * (size_t)n];
+    py[b] = x[3 * (size_t)n + 1];
+    pz[b] = x[3 * (size_t)n + 2];
   }
-  float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1], k_damp = a.hex_mat[3 * e + 2];
-  float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
+}
+
+__device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __restrict__ x,
+                                             const float* __restrict__ v, int e, int la) {
+  float px[8], py[8], pz[8], vx[8], vy[8], vz[8];
+  hex_gather(a.hex_idx + 8 * (size_t)e, x, px, py, pz);
+  hex_gather(a.hex_idx + 8 * (size_t)e, v, vx, vy, vz);
+  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1], k_damp = a.hex_mat[3 * e + 2];
+  const float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
   v3 f = {0.f, 0.f, 0.f};
   for (int g = 0; g < 8; ++g) {
-    v3 dN[8];
-    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, dN);
+    float gx[8], gy[8], gz[8];
+    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
     float F[9], dF[9];
-    hex_F(xs, dN, F);
+    hex_F(px, py, pz, gx, gy, gz, F);
     clamp_stretch(F, a.mn, a.mx);
-    hex_F(vs, dN, dF);
+    hex_F(vx, vy, vz, gx, gy, gz, dF);
     float ic = 0.f;
 #pragma unroll
     for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
-    float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
+    const float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
     float cof[9];
     cofactor(F, cof);
-    float J = fmaxf(det3(F), 0.01f);
-    float vol = k_lambda * (J - alpha);
+    const float J = fmaxf(det3(F), 0.01f);
+    const float vol = k_lambda * (J - alpha);
     float P[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) P[k] = F[k] * sdev + dF[k] * k_damp + cof[k] * vol;
-    float w = a.hex_detw[8 * (size_t)e + g];
-    v3 dNa = (la == 0) ? dN[0] : (la == 1) ? dN[1] : (la == 2) ? dN[2] : (la == 3) ? dN[3]
-           : (la == 4) ? dN[4] : (la == 5) ? dN[5] : (la == 6) ? dN[6] : dN[7];
+    const float w = a.hex_detw[8 * (size_t)e + g];
+    const v3 dNa = hex_grad_corner(a.hex_invJ + 72 * (size_t)e + 9 * g, g, la);
     f = f - mv(P, dNa) * w;
   }
   return f;
@@ -167,49 +175,37 @@ __device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __re
 // Row `la` of the Hex8 tangent: K_ab, b = 0..7, summed over Gauss points, x h^2.
 __device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* __restrict__ x, int e, int la,
                                                 float h2, float K[8][9]) {
-  int nd[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) nd[b] = a.hex_idx[8 * (size_t)e + b];
-  v3 xs[8];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) xs[b] = ld3(x, nd[b]);
-  float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1];
+  float px[8], py[8], pz[8];
+  hex_gather(a.hex_idx + 8 * (size_t)e, x, px, py, pz);
+  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1];
 #pragma unroll
   for (int b = 0; b < 8; ++b)
 #pragma unroll
     for (int k = 0; k < 9; ++k) K[b][k] = 0.f;
   for (int g = 0; g < 8; ++g) {
-    v3 dN[8];
-    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, dN);
+    float gx[8], gy[8], gz[8];
+    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
     float F[9];
-    hex_F(xs, dN, F);
+    hex_F(px, py, pz, gx, gy, gz, F);
     clamp_stretch(F, a.mn, a.mx);
     float ic = 0.f;
 #pragma unroll
     for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
-    float ic1 = fmaxf(ic + 1.f, 1e-6f);
-    float s = 1.f - 1.f / ic1;
-    float coef = k_mu * 2.f / (ic1 * ic1);
+    const float ic1 = fmaxf(ic + 1.f, 1e-6f);
+    const float s = 1.f - 1.f / ic1;
+    const float coef = k_mu * 2.f / (ic1 * ic1);
     float cof[9];
     cofactor(F, cof);
-    float w = a.hex_detw[8 * (size_t)e + g];
-    v3 dNa = (la == 0) ? dN[0] : (la == 1) ? dN[1] : (la == 2) ? dN[2] : (la == 3) ? dN[3]
-           : (la == 4) ? dN[4] : (la == 5) ? dN[5] : (la == 6) ? dN[6] : dN[7];
-    v3 fa = mv(F, dNa), ja = mv(cof, dNa);
-    const float fa3[3] = {fa.x, fa.y, fa.z}, ja3[3] = {ja.x, ja.y, ja.z};
+    const float w = a.hex_detw[8 * (size_t)e + g];
+    const v3 dNa = hex_grad_corner(a.hex_invJ + 72 * (size_t)e + 9 * g, g, la);
+    const v3 fa = mv(F, dNa), ja = mv(cof, dNa);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      v3 fb = mv(F, dN[b]), jb = mv(cof, dN[b]);
-      float dab = dot(dNa, dN[b]);
-      const float fb3[3] = {fb.x, fb.y, fb.z}, jb3[3] = {jb.x, jb.y, jb.z};
+      const v3 dNb = {gx[b], gy[b], gz[b]};
+      float blk[9];
+      tangent_block(blk, k_mu * s, coef, k_lambda, w, dNa, fa, ja, dNb, mv(F, dNb), mv(cof, dNb));
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          float kdev = ((r == c) ? k_mu * s * dab : 0.f) + fa3[r] * fb3[c] * coef;
-          float kvol = ja3[r] * jb3[c] * k_lambda;
-          K[b][3 * r + c] = K[b][3 * r + c] + (kdev + kvol) * w;
-        }
+      for (int k = 0; k < 9; ++k) K[b][k] += blk[k];
     }
   }
 #pragma unroll
@@ -227,78 +223,107 @@ __device__ __forceinline__ float group_sum(float v) {
 
 // ---------------------------------------------------------------------------
 // k_assemble: A = M + h^2 K at positions x, and M^-1 = inv(diag blocks).
+// Atomic-free and deterministic.  A workgroup owns RB consecutive rows.
+// Phase 1: every incidence (element, corner) of those rows computes its
+// element-tangent row (4 blocks for a Tet4, 8 for a Hex8) into LDS staging
+// with plain stores.  Phase 2: one thread per BSR slot of the rows sums, in
+// incidence order, the staged blocks whose precomputed slot byte matches,
+// adds the lumped mass on the diagonal, writes the slot (consecutive threads
+// -> consecutive 36-byte blocks) and inverts the diagonal block.
 // ---------------------------------------------------------------------------
-template <int G>
+// Staging strides (floats) per incidence: odd so that consecutive incidences
+// fall in different LDS banks.
+constexpr int ST_TET = 37, ST_HEX = 73;
+
+template <int RB>
 __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const float* __restrict__ x, float h2,
-                                                    float* __restrict__ vals, float* __restrict__ minv, int maxnnz) {
+                                                    float* __restrict__ vals, float* __restrict__ minv) {
   extern __shared__ float lds[];
-  constexpr int NG = BLOCK / G;
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  __shared__ int roff[RB + 1], rtb[RB], rhb[RB], rdiag[RB];
+  __shared__ float rmass[RB];
   const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int i = blk * NG + grp;
-  const bool active = i < a.P;
-  float* acc = lds + (size_t)grp * maxnnz * 9;
-  int k0 = 0, nnz = 0, drel = -1;
-  float m = 0.f;
-  if (active) {
-    k0 = a.offsets[i];
-    nnz = a.offsets[i + 1] - k0;
-    drel = a.diag_rel[i];
-    m = a.mass[i];
+  const int r0 = blk * RB;
+  if (r0 >= a.P) return;  // uniform over the workgroup
+  const int r1 = min(r0 + RB, a.P);
+  const int nr = r1 - r0;
+  const int tb = a.tinc_ptr[r0], nt = a.tinc_ptr[r1] - tb;
+  const int hb = a.hinc_ptr[r0], nh = a.hinc_ptr[r1] - hb;
+  const int kb = a.offsets[r0], ke = a.offsets[r1];
+  const int cb = a.slot_ptr[kb], nc = a.slot_ptr[ke] - cb;
+  float* st_t = lds;                                                  // [nt][ST_TET]: 4 blocks
+  float* st_h = st_t + ST_TET * (size_t)nt;                           // [nh][ST_HEX]: 8 blocks
+  int* sptr = reinterpret_cast<int*>(st_h + ST_HEX * (size_t)nh);     // [ke-kb+1] slot -> first contribution
+  uint16_t* codes = reinterpret_cast<uint16_t*>(sptr + (ke - kb + 1));  // [nc]
+  if (threadIdx.x <= nr) roff[threadIdx.x] = a.offsets[r0 + threadIdx.x] - kb;
+  if (threadIdx.x < nr) {
+    const int i = r0 + threadIdx.x;
+    rtb[threadIdx.x] = a.tinc_ptr[i] - tb;
+    rhb[threadIdx.x] = a.hinc_ptr[i] - hb;
+    rdiag[threadIdx.x] = a.diag_rel[i];
+    rmass[threadIdx.x] = a.mass[i];
   }
-  for (int f = lane; f < nnz * 9; f += G) {
-    int blkid = f / 9, c = f - 9 * blkid;
-    acc[f] = (blkid == drel && (c == 0 || c == 4 || c == 8)) ? m : 0.f;
+  for (int k = threadIdx.x; k <= ke - kb; k += BLOCK) sptr[k] = a.slot_ptr[kb + k] - cb;
+  for (int j = threadIdx.x; j < nc; j += BLOCK) codes[j] = a.contrib[cb + j];
+  for (int t = threadIdx.x; t < nt; t += BLOCK) {
+    const int code = a.tinc[tb + t];
+    const int e = code >> 2, la = code & 3;
+    const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+    TetGeo g;
+    tet_geo(ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w), a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+    float K[4][9];
+    tet_tangent_row(g, a.tet_mat[3 * e], a.tet_mat[3 * e + 1], h2, la, K);
+    float* o = st_t + ST_TET * t;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) o[9 * b + c] = K[b][c];
   }
-  __syncthreads();
-  if (active) {
-    const int t1 = a.tinc_ptr[i + 1];
-    for (int t = a.tinc_ptr[i] + lane; t < t1; t += G) {
-      const int code = a.tinc[t];
-      const int e = code >> 2, la = code & 3;
-      const uint32_t sl = a.tslots[t];
-      const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
-      const v3 xs[4] = {ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w)};
-      TetGeo g;
-      tet_geo(xs, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
-      float K[4][9];
-      tet_tangent_row(g, a.tet_mat[3 * e], a.tet_mat[3 * e + 1], h2, la, K);
+  for (int t = threadIdx.x; t < nh; t += BLOCK) {
+    const int code = a.hinc[hb + t];
+    float K[8][9];
+    hex_tangent_row(a, x, code >> 3, code & 7, h2, K);
+    float* o = st_h + ST_HEX * t;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        float* dst = acc + 9 * ((sl >> (8 * b)) & 255u);
+    for (int b = 0; b < 8; ++b)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) atomicAdd(dst + c, K[b][c]);
-      }
-    }
-    const int h1 = a.hinc_ptr[i + 1];
-    for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
-      const int code = a.hinc[t];
-      const int e = code >> 3, la = code & 7;
-      const uint32_t s0 = a.hslots[2 * (size_t)t], s1 = a.hslots[2 * (size_t)t + 1];
-      float K[8][9];
-      hex_tangent_row(a, x, e, la, h2, K);
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        uint32_t rel = (b < 4) ? ((s0 >> (8 * b)) & 255u) : ((s1 >> (8 * (b - 4))) & 255u);
-        float* dst = acc + 9 * rel;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) atomicAdd(dst + c, K[b][c]);
-      }
-    }
+      for (int c = 0; c < 9; ++c) o[9 * b + c] = K[b][c];
   }
   __syncthreads();
-  if (active) {
-    float* out = vals + (size_t)k0 * 9;
-    for (int f = lane; f < nnz * 9; f += G) out[f] = acc[f];
-    if (lane == 0) {
-      float D[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-      if (drel >= 0)
+  for (int kk = threadIdx.x; kk < ke - kb; kk += BLOCK) {
+    int lr = 0;
+    while (roff[lr + 1] <= kk) ++lr;
+    const bool diag = (kk - roff[lr]) == rdiag[lr];
+    float acc[9];
+    const float m = diag ? rmass[lr] : 0.f;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) D[c] = acc[9 * drel + c];
+    for (int c = 0; c < 9; ++c) acc[c] = (c == 0 || c == 4 || c == 8) ? m : 0.f;
+    const int tbase = rtb[lr], hbase = rhb[lr];
+    const int j1 = sptr[kk + 1];
+    for (int j = sptr[kk]; j < j1; ++j) {
+      const uint32_t code = codes[j];
+      const uint32_t t = (code >> 3) & 4095u, b = code & 7u;
+      const float* src = (code & 0x8000u) ? st_h + ST_HEX * (hbase + t) + 9 * b : st_t + ST_TET * (tbase + t) + 9 * b;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) acc[c] += src[c];
+    }
+    float* out = vals + 9 * (size_t)(kb + kk);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) out[c] = acc[c];
+    if (diag) {
       float inv[9];
-      inverse_diagonal_block(D, inv);
+      inverse_diagonal_block(acc, inv);
 #pragma unroll
-      for (int c = 0; c < 9; ++c) minv[9 * (size_t)i + c] = inv[c];
+      for (int c = 0; c < 9; ++c) minv[9 * (size_t)(r0 + lr) + c] = inv[c];
+    }
+  }
+  // Rows without a stored diagonal use the identity (kernels.py:1840-1843).
+  for (int lr = threadIdx.x; lr < nr; lr += BLOCK) {
+    if (rdiag[lr] < 0) {
+      const float eye[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+      float inv[9];
+      inverse_diagonal_block(eye, inv);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) minv[9 * (size_t)(r0 + lr) + c] = inv[c];
     }
   }
 }
@@ -326,13 +351,12 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
         const int code = a.tinc[t];
         const int e = code >> 2, la = code & 3;
         const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
-        const v3 xs[4] = {ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w)};
-        const v3 vs[4] = {ld3(v, nd.x), ld3(v, nd.y), ld3(v, nd.z), ld3(v, nd.w)};
+        const v3 x0 = ld3(x, nd.x), x1 = ld3(x, nd.y), x2 = ld3(x, nd.z), x3 = ld3(x, nd.w);
         const float mu = a.tet_mat[3 * e], lam = a.tet_mat[3 * e + 1], kd = a.tet_mat[3 * e + 2];
         TetGeo g;
-        tet_geo(xs, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
-        ft = ft + tet_node_force(g, vs, mu, lam, kd, la);
-        if (a.use_act && act) ft = ft + tet_activation_force(xs, lam, act[e], la);
+        tet_geo(x0, x1, x2, x3, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+        ft = ft + tet_node_force(g, ld3(v, nd.x), ld3(v, nd.y), ld3(v, nd.z), ld3(v, nd.w), mu, lam, kd, la);
+        if (a.use_act && act) ft = ft + tet_activation_force(x0, x1, x2, x3, lam, act[e], la);
       }
     }
     if (terms & FEM_TERM_HEX) {
@@ -405,13 +429,22 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     st3(out, i, f);
   }
   if (mode == MODE_STEP) {
-    // maximum_rhs_component kernels.py:1779-1782, per instance.
+    // maximum_rhs_component kernels.py:1779-1782, per instance, spread over
+    // RHS_MAX_SLOTS words so that no address sees more than a few atomics.
     if (a.row_inst == nullptr) {
+      __shared__ float wmax[BLOCK / 64];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) mval = fmaxf(mval, __shfl_xor(mval, o));
-      if ((threadIdx.x & 63) == 0 && mval > 0.f) atomicMax(rhs_max, __float_as_uint(mval));
+      if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mval;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float m = wmax[0];
+#pragma unroll
+        for (int w = 1; w < BLOCK / 64; ++w) m = fmaxf(m, wmax[w]);
+        if (m > 0.f) atomicMax(rhs_max + (blk & (RHS_MAX_SLOTS - 1)), __float_as_uint(m));
+      }
     } else if (active && lane == 0 && mval > 0.f) {
-      atomicMax(rhs_max + a.row_inst[i], __float_as_uint(mval));
+      atomicMax(rhs_max + RHS_MAX_SLOTS * a.row_inst[i] + (i & (RHS_MAX_SLOTS - 1)), __float_as_uint(mval));
     }
   }
 }
@@ -679,7 +712,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
   const int t0 = a.chunk_tiles[c], t1 = a.chunk_tiles[c + 1];
   unsigned* bar = a.bar + BAR_WORDS * inst;
-  const float m = __uint_as_float(a.rhs_max[inst]);
+  float m = 0.f;
+  for (int k = 0; k < RHS_MAX_SLOTS; ++k) m = fmaxf(m, __uint_as_float(a.rhs_max[RHS_MAX_SLOTS * inst + k]));
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
   const bool zero_start = (m == 0.f);
 
@@ -893,20 +927,19 @@ int padded_blocks(int rows, int rows_per_block) {
   return std::max(n, 1);
 }
 
-template <int G>
+template <int RB>
 struct LaunchAssemble {
-  static int run(const RowArgs& ra, const float* x, float h2, float* vals, float* minv, int maxnnz,
-                 hipStream_t st) {
-    size_t lds = (size_t)(BLOCK / G) * maxnnz * 9 * sizeof(float);
-    if (lds > 160 * 1024) return fail(FEM_ERR_UNSUPPORTED, "assembly LDS exceeds 160 KiB; use more lanes per row");
-    static bool attr = false;
-    if (!attr) {
-      HIP_TRY(hipFuncSetAttribute((const void*)k_assemble<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024));
-      attr = true;
+  static int run(const RowArgs& ra, const float* x, float h2, float* vals, float* minv, size_t lds, hipStream_t st) {
+    // The static roff[] array shares the 160 KiB with the dynamic staging.
+    const size_t cap = 160 * 1024 - 1024;
+    if (lds > cap) return fail(FEM_ERR_UNSUPPORTED, "assembly staging exceeds the LDS; use fewer rows per workgroup");
+    static size_t attr = 0;
+    if (lds > 48 * 1024 && lds > attr) {
+      HIP_TRY(hipFuncSetAttribute((const void*)k_assemble<RB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
+      attr = cap;
     }
-    int nb = padded_blocks(ra.P, BLOCK / G);
-    hipLaunchKernelGGL(k_assemble<G>, dim3(nb), dim3(BLOCK), lds, st, ra, x, h2, vals, minv, maxnnz);
+    int nb = padded_blocks(ra.P, RB);
+    hipLaunchKernelGGL(k_assemble<RB>, dim3(nb), dim3(BLOCK), lds, st, ra, x, h2, vals, minv);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -946,6 +979,8 @@ struct fem_system {
   int *d_offsets = nullptr, *d_cols = nullptr, *d_diag = nullptr;
   int *d_tinc_ptr = nullptr, *d_tinc = nullptr, *d_hinc_ptr = nullptr, *d_hinc = nullptr;
   uint32_t *d_tslots = nullptr, *d_hslots = nullptr;
+  int* d_slot_ptr = nullptr;
+  uint16_t* d_contrib = nullptr;
   int *d_tet_idx = nullptr, *d_hex_idx = nullptr;
   float *d_tet_pose = nullptr, *d_tet_mat = nullptr, *d_hex_invJ = nullptr, *d_hex_detw = nullptr,
         *d_hex_mat = nullptr;
@@ -967,7 +1002,8 @@ struct fem_system {
   int* b_cols = nullptr;
   float *b_vals = nullptr, *b_minv = nullptr, *b_dv = nullptr;
   fem_params prm{};
-  int elem_g = 16, cg_g = 16, cg_wg_request = 0;
+  int elem_g = 4, asm_rb = 16, cg_wg_request = 0;
+  size_t asm_lds = 0;
   int max_resident = 0;
   std::vector<void*> allocs;
 
@@ -998,6 +1034,8 @@ struct fem_system {
     a.hinc_ptr = d_hinc_ptr;
     a.hinc = d_hinc;
     a.hslots = d_hslots;
+    a.slot_ptr = d_slot_ptr;
+    a.contrib = d_contrib;
     a.tet_idx = d_tet_idx;
     a.tet_pose = d_tet_pose;
     a.tet_mat = d_tet_mat;
@@ -1025,6 +1063,21 @@ struct fem_system {
                    prm.gravity[2] * prm.gravity[2]);
     a.use_act = prm.enable_tet_activations;
     return a;
+  }
+
+  // LDS staging bytes of the assembly for `rb` rows per workgroup.
+  size_t assemble_lds(int rb) const {
+    size_t worst = 0;
+    for (int r0 = 0; r0 < P; r0 += rb) {
+      const int r1 = std::min(r0 + rb, P);
+      const size_t nt = (size_t)(pt.tinc_ptr[r1] - pt.tinc_ptr[r0]);
+      const size_t nh = (size_t)(pt.hinc_ptr[r1] - pt.hinc_ptr[r0]);
+      const size_t ns = (size_t)(pt.offsets[r1] - pt.offsets[r0]);
+      const size_t nc = (size_t)(pt.slot_ptr[pt.offsets[r1]] - pt.slot_ptr[pt.offsets[r0]]);
+      worst = std::max(worst, ((size_t)ST_TET * nt + (size_t)ST_HEX * nh + ns + 1) * sizeof(float) +
+                                  nc * sizeof(uint16_t));
+    }
+    return std::max<size_t>(worst, 16);
   }
 
   // Row partition of the persistent CG: per instance, contiguous chunks
@@ -1225,6 +1278,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_hinc_ptr, hptr.data(), hptr.size())) ||
       (rc = s->upload(&s->d_hinc, s->pt.hinc.data(), s->pt.hinc.size())) ||
       (rc = s->upload(&s->d_hslots, s->pt.hslots.data(), s->pt.hslots.size())) ||
+      (rc = s->upload(&s->d_slot_ptr, s->pt.slot_ptr.data(), s->pt.slot_ptr.size())) ||
+      (rc = s->upload(&s->d_contrib, s->pt.contrib.data(), s->pt.contrib.size())) ||
       (rc = s->upload(&s->d_tet_idx, d->tet_indices, 4 * (size_t)s->T)) ||
       (rc = s->upload(&s->d_tet_pose, d->tet_poses, 9 * (size_t)s->T)) ||
       (rc = s->upload(&s->d_tet_mat, d->tet_materials, 3 * (size_t)s->T)) ||
@@ -1240,14 +1295,22 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_row_fixed, fixed_mask.data(), fixed_mask.size())) ||
       (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
       (rc = s->alloc(&s->d_z, 3 * (size_t)P)) || (rc = s->alloc(&s->d_p, 3 * (size_t)P)) ||
-      (rc = s->alloc(&s->d_rhs_max, s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
+      (rc = s->alloc(&s->d_rhs_max, (size_t)RHS_MAX_SLOTS * s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
       (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)))
     return cleanup(rc);
   double avg_inc = P ? (double)(s->pt.tinc_ptr[P] + s->pt.hinc_ptr[P]) / P : 1.0;
   double avg_nnz = P ? (double)s->pt.offsets[P] / P : 1.0;
-  s->elem_g = std::min(16, pick_lanes(avg_inc));
-  s->cg_g = std::min(32, pick_lanes(avg_nnz));
-  while ((size_t)(BLOCK / s->elem_g) * s->pt.max_row_nnz * 36 > 64 * 1024 && s->elem_g < 64) s->elem_g *= 2;
+  (void)avg_nnz;
+  s->elem_g = std::min(8, pick_lanes(avg_inc / 4.0));
+  // Rows per assembly workgroup: the most that keep the staging <= 40 KiB.
+  s->asm_rb = 4;
+  for (int rb : {32, 16, 8}) {
+    if (s->assemble_lds(rb) <= 56 * 1024) {
+      s->asm_rb = rb;
+      break;
+    }
+  }
+  s->asm_lds = s->assemble_lds(s->asm_rb);
   s->prm.min_stretch = 0.05f;
   s->prm.max_stretch = 20.f;
   s->prm.tolerance = 1e-2f;
@@ -1294,11 +1357,13 @@ int fem_set_params(fem_system* s, const fem_params* p) {
   return FEM_OK;
 }
 
-int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t cg_lanes, int32_t cg_workgroups) {
+int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   auto ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
   if (elem_lanes && !ok(elem_lanes)) return fail(FEM_ERR_ARG, "elem_lanes must be a power of two in [4, 64]");
-  if (cg_lanes && !ok(cg_lanes)) return fail(FEM_ERR_ARG, "cg_lanes must be a power of two in [4, 64]");
+  if (assemble_rows && !ok(assemble_rows)) return fail(FEM_ERR_ARG, "assemble_rows must be a power of two in [4, 64]");
+  if (assemble_rows && s->assemble_lds(assemble_rows) > 159 * 1024)
+    return fail(FEM_ERR_UNSUPPORTED, "assembly staging for that many rows exceeds 160 KiB");
   if (cg_workgroups < 0) return fail(FEM_ERR_ARG, "cg_workgroups must be >= 0");
   const int old_request = s->cg_wg_request;
   s->cg_wg_request = cg_workgroups;
@@ -1310,15 +1375,18 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t cg_lanes, int32_t 
     }
   }
   if (elem_lanes) s->elem_g = elem_lanes;
-  if (cg_lanes) s->cg_g = cg_lanes;
+  if (assemble_rows) {
+    s->asm_rb = assemble_rows;
+    s->asm_lds = s->assemble_lds(assemble_rows);
+  }
   return FEM_OK;
 }
 
-int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* cg_lanes, int32_t* cg_workgroups,
+int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
                    int32_t* cg_rows_per_thread) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   if (elem_lanes) *elem_lanes = s->elem_g;
-  if (cg_lanes) *cg_lanes = s->cg_g;
+  if (assemble_rows) *assemble_rows = s->asm_rb;
   if (cg_workgroups) *cg_workgroups = s->nchunks;
   if (cg_rows_per_thread) *cg_rows_per_thread = s->cg_k;
   return FEM_OK;
@@ -1340,7 +1408,7 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchAssemble>(s->elem_g, ra, q, dt * dt, s->b_vals, s->b_minv, std::max(s->pt.max_row_nnz, 1),
+  return dispatch_g<LaunchAssemble>(s->asm_rb, ra, q, dt * dt, s->b_vals, s->b_minv, s->asm_lds,
                                     (hipStream_t)stream);
 }
 
@@ -1361,7 +1429,7 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   {
-    int n = std::max(std::max(s->nfixed, s->ninst), 1);
+    int n = std::max(std::max(s->nfixed, s->ninst * RHS_MAX_SLOTS), 1);
     hipLaunchKernelGGL(k_prepare, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_fixed, s->nfixed,
                        s->d_anchor, q_in, qd_in, s->d_rhs_max, s->ninst);
     HIP_TRY(hipGetLastError());

@@ -30,6 +30,20 @@ FEM_DI void st3(float* __restrict__ p, int i, v3 v) {
 FEM_DI float det3(const float m[9]) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
+// Selection of one of several values by a runtime index, component by
+// component on scalars so that it lowers to v_cndmask selects (a ?: over
+// structs or array elements makes clang select pointers into a private,
+// scratch-resident array).
+FEM_DI float sel4f(int a, float a0, float a1, float a2, float a3) {
+  float r = a3;
+  r = (a == 2) ? a2 : r;
+  r = (a == 1) ? a1 : r;
+  r = (a == 0) ? a0 : r;
+  return r;
+}
+FEM_DI v3 sel4(int a, v3 a0, v3 a1, v3 a2, v3 a3) {
+  return {sel4f(a, a0.x, a1.x, a2.x, a3.x), sel4f(a, a0.y, a1.y, a2.y, a3.y), sel4f(a, a0.z, a1.z, a2.z, a3.z)};
+}
 // y = M x for row-major M.
 FEM_DI v3 mv(const float m[9], v3 x) {
   return {m[0] * x.x + m[1] * x.y + m[2] * x.z, m[3] * x.x + m[4] * x.y + m[5] * x.z,
@@ -70,7 +84,14 @@ FEM_DI void jacobi_rotate(float s[3][3], float v[3][3], int p, int q) {
   }
 }
 
-__device__ __noinline__ void svd_clamp_slow(float F[9], float mn, float mx) {
+struct M9 {
+  float m[9];
+};
+
+// Out of line and by value: the rarely taken SVD path neither inflates the
+// register budget of the element kernels nor forces F into scratch memory.
+__device__ __noinline__ M9 svd_clamp_slow(M9 Fin, float mn, float mx) {
+  float* F = Fin.m;
   float s[3][3], v[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -142,6 +163,7 @@ __device__ __noinline__ void svd_clamp_slow(float F[9], float mn, float mx) {
       F[3 * i + j] = ui0 * sg[0] * v[j][0] + ui1 * sg[1] * v[j][1] + ui2 * sg[2] * v[j][2];
     }
   }
+  return Fin;
 }
 
 // Principal-stretch clamp F <- U clamp(S) V^T (reference kernels.py:421-444),
@@ -156,7 +178,12 @@ FEM_DI void clamp_stretch(float F[9], float mn, float mx) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
   if (det > 0.f && ic <= mx * mx && 2.f * det >= mn * ic) return;
-  svd_clamp_slow(F, mn, mx);
+  M9 in;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) in.m[k] = F[k];
+  const M9 out = svd_clamp_slow(in, mn, mx);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k] = out.m[k];
 }
 
 // Scale-invariant 3x3 inverse with point-Jacobi fallback
@@ -192,7 +219,8 @@ FEM_DI void inverse_diagonal_block(const float B[9], float out[9]) {
 
 // ---------------------------------------------------------------------------
 // Tet4 Stable Neo-Hookean (reference eval_tetrahedra kernels.py:511-632 and
-// build_system_matrix_tet_kernel :635-850).
+// build_system_matrix_tet_kernel :635-850).  Only scalars, float arrays with
+// compile-time indices and by-value vectors, so everything stays in VGPRs.
 // ---------------------------------------------------------------------------
 struct TetGeo {
   v3 e1, e2, e3;  // x1-x0, x2-x0, x3-x0
@@ -201,10 +229,10 @@ struct TetGeo {
   float inv_rest_volume, rest_volume;
 };
 
-FEM_DI void tet_geo(const v3 x[4], const float* __restrict__ pose, float mn, float mx, TetGeo& g) {
-  g.e1 = x[1] - x[0];
-  g.e2 = x[2] - x[0];
-  g.e3 = x[3] - x[0];
+FEM_DI void tet_geo(v3 x0, v3 x1, v3 x2, v3 x3, const float* __restrict__ pose, float mn, float mx, TetGeo& g) {
+  g.e1 = x1 - x0;
+  g.e2 = x2 - x0;
+  g.e3 = x3 - x0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) g.Dm[k] = pose[k];
   g.inv_rest_volume = det3(g.Dm) * 6.f;
@@ -217,58 +245,66 @@ FEM_DI void tet_geo(const v3 x[4], const float* __restrict__ pose, float mn, flo
   clamp_stretch(g.F, mn, mx);
 }
 
+// Row c of Dm as a vector.
+FEM_DI v3 dm_row(const float Dm[9], int c) { return {Dm[3 * c], Dm[3 * c + 1], Dm[3 * c + 2]}; }
+
 // Force added to node `a` of the element (the reference subtracts f_a from
 // the force buffer).  Includes Rayleigh damping.
-FEM_DI v3 tet_node_force(const TetGeo& g, const v3 v[4], float mu, float lam, float kd, int a) {
-  float alpha = 1.f + mu / lam - mu / (4.f * lam);
-  float k_mu = mu * g.rest_volume, k_lambda = lam * g.rest_volume, k_damp = kd * g.rest_volume;
-  v3 v10 = v[1] - v[0], v20 = v[2] - v[0], v30 = v[3] - v[0];
-  float dF[9];
+FEM_DI v3 tet_node_force(const TetGeo& g, v3 v0, v3 v1, v3 v2, v3 v3_, float mu, float lam, float kd, int a) {
+  const float alpha = 1.f + mu / lam - mu / (4.f * lam);
+  const float k_mu = mu * g.rest_volume, k_lambda = lam * g.rest_volume, k_damp = kd * g.rest_volume;
+  const v3 v10 = v1 - v0, v20 = v2 - v0, v30 = v3_ - v0;
   const float vx[3] = {v10.x, v10.y, v10.z}, vy[3] = {v20.x, v20.y, v20.z}, vz[3] = {v30.x, v30.y, v30.z};
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) dF[3 * r + c] = vx[r] * g.Dm[c] + vy[r] * g.Dm[3 + c] + vz[r] * g.Dm[6 + c];
   float ic = 0.f;
 #pragma unroll
   for (int k = 0; k < 9; ++k) ic += g.F[k] * g.F[k];
-  float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
+  const float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
   float P[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) P[k] = g.F[k] * sdev + dF[k] * k_damp;
-  // H = P Dm^T; f_{1..3} are its columns.
-  v3 f[4];
+  for (int r = 0; r < 3; ++r)
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    v3 col = {P[0] * g.Dm[3 * c] + P[1] * g.Dm[3 * c + 1] + P[2] * g.Dm[3 * c + 2],
-              P[3] * g.Dm[3 * c] + P[4] * g.Dm[3 * c + 1] + P[5] * g.Dm[3 * c + 2],
-              P[6] * g.Dm[3 * c] + P[7] * g.Dm[3 * c + 1] + P[8] * g.Dm[3 * c + 2]};
-    f[c + 1] = col;
-  }
-  float J = fmaxf(det3(g.F), 0.01f);
-  float s = g.inv_rest_volume / 6.f;
-  v3 d1 = cross(g.e2, g.e3) * s, d2 = cross(g.e3, g.e1) * s, d3 = cross(g.e1, g.e2) * s;
-  float f_volume = (J - alpha) * k_lambda;
-  float f_damp = (dot(d1, v10) + dot(d2, v20) + dot(d3, v30)) * k_damp;
-  float ft = f_volume + f_damp;
-  f[1] = f[1] + d1 * ft;
-  f[2] = f[2] + d2 * ft;
-  f[3] = f[3] + d3 * ft;
-  f[0] = (f[1] + f[2] + f[3]) * -1.f;
-  v3 fa = (a == 0) ? f[0] : (a == 1 ? f[1] : (a == 2 ? f[2] : f[3]));
-  return fa * -1.f;
+    for (int c = 0; c < 3; ++c) {
+      const float dF = vx[r] * g.Dm[c] + vy[r] * g.Dm[3 + c] + vz[r] * g.Dm[6 + c];
+      P[3 * r + c] = g.F[3 * r + c] * sdev + dF * k_damp;
+    }
+  // H = P Dm^T; f_{1..3} are its columns: f_c = P * (row c of Dm).
+  v3 f1 = mv(P, dm_row(g.Dm, 0)), f2 = mv(P, dm_row(g.Dm, 1)), f3 = mv(P, dm_row(g.Dm, 2));
+  const float J = fmaxf(det3(g.F), 0.01f);
+  const float s = g.inv_rest_volume / 6.f;
+  const v3 d1 = cross(g.e2, g.e3) * s, d2 = cross(g.e3, g.e1) * s, d3 = cross(g.e1, g.e2) * s;
+  const float f_volume = (J - alpha) * k_lambda;
+  const float f_damp = (dot(d1, v10) + dot(d2, v20) + dot(d3, v30)) * k_damp;
+  const float ft = f_volume + f_damp;
+  f1 = f1 + d1 * ft;
+  f2 = f2 + d2 * ft;
+  f3 = f3 + d3 * ft;
+  const v3 f0 = (f1 + f2 + f3) * -1.f;
+  return sel4(a, f0, f1, f2, f3) * -1.f;
 }
 
 // Volumetric activation force on node a (eval_tet_activations kernels.py:484-508).
-FEM_DI v3 tet_activation_force(const v3 x[4], float lam, float act, int a) {
-  float scale = lam * act / 6.f;
-  v3 fb = cross(x[2] - x[0], x[3] - x[0]) * scale;
-  v3 fc = cross(x[3] - x[0], x[1] - x[0]) * scale;
-  v3 fd = cross(x[1] - x[0], x[2] - x[0]) * scale;
-  if (a == 0) return fb + fc + fd;
-  if (a == 1) return fb * -1.f;
-  if (a == 2) return fc * -1.f;
-  return fd * -1.f;
+FEM_DI v3 tet_activation_force(v3 x0, v3 x1, v3 x2, v3 x3, float lam, float act, int a) {
+  const float scale = lam * act / 6.f;
+  const v3 fb = cross(x2 - x0, x3 - x0) * scale;
+  const v3 fc = cross(x3 - x0, x1 - x0) * scale;
+  const v3 fd = cross(x1 - x0, x2 - x0) * scale;
+  return sel4(a, fb + fc + fd, fb * -1.f, fc * -1.f, fd * -1.f);
+}
+
+// One 3x3 block h2*[c1 (ga.gb) I + c2 (Fga)(Fgb)^T + c3 (Ja)(Jb)^T].
+FEM_DI void tangent_block(float out[9], float c1, float c2, float c3, float h2, v3 ga, v3 fa, v3 ja, v3 gb, v3 fb,
+                          v3 jb) {
+  const float dab = c1 * dot(ga, gb);
+  const float fa3[3] = {fa.x, fa.y, fa.z}, ja3[3] = {ja.x, ja.y, ja.z};
+  const float fb3[3] = {fb.x, fb.y, fb.z}, jb3[3] = {jb.x, jb.y, jb.z};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float kdev = ((r == c) ? dab : 0.f) + fa3[r] * fb3[c] * c2;
+      const float kvol = ja3[r] * jb3[c] * c3;
+      out[3 * r + c] = (kdev + kvol) * h2;
+    }
 }
 
 // Row `a` of the 12x12 element tangent: blocks K_ab, b = 0..3, scaled by h^2.
@@ -280,80 +316,79 @@ FEM_DI void tet_tangent_row(const TetGeo& g, float mu, float lam, float h2, int 
       for (int k = 0; k < 9; ++k) K[b][k] = 0.f;
     return;
   }
-  float k_mu = mu * g.rest_volume, k_lambda = lam * g.rest_volume;
+  const float k_mu = mu * g.rest_volume, k_lambda = lam * g.rest_volume;
   float ic = 0.f;
 #pragma unroll
   for (int k = 0; k < 9; ++k) ic += g.F[k] * g.F[k];
-  float ic1 = fmaxf(ic + 1.f, 1e-6f);
-  float s = 1.f - 1.f / ic1;
-  float coef = k_mu * 2.f / (ic1 * ic1);
-  float sv = g.inv_rest_volume / 6.f;
-  v3 dJ[4], gr[4], Fg[4];
-  dJ[1] = cross(g.e2, g.e3) * sv;
-  dJ[2] = cross(g.e3, g.e1) * sv;
-  dJ[3] = cross(g.e1, g.e2) * sv;
-  dJ[0] = (dJ[1] + dJ[2] + dJ[3]) * -1.f;
-  gr[1] = {g.Dm[0], g.Dm[1], g.Dm[2]};
-  gr[2] = {g.Dm[3], g.Dm[4], g.Dm[5]};
-  gr[3] = {g.Dm[6], g.Dm[7], g.Dm[8]};
-  gr[0] = (gr[1] + gr[2] + gr[3]) * -1.f;
-#pragma unroll
-  for (int b = 0; b < 4; ++b) Fg[b] = mv(g.F, gr[b]);
-  v3 ga = (a == 0) ? gr[0] : (a == 1 ? gr[1] : (a == 2 ? gr[2] : gr[3]));
-  v3 fa = (a == 0) ? Fg[0] : (a == 1 ? Fg[1] : (a == 2 ? Fg[2] : Fg[3]));
-  v3 ja = (a == 0) ? dJ[0] : (a == 1 ? dJ[1] : (a == 2 ? dJ[2] : dJ[3]));
-  const float fa3[3] = {fa.x, fa.y, fa.z}, ja3[3] = {ja.x, ja.y, ja.z};
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    float dab = dot(ga, gr[b]);
-    const float fb3[3] = {Fg[b].x, Fg[b].y, Fg[b].z}, jb3[3] = {dJ[b].x, dJ[b].y, dJ[b].z};
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        float kdev = ((r == c) ? k_mu * s * dab : 0.f) + fa3[r] * fb3[c] * coef;
-        float kvol = ja3[r] * jb3[c] * k_lambda;
-        K[b][3 * r + c] = (kdev + kvol) * h2;
-      }
-  }
+  const float ic1 = fmaxf(ic + 1.f, 1e-6f);
+  const float s = 1.f - 1.f / ic1;
+  const float coef = k_mu * 2.f / (ic1 * ic1);
+  const float sv = g.inv_rest_volume / 6.f;
+  const v3 j1 = cross(g.e2, g.e3) * sv, j2 = cross(g.e3, g.e1) * sv, j3 = cross(g.e1, g.e2) * sv;
+  const v3 j0 = (j1 + j2 + j3) * -1.f;
+  const v3 g1 = dm_row(g.Dm, 0), g2 = dm_row(g.Dm, 1), g3 = dm_row(g.Dm, 2);
+  const v3 g0 = (g1 + g2 + g3) * -1.f;
+  const v3 f0 = mv(g.F, g0), f1 = mv(g.F, g1), f2 = mv(g.F, g2), f3 = mv(g.F, g3);
+  const v3 ga = sel4(a, g0, g1, g2, g3), fa = sel4(a, f0, f1, f2, f3), ja = sel4(a, j0, j1, j2, j3);
+  const float c1 = k_mu * s;
+  tangent_block(K[0], c1, coef, k_lambda, h2, ga, fa, ja, g0, f0, j0);
+  tangent_block(K[1], c1, coef, k_lambda, h2, ga, fa, ja, g1, f1, j1);
+  tangent_block(K[2], c1, coef, k_lambda, h2, ga, fa, ja, g2, f2, j2);
+  tangent_block(K[3], c1, coef, k_lambda, h2, ga, fa, ja, g3, f3, j3);
 }
 
 // ---------------------------------------------------------------------------
 // Hex8 Q1 element with 2x2x2 Gauss quadrature (reference kernels.py:1356-1747).
 // Corner / Gauss-point signs: xi+ for {1,2,5,6}, eta+ for {2,3,6,7}, zeta+ for {4..7}.
+// Node data are held as scalar arrays (gx[8], gy[8], gz[8]).
 // ---------------------------------------------------------------------------
 FEM_DI float hex_sx(int a) { return (a == 1 || a == 2 || a == 5 || a == 6) ? 1.f : -1.f; }
 FEM_DI float hex_sy(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1.f : -1.f; }
 FEM_DI float hex_sz(int a) { return (a >= 4) ? 1.f : -1.f; }
 
-// Reference-configuration gradients dN[b] = inv_J0^T grad_xi N_b at Gauss point g.
-FEM_DI void hex_grads(const float* __restrict__ invJ, int g, v3 dN[8]) {
+// Reference-configuration gradients dN_b = inv_J0^T grad_xi N_b at Gauss point g.
+FEM_DI void hex_grads(const float* __restrict__ invJ, int g, float gx[8], float gy[8], float gz[8]) {
   const float gp = 0.5773502691896258f;
-  float xi = hex_sx(g) * gp, eta = hex_sy(g) * gp, zeta = hex_sz(g) * gp;
+  const float xi = hex_sx(g) * gp, eta = hex_sy(g) * gp, zeta = hex_sz(g) * gp;
   float J[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) J[k] = invJ[k];
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    float xa = hex_sx(b), ya = hex_sy(b), za = hex_sz(b);
-    v3 d = {xa * (1.f + ya * eta) * (1.f + za * zeta) * 0.125f, (1.f + xa * xi) * ya * (1.f + za * zeta) * 0.125f,
-            (1.f + xa * xi) * (1.f + ya * eta) * za * 0.125f};
-    dN[b] = {J[0] * d.x + J[3] * d.y + J[6] * d.z, J[1] * d.x + J[4] * d.y + J[7] * d.z,
-             J[2] * d.x + J[5] * d.y + J[8] * d.z};
+    const float xa = hex_sx(b), ya = hex_sy(b), za = hex_sz(b);
+    const float dx = xa * (1.f + ya * eta) * (1.f + za * zeta) * 0.125f;
+    const float dy = (1.f + xa * xi) * ya * (1.f + za * zeta) * 0.125f;
+    const float dz = (1.f + xa * xi) * (1.f + ya * eta) * za * 0.125f;
+    gx[b] = J[0] * dx + J[3] * dy + J[6] * dz;
+    gy[b] = J[1] * dx + J[4] * dy + J[7] * dz;
+    gz[b] = J[2] * dx + J[5] * dy + J[8] * dz;
   }
 }
 
-FEM_DI void hex_F(const v3 x[8], const v3 dN[8], float F[9]) {
+// F = sum_b x_b (outer) dN_b.
+FEM_DI void hex_F(const float px[8], const float py[8], const float pz[8], const float gx[8], const float gy[8],
+                  const float gz[8], float F[9]) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) F[k] = 0.f;
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    const float xs[3] = {x[b].x, x[b].y, x[b].z}, ds[3] = {dN[b].x, dN[b].y, dN[b].z};
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) F[3 * r + c] += xs[r] * ds[c];
+    F[0] += px[b] * gx[b]; F[1] += px[b] * gy[b]; F[2] += px[b] * gz[b];
+    F[3] += py[b] * gx[b]; F[4] += py[b] * gy[b]; F[5] += py[b] * gz[b];
+    F[6] += pz[b] * gx[b]; F[7] += pz[b] * gy[b]; F[8] += pz[b] * gz[b];
   }
+}
+
+// dN_a at Gauss point g for a runtime corner a, from its sign pattern
+// (no indexing into per-corner arrays).
+FEM_DI v3 hex_grad_corner(const float* __restrict__ invJ, int g, int a) {
+  const float gp = 0.5773502691896258f;
+  const float xi = hex_sx(g) * gp, eta = hex_sy(g) * gp, zeta = hex_sz(g) * gp;
+  const float xa = hex_sx(a), ya = hex_sy(a), za = hex_sz(a);
+  const float dx = xa * (1.f + ya * eta) * (1.f + za * zeta) * 0.125f;
+  const float dy = (1.f + xa * xi) * ya * (1.f + za * zeta) * 0.125f;
+  const float dz = (1.f + xa * xi) * (1.f + ya * eta) * za * 0.125f;
+  return {invJ[0] * dx + invJ[3] * dy + invJ[6] * dz, invJ[1] * dx + invJ[4] * dy + invJ[7] * dz,
+          invJ[2] * dx + invJ[5] * dy + invJ[8] * dz};
 }
 
 FEM_DI void cofactor(const float F[9], float cof[9]) {

@@ -139,6 +139,44 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
       pt.hslots[2 * k + 1] = hi;
     }
   }
+  // Slot -> contributing element blocks.
+  const size_t nnz = (size_t)pt.offsets[P];
+  pt.slot_ptr.assign(nnz + 1, 0);
+  for (int i = 0; i < P; ++i) {
+    if (pt.tinc_ptr[i + 1] - pt.tinc_ptr[i] > 4096 || pt.hinc_ptr[i + 1] - pt.hinc_ptr[i] > 4096) {
+      err = "a node belongs to more than 4096 elements of one kind";
+      return 3;
+    }
+    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k)
+      for (int b = 0; b < 4; ++b) pt.slot_ptr[pt.offsets[i] + ((pt.tslots[k] >> (8 * b)) & 255u) + 1]++;
+    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k)
+      for (int b = 0; b < 8; ++b) {
+        uint32_t w = pt.hslots[2 * k + (b >> 2)];
+        pt.slot_ptr[pt.offsets[i] + ((w >> (8 * (b & 3))) & 255u) + 1]++;
+      }
+  }
+  for (size_t k = 0; k < nnz; ++k) pt.slot_ptr[k + 1] += pt.slot_ptr[k];
+  pt.contrib.resize(pt.slot_ptr[nnz]);
+  {
+    std::vector<int32_t> fill(pt.slot_ptr.begin(), pt.slot_ptr.end() - 1);
+    for (int i = 0; i < P; ++i) {
+      for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
+        const uint32_t t = (uint32_t)(k - pt.tinc_ptr[i]);
+        for (int b = 0; b < 4; ++b) {
+          const int64_t slot = pt.offsets[i] + ((pt.tslots[k] >> (8 * b)) & 255u);
+          pt.contrib[fill[slot]++] = (uint16_t)((t << 3) | (uint32_t)b);
+        }
+      }
+      for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
+        const uint32_t t = (uint32_t)(k - pt.hinc_ptr[i]);
+        for (int b = 0; b < 8; ++b) {
+          uint32_t w = pt.hslots[2 * k + (b >> 2)];
+          const int64_t slot = pt.offsets[i] + ((w >> (8 * (b & 3))) & 255u);
+          pt.contrib[fill[slot]++] = (uint16_t)(0x8000u | (t << 3) | (uint32_t)b);
+        }
+      }
+    }
+  }
   return 0;
 }
 

@@ -18,6 +18,10 @@ struct Pattern {
   std::vector<int64_t> hinc_ptr;  // [P+1]
   std::vector<int32_t> hinc;      // (element << 3) | corner
   std::vector<uint32_t> hslots;   // 8 relative slots, two words per incidence
+  // Per slot, the element blocks that sum into it, in fixed order (tets by
+  // element, then hexes): (is_hex << 15) | (incidence index in row << 3) | corner b.
+  std::vector<int32_t> slot_ptr;  // [nnz+1]
+  std::vector<uint16_t> contrib;
   int max_row_nnz = 0;
 };
 

@@ -296,14 +296,14 @@ class SolverFEM:
         """Persistent solved velocity increment (CG warm start)."""
         return self.dv
 
-    def set_launch(self, elem_lanes: int = 0, cg_lanes: int = 0, cg_workgroups: int = 0):
+    def set_launch(self, elem_lanes: int = 0, assemble_rows: int = 0, cg_workgroups: int = 0):
         """Override kernel launch shapes (0 keeps the current value / auto)."""
-        _lib.check(self._lib.fem_set_launch(self._sys, elem_lanes, cg_lanes, cg_workgroups), "fem_set_launch")
+        _lib.check(self._lib.fem_set_launch(self._sys, elem_lanes, assemble_rows, cg_workgroups), "fem_set_launch")
 
     def launch_config(self) -> dict:
         v = [ctypes.c_int32() for _ in range(4)]
         _lib.check(self._lib.fem_get_launch(self._sys, *[ctypes.byref(x) for x in v]), "fem_get_launch")
-        return dict(zip(("elem_lanes", "cg_lanes", "cg_workgroups", "cg_rows_per_thread"), (x.value for x in v)))
+        return dict(zip(("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread"), (x.value for x in v)))
 
     # ------------------------------------------------------------------
     def reset(self, state=None, world_mask=None, flags=None):
