@@ -11,9 +11,9 @@
 //                (block row of A), fixed-row projection, per-instance max|b|
 //   k_cg         persistent block-Jacobi PCG: every workgroup owns a
 //                contiguous row range for the whole solve (x, r, z, p, q and
-//                M^-1 of its rows stay in registers), SpMV with G lanes per
-//                row, two instance-wide barriers per iteration, device-side
-//                residual test, fused update_state + fixed restore epilogue
+//                its row state stays in registers), tiled SpMV, one
+//                instance-wide barrier per iteration, device-side residual
+//                test, fused update_state + fixed restore epilogue
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +36,12 @@ constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
 // Barrier words of one CG instance (uint offsets inside its BAR_WORDS region;
 // every counter on its own 64-byte line).
 constexpr int BAR_WORDS = 512;
-constexpr int BAR_TOP_COUNT = 0, BAR_TOP_GEN = 16, BAR_XCD = 32, BAR_REG = 320;
+constexpr int BAR_TOP_COUNT = 0, BAR_TOP_GEN = 16, BAR_HIER_TOP = 288, BAR_XCD = 32, BAR_REG = 320;
 // Per-instance words holding max|b| (reduced by the solve).
 constexpr int RHS_MAX_SLOTS = 64;
+// CG partial sums per chunk and phase.
+constexpr int NPART = 8;
+enum { PT_PAP = 0, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR, PT_BB };
 
 thread_local std::string g_error;
 
@@ -110,15 +113,18 @@ __global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float
   if (t < ninst * RHS_MAX_SLOTS) rhs_max[t] = 0u;
 }
 
-// Per-solve reset of the CG barrier counters / XCC registration and the
-// device error word (generation words keep counting across launches).
+// Per-solve reset of the CG barrier counters, XCC epochs and registration
+// and of the device error word (the flat barrier's generation word keeps
+// counting across launches).
 __global__ void k_cg_reset(unsigned* bar, int ninst, int* err) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ninst) {
     unsigned* b = bar + BAR_WORDS * (size_t)t;
     b[BAR_TOP_COUNT] = 0u;
+    b[BAR_HIER_TOP] = 0u;
     for (int x = 0; x < 8; ++x) {
       b[BAR_XCD + 32 * x] = 0u;
+      b[BAR_XCD + 32 * x + 16] = 0u;
       b[BAR_REG + x] = 0u;
     }
   }
@@ -467,13 +473,15 @@ struct CgArgs {
   float* qd_out;
   const float* anchor;
   float h;
-  float* zg;
-  float* pg;
+  float* xw;  // 2 x P exchange records (double buffered)
   const int* chunk_rows;
   const int* chunk_inst;
   const int* inst_chunks;
   const int* chunk_tiles;  // [nchunks+1] first tile of each chunk
   const int* tile_rows;    // [ntiles] first row of each tile (row-aligned, <= TILE blocks)
+  const int* tile_cptr;    // [ntiles+1] offsets into tile_cols
+  const int* tile_cols;    // distinct columns of each tile, ascending
+  const uint16_t* lidx;    // [nnz] tile-local column number of each block
   const unsigned* rhs_max;
   float tolerance;
   int maxiter;
@@ -496,6 +504,19 @@ __device__ __forceinline__ unsigned ld_acq(unsigned* p) {
 __device__ __forceinline__ bool spin_gen(unsigned* gen, unsigned g, int* err) {
   long spins = 0;
   while (ld_acq(gen) == g) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > BARRIER_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+  return true;
+}
+
+// Spin until the monotonic counter *p reaches target (bounded like spin_gen).
+__device__ __forceinline__ bool spin_until(unsigned* p, unsigned target, int* err) {
+  long spins = 0;
+  while ((int)(ld_acq(p) - target) < 0) {
     __builtin_amdgcn_s_sleep(1);
     if (++spins > BARRIER_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
       __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -540,13 +561,16 @@ struct XcdGroup {
   int ngroups;  // XCCs the instance spans
 };
 
-// Two-level barrier.  Workgroups of one XCC share its L2: each makes its own
-// stores complete (s_waitcnt) and arrives on the XCC counter.  The last
-// arriver of an XCC writes that L2 back, meets the other XCCs' last arrivers
-// on the top counter, invalidates and releases its XCC; the others then
-// acquire.  One L2 writeback per XCC per barrier instead of one per
-// workgroup, and no counter sees more arrivals than the workgroups of one XCC.
-__device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, int* err, int* lds_flag) {
+// Two-level barrier with monotonic counters (reset before every launch; the
+// caller passes the 1-based barrier index `epoch`).  Workgroups of one XCC
+// share its L2: each drains its own stores into L2 (s_waitcnt) and arrives on
+// the XCC counter.  The last arriver of an XCC (its leader) writes that L2
+// back, arrives on the top counter, waits for every XCC, invalidates its L2
+// once and releases its XCC through the XCC epoch word.  The other
+// workgroups only wait for that word: they read data published by other
+// workgroups with non-temporal loads, which bypass the vector L1 (see
+// ld_shared below), so no per-workgroup cache invalidate is needed.
+__device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, unsigned epoch, int* err, int* lds_flag) {
   // s_waitcnt is per wave and the workgroup barrier's release does not wait
   // for global stores: every wave drains its own stores into L2 first.
   asm volatile("" ::: "memory");
@@ -558,31 +582,19 @@ __device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, int* err, i
     if (n > 1) {
       unsigned* xc = bar_xcount(b, xg.xcc);
       unsigned* xgen = bar_xgen(b, xg.xcc);
-      const unsigned g = ld_acq(xgen);
       const unsigned a = __hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a == (unsigned)(xg.nx - 1)) {
-        __hip_atomic_store(xc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a == (unsigned)xg.nx * epoch - 1u) {
         if (xg.ngroups > 1) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          unsigned* tc = b + BAR_TOP_COUNT;
-          unsigned* tg = b + BAR_TOP_GEN;
-          const unsigned gt = ld_acq(tg);
-          const unsigned at = __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (at == (unsigned)(xg.ngroups - 1)) {
-            __hip_atomic_store(tc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(tg, gt + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          } else {
-            ok = spin_gen(tg, gt, err);
-          }
+          unsigned* tc = b + BAR_HIER_TOP;
+          __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = spin_until(tc, (unsigned)xg.ngroups * epoch, err);
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         asm volatile("" ::: "memory");
-        __hip_atomic_store(xgen, g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(xgen, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        ok = spin_gen(xgen, g, err);
-        // A workgroup-scope invalidate leaves this CU's L1 intact; the
-        // agent-scope acquire drops stale L1 (and clean L2) lines.
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = spin_until(xgen, epoch, err);
       }
     }
     *lds_flag = ok;
@@ -590,6 +602,15 @@ __device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, int* err, i
   __syncthreads();
   return *lds_flag != 0;
 }
+
+// Loads of data that other workgroups of the running CG publish between
+// barriers (exchange records, partial sums).  Non-temporal loads are served
+// from L2 and never from a line the vector L1 kept from an earlier phase.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f3v __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ f4v ld_shared(const f4v* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_shared(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double ld_shared(const double* p) { return __builtin_nontemporal_load(p); }
 
 // Sum of N floats over the workgroup (fp32 within waves, fp64 across waves,
 // fixed order); every thread receives the result.
@@ -609,7 +630,7 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
   __syncthreads();
 }
 
-// Sum of part[c*4 + slot[n]] over the instance's chunks, identical in every
+// Sum of part[c*NPART + slot[n]] over the instance's chunks, identical in every
 // workgroup (same data, same reduction tree).
 template <int N>
 __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, const int (&slot)[N], double (&out)[N],
@@ -619,7 +640,7 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
   for (int n = 0; n < N; ++n) acc[n] = 0.0;
   for (int c = cb + (int)threadIdx.x; c < ce; c += BLOCK)
 #pragma unroll
-    for (int n = 0; n < N; ++n) acc[n] += part[4 * (size_t)c + slot[n]];
+    for (int n = 0; n < N; ++n) acc[n] += ld_shared(part + NPART * (size_t)c + slot[n]);
 #pragma unroll
   for (int n = 0; n < N; ++n)
 #pragma unroll
@@ -636,43 +657,95 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
 
 constexpr int CG_J = 8;                   // blocks in flight per thread
 constexpr int CG_TILE = CG_J * BLOCK;     // blocks per row-aligned tile
+// Exchange record of one row: z(3) w(3) p(3) + pad, 48 bytes (three 16-byte
+// aligned loads per gathered column).
+constexpr int XW = 12;
 
 // q[k] = sum_j A_ij u_j for the thread's own rows i = R0 + t + 256 k.
-// Row-aligned tiles of <= CG_TILE blocks are streamed with CG_J coalesced
-// blocks per thread; block products go to LDS and each row is summed by its
-// owner thread.  u = s*dv with fixed columns masked (prologue) or
-// z + beta*p (iterations).  Fixed rows give q = 0.
+// Row-aligned tiles of <= CG_TILE blocks.  Per tile, the operand u of every
+// distinct column the tile references is formed once into LDS (u_l, indexed
+// by the tile-local column number lidx), then CG_J coalesced blocks per
+// thread are multiplied against it; block products go to LDS and each row is
+// summed by its owner thread.  PROLOGUE: u = s*dv with fixed columns masked.
+// Otherwise u_j = p_j' = (z_j - alpha w_j) + beta p_j from the previous
+// phase's exchange records.  Fixed rows give q = 0.
+// Distinct columns of `tile` held by this thread (-1 past the end).
+__device__ __forceinline__ void load_tile_cols(const CgArgs& a, int tile, int (&ucol)[CG_J]) {
+  const int c0 = a.tile_cptr[tile], nu = a.tile_cptr[tile + 1] - c0;
+#pragma unroll
+  for (int j = 0; j < CG_J; ++j) {
+    const int u = threadIdx.x + BLOCK * j;
+    ucol[j] = u < nu ? a.tile_cols[c0 + u] : -1;
+  }
+}
+
+// ucol holds the first tile's distinct columns on entry and again on return:
+// each tile prefetches the next one's list (the first tile's after the last)
+// so that the operand loads of a tile never wait for its column list.
 template <int K, bool PROLOGUE>
 __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0, int t1, const int* off_l,
-                                        float s, float beta, float* part_l, const bool (&fx)[K], float (&q)[K][3]) {
+                                        float s, float alpha, float beta, const float* __restrict__ xw,
+                                        float* part_l, float* u_l, int (&ucol)[CG_J], const bool (&fx)[K],
+                                        float (&q)[K][3]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) q[k][0] = q[k][1] = q[k][2] = 0.f;
   for (int tile = t0; tile < t1; ++tile) {
     const int ta = a.tile_rows[tile] - R0;
     const int tb = (tile + 1 < t1 ? a.tile_rows[tile + 1] : R0 + nr) - R0;
     const int kb = off_l[ta], ke = off_l[tb];
-    int col[CG_J];
+    // This thread's blocks: local column numbers and values (issued before
+    // the staging so that their latency overlaps it).
+    int li[CG_J];
+    float bv[CG_J][9];
 #pragma unroll
     for (int j = 0; j < CG_J; ++j) {
       const int k = kb + threadIdx.x + BLOCK * j;
-      col[j] = k < ke ? a.cols[k] : -1;
+      li[j] = k < ke ? (int)a.lidx[k] : -1;
+      if (k < ke) {
+        // Three 12-byte loads per block: a wave's load then touches each
+        // cache line of its 64 blocks 3 times instead of 9.
+        // (f3v is 16 bytes in size: address each row explicitly.)
+        const float* b = a.vals + 9 * (size_t)k;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const f3v v = *reinterpret_cast<const f3v*>(b + 3 * r);
+          bv[j][3 * r] = v.x;
+          bv[j][3 * r + 1] = v.y;
+          bv[j][3 * r + 2] = v.z;
+        }
+      }
     }
 #pragma unroll
     for (int j = 0; j < CG_J; ++j) {
-      const int k = kb + threadIdx.x + BLOCK * j;
-      if (col[j] >= 0) {
-        const float* b = a.vals + 9 * (size_t)k;
+      if (ucol[j] >= 0) {
+        const int c = ucol[j];
         v3 u;
         if (PROLOGUE) {
-          u = a.row_fixed[col[j]] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, col[j]) * s;
+          u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, c) * s;
         } else {
-          const v3 zz = ld3(a.zg, col[j]), pp = ld3(a.pg, col[j]);
-          u = {fmaf(beta, pp.x, zz.x), fmaf(beta, pp.y, zz.y), fmaf(beta, pp.z, zz.z)};
+          const f4v* e = reinterpret_cast<const f4v*>(xw + XW * (size_t)c);
+          const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
+          const float p2 = ld_shared(xw + XW * (size_t)c + 8);
+          u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
+               fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
         }
-        float* o = part_l + 3 * (k - kb);
-        o[0] = b[0] * u.x + b[1] * u.y + b[2] * u.z;
-        o[1] = b[3] * u.x + b[4] * u.y + b[5] * u.z;
-        o[2] = b[6] * u.x + b[7] * u.y + b[8] * u.z;
+        float* o = u_l + 3 * (threadIdx.x + BLOCK * j);
+        o[0] = u.x;
+        o[1] = u.y;
+        o[2] = u.z;
+      }
+    }
+    load_tile_cols(a, tile + 1 < t1 ? tile + 1 : t0, ucol);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CG_J; ++j) {
+      if (li[j] >= 0) {
+        const float* uu = u_l + 3 * li[j];
+        const float ux = uu[0], uy = uu[1], uz = uu[2];
+        float* o = part_l + 3 * (threadIdx.x + BLOCK * j);
+        o[0] = bv[j][0] * ux + bv[j][1] * uy + bv[j][2] * uz;
+        o[1] = bv[j][3] * ux + bv[j][4] * uy + bv[j][5] * uz;
+        o[2] = bv[j][6] * ux + bv[j][7] * uy + bv[j][8] * uz;
       }
     }
     __syncthreads();
@@ -692,18 +765,45 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
         q[k][2] = s2;
       }
     }
-    __syncthreads();
+    // The next tile writes u_l only (its products follow its own barrier),
+    // so no barrier is needed between these row sums and its staging.
   }
+}
+
+__device__ __forceinline__ void st_xw(float* xw, int i, const float (&z)[3], const float (&w)[3],
+                                      const float (&p)[3]) {
+  float4* e = reinterpret_cast<float4*>(xw + XW * (size_t)i);
+  e[0] = make_float4(z[0], z[1], z[2], w[0]);
+  e[1] = make_float4(w[1], w[2], p[0], p[1]);
+  xw[XW * (size_t)i + 8] = p[2];
 }
 
 // Warp's absolute floor eps32^(9/4) on the residual norm (tol given, atol=0).
 __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-16; }
 
+// Block-Jacobi PCG with one instance-wide barrier per iteration.
+//
+// The iterates are those of the standard PCG of warp.optim.linear.cg
+// (alpha_k = rz_k / p_k.Ap_k, beta_k = rz_{k+1} / rz_k, residual-masked
+// updates), reorganised so that the reduction and the exchange of the SpMV
+// operand share one synchronisation point.  Phase k (after the barrier):
+//   alpha_{k-1} from rz_{k-1}, p_{k-1}.q_{k-1};
+//   rz_k = rz_{k-1} - 2 alpha q.z + alpha^2 q.w      (w = M^-1 q; M^-1 is
+//   rr_k = rr_{k-1} - 2 alpha q.r + alpha^2 q.q       symmetric) give beta_{k-1};
+//   x += alpha p, r -= alpha q, z -= alpha w, p = z + beta p   (own rows);
+//   q = A p, gathering neighbour p from the exchange records (z, w, p) of
+//   phase k-1; w = M^-1 q; publish (z, w, p) and the partial sums
+//   p.q, q.z, q.w, q.r, q.q, r.z, r.r; barrier.
+// The direct r.z and r.r of phase k replace the recurrence values at phase
+// k+1, so recurrence error does not accumulate.  Exchange records and
+// partial sums are double buffered: phase k writes buffer (k+1)&1, which no
+// workgroup reads after the barrier that ended phase k-1.
 template <int K>
 __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   __shared__ float part_l[3 * CG_TILE];
+  __shared__ float u_l[3 * CG_TILE];
   __shared__ int off_l[K * BLOCK + 1];
-  __shared__ double red[16];
+  __shared__ double red[4 * NPART];
   __shared__ int bflag;
   __shared__ int xinfo[3];
   const int c = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -711,6 +811,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
   const int t0 = a.chunk_tiles[c], t1 = a.chunk_tiles[c + 1];
+  const size_t nch = (size_t)gridDim.x;
   unsigned* bar = a.bar + BAR_WORDS * inst;
   float m = 0.f;
   for (int k = 0; k < RHS_MAX_SLOTS; ++k) m = fmaxf(m, __uint_as_float(a.rhs_max[RHS_MAX_SLOTS * inst + k]));
@@ -742,17 +843,36 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   }
   __syncthreads();
 
-  float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3];
+  float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3], w[K][3];
+  // M^-1 blocks of the own rows: resident for the whole solve when the
+  // register budget allows (K <= 2), else reloaded where used.
+  constexpr bool MRES = K <= 2;
+  float Mk[MRES ? K : 1][9];
   bool own[K], fx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int lr = threadIdx.x + BLOCK * k;
     own[k] = lr < nr;
     fx[k] = own[k] ? (a.row_fixed[R0 + lr] != 0) : true;
+    if (MRES) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Mk[MRES ? k : 0][e] = own[k] ? a.minv[9 * (size_t)(R0 + lr) + e] : 0.f;
+    }
   }
+  auto minv_mul = [&](int k, int i, v3 v) -> v3 {
+    if (MRES) return mv(Mk[MRES ? k : 0], v);
+    float m[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[e] = a.minv[9 * (size_t)i + e];
+    return mv(m, v);
+  };
+  int ucol[CG_J];
+  load_tile_cols(a, t0, ucol);
 
+  // Prologue: r0 = b - A x0, z0 = M^-1 r0; records (z0, 0, 0) so that the
+  // first phase gathers p0 = z0.
   if (!zero_start) {
-    cg_spmv<K, true>(a, R0, nr, t0, t1, off_l, s, 0.f, part_l, fx, qv);
+    cg_spmv<K, true>(a, R0, nr, t0, t1, off_l, s, 0.f, 0.f, a.xw, part_l, u_l, ucol, fx, qv);
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k][0] = qv[k][1] = qv[k][2] = 0.f;
@@ -761,115 +881,112 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = 0.f;
+    for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = w[k][d] = 0.f;
     if (own[k]) {
       const int i = R0 + threadIdx.x + BLOCK * k;
       const v3 b = ld3(a.rhs, i) * s;
       const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : ld3(a.dv, i) * s;
-      float Mk[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
       const v3 rr0 = b - v3{qv[k][0], qv[k][1], qv[k][2]};
-      const v3 zz = mv(Mk, rr0);
+      const v3 zz = minv_mul(k, i, rr0);
       x[k][0] = x0.x; x[k][1] = x0.y; x[k][2] = x0.z;
       r[k][0] = rr0.x; r[k][1] = rr0.y; r[k][2] = rr0.z;
       z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
-      p[k][0] = zz.x; p[k][1] = zz.y; p[k][2] = zz.z;
-      st3(a.zg, i, zz);
-      st3(a.pg, i, zz);
+      st_xw(a.xw, i, z[k], w[k], p[k]);
       pb[0] += dot(b, b);
       pb[1] += dot(rr0, rr0);
       pb[2] += dot(rr0, zz);
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qv[k][d] = 0.f;
   }
-  double wsum[3];
-  wg_sum<3>(pb, wsum, red);
-  if (threadIdx.x == 0) {
-    a.part[4 * (size_t)c + 3] = wsum[0];
-    a.part[4 * (size_t)c + 1] = wsum[1];
-    a.part[4 * (size_t)c + 2] = wsum[2];
+  {
+    double wsum[3];
+    wg_sum<3>(pb, wsum, red);
+    if (threadIdx.x == 0) {
+      a.part[NPART * (size_t)c + PT_BB] = wsum[0];
+      a.part[NPART * (size_t)c + PT_RR] = wsum[1];
+      a.part[NPART * (size_t)c + PT_RZ] = wsum[2];
+    }
   }
-  ok = ok && hier_barrier(bar, xg, nci, a.err, &bflag);
+  unsigned epoch = 1;
+  ok = ok && hier_barrier(bar, xg, nci, epoch, a.err, &bflag);
   double tot[3];
   {
-    const int sl[3] = {3, 1, 2};
+    const int sl[3] = {PT_BB, PT_RR, PT_RZ};
     inst_sum<3>(a.part, cb, ce, sl, tot, red);
   }
-  const double bb = tot[0];
   double rr = tot[1], rz = tot[2];
-  double atol = fmax(0.0, (double)a.tolerance * sqrt(bb));
+  double atol = fmax(0.0, (double)a.tolerance * sqrt(tot[0]));
   atol = fmax(atol, residual_floor());
   const double atol_sq = atol * atol;
-  float beta = 0.f;
   int iters = 0;
-  for (int it = 0; it < a.maxiter && ok; ++it) {
-    const bool live = rr > atol_sq;
-    if (!live && a.early_exit) break;
-    // Phase A: q = A p_new with p_new = z + beta p (gathered on the fly).
-    cg_spmv<K, false>(a, R0, nr, t0, t1, off_l, s, beta, part_l, fx, qv);
-    float pp[1] = {0.f};
+  for (int ph = 0; ph <= a.maxiter && ok; ++ph) {
+    const int cur = ph & 1;
+    float alpha = 0.f, beta = 0.f;
+    if (ph > 0) {
+      double g[7];
+      {
+        const int sl[7] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR};
+        inst_sum<7>(a.part + NPART * nch * cur, cb, ce, sl, g, red);
+      }
+      rz = g[5];
+      rr = g[6];
+      const bool live = rr > atol_sq;
+      if (!live && a.early_exit) break;
+      if (live) ++iters;
+      alpha = (live && g[0] != 0.0) ? (float)(rz / g[0]) : 0.f;
+      const double al = (double)alpha;
+      const double rr_next = rr - 2.0 * al * g[3] + al * al * g[4];
+      const double rz_next = rz - 2.0 * al * g[1] + al * al * g[2];
+      beta = (rr_next > atol_sq && rz != 0.0) ? (float)(rz_next / rz) : 0.f;
+      rr = rr_next;
+    }
+    // x += alpha p, r -= alpha q, z -= alpha w, p = z + beta p (own rows,
+    // bit-identical to what the neighbours rebuild in cg_spmv).
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (own[k]) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          p[k][d] = fmaf(beta, p[k][d], z[k][d]);
-          pp[0] += p[k][d] * qv[k][d];
-        }
+      for (int d = 0; d < 3; ++d) {
+        x[k][d] = fmaf(alpha, p[k][d], x[k][d]);
+        r[k][d] = fmaf(-alpha, qv[k][d], r[k][d]);
+        z[k][d] = fmaf(-alpha, w[k][d], z[k][d]);
+        p[k][d] = fmaf(beta, p[k][d], z[k][d]);
       }
     }
-    double w1[1];
-    wg_sum<1>(pp, w1, red);
-    if (threadIdx.x == 0) a.part[4 * (size_t)c] = w1[0];
-    ok = hier_barrier(bar, xg, nci, a.err, &bflag);
-    if (!ok) break;
-    double pAp[1];
-    {
-      const int sl[1] = {0};
-      inst_sum<1>(a.part, cb, ce, sl, pAp, red);
-    }
-    const float alpha = (live && pAp[0] != 0.0) ? (float)(rz / pAp[0]) : 0.f;
-    // Phase B: x += alpha p, r -= alpha q, z = M^-1 r.
-    float pr[2] = {0.f, 0.f};
+    if (ph == a.maxiter) break;
+    cg_spmv<K, false>(a, R0, nr, t0, t1, off_l, s, alpha, beta, a.xw + XW * (size_t)a.P * cur, part_l, u_l,
+                      ucol, fx, qv);
+    float pp[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float* xw_next = a.xw + XW * (size_t)a.P * (cur ^ 1);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (own[k]) {
         const int i = R0 + threadIdx.x + BLOCK * k;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          x[k][d] = fmaf(alpha, p[k][d], x[k][d]);
-          r[k][d] = fmaf(-alpha, qv[k][d], r[k][d]);
-        }
-        float Mk[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) Mk[e] = a.minv[9 * (size_t)i + e];
-        const v3 rv = {r[k][0], r[k][1], r[k][2]};
-        const v3 zz = mv(Mk, rv);
-        z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
-        pr[0] += dot(rv, rv);
-        pr[1] += dot(rv, zz);
-        st3(a.zg, i, zz);
-        st3(a.pg, i, v3{p[k][0], p[k][1], p[k][2]});
+        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]};
+        const v3 ww = minv_mul(k, i, qq);
+        w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
+        st_xw(xw_next, i, z[k], w[k], p[k]);
+        const v3 pv = {p[k][0], p[k][1], p[k][2]}, zv = {z[k][0], z[k][1], z[k][2]},
+                 rv = {r[k][0], r[k][1], r[k][2]};
+        pp[0] += dot(pv, qq);
+        pp[1] += dot(qq, zv);
+        pp[2] += dot(qq, ww);
+        pp[3] += dot(qq, rv);
+        pp[4] += dot(qq, qq);
+        pp[5] += dot(rv, zv);
+        pp[6] += dot(rv, rv);
       }
     }
-    double w2[2];
-    wg_sum<2>(pr, w2, red);
-    if (threadIdx.x == 0) {
-      a.part[4 * (size_t)c + 1] = w2[0];
-      a.part[4 * (size_t)c + 2] = w2[1];
-    }
-    ok = hier_barrier(bar, xg, nci, a.err, &bflag);
-    if (!ok) break;
-    double t2[2];
     {
-      const int sl[2] = {1, 2};
-      inst_sum<2>(a.part, cb, ce, sl, t2, red);
+      double ws[7];
+      wg_sum<7>(pp, ws, red);
+      if (threadIdx.x == 0) {
+        double* dst = a.part + NPART * (nch * (cur ^ 1) + c);
+#pragma unroll
+        for (int n = 0; n < 7; ++n) dst[n] = ws[n];
+      }
     }
-    const double rr_new = t2[0], rz_new = t2[1];
-    beta = (rr_new > atol_sq && rz != 0.0) ? (float)(rz_new / rz) : 0.f;
-    rz = rz_new;
-    rr = rr_new;
-    if (live) ++iters;
+    ok = hier_barrier(bar, xg, nci, ++epoch, a.err, &bflag);
   }
   // restore_linear_increment kernels.py:1808-1811, update_state :115-128,
   // FixedParticles.restore(state_out) solver.py:1350-1351.
@@ -990,12 +1107,13 @@ struct fem_system {
   int* d_fixed = nullptr;
   int* d_row_inst = nullptr;
   // solve scratch
-  float *d_z = nullptr, *d_p = nullptr;
+  float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_bar = nullptr;
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
-      *d_tile_rows = nullptr;
+      *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
+  uint16_t* d_lidx = nullptr;
   int nchunks = 0, cg_k = 1;
   // bound
   int* b_offsets = nullptr;
@@ -1164,6 +1282,25 @@ struct fem_system {
       }
       chunk_tiles.push_back((int)tile_rows.size());
     }
+    // Distinct columns of every tile and the tile-local column number of
+    // every block (the SpMV forms each distinct operand once per tile).
+    const int ntiles = (int)tile_rows.size();
+    std::vector<int32_t> tile_cptr(1, 0), tile_cols;
+    std::vector<uint16_t> lidx(pt.offsets[P]);
+    {
+      std::vector<int32_t> uniq;
+      for (int t = 0; t < ntiles; ++t) {
+        const int ra = tile_rows[t], rb = t + 1 < ntiles ? tile_rows[t + 1] : P;
+        const int64_t ka = pt.offsets[ra], kb = pt.offsets[rb];
+        uniq.assign(pt.columns.begin() + ka, pt.columns.begin() + kb);
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        for (int64_t k = ka; k < kb; ++k)
+          lidx[k] = (uint16_t)(std::lower_bound(uniq.begin(), uniq.end(), pt.columns[k]) - uniq.begin());
+        tile_cols.insert(tile_cols.end(), uniq.begin(), uniq.end());
+        tile_cptr.push_back((int32_t)tile_cols.size());
+      }
+    }
     const int k_rows = maxrows <= BLOCK ? 1 : maxrows <= 2 * BLOCK ? 2 : maxrows <= 4 * BLOCK ? 4 : 8;
     const void* kp = cg_kernel_ptr(k_rows);
     if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
@@ -1174,17 +1311,23 @@ struct fem_system {
       std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups, device holds %d", n_chunks, nb * ncu);
       return fail(FEM_ERR_UNSUPPORTED, buf);
     }
-    int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr;
+    int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr,
+        *n_tcptr = nullptr, *n_tcols = nullptr;
+    uint16_t* n_lidx = nullptr;
     double* n_part = nullptr;
     int rc;
-    if ((rc = upload(&n_rows, chunk_rows.data(), chunk_rows.size())) ||
+    if ((rc = upload(&n_tcptr, tile_cptr.data(), tile_cptr.size())) ||
+        (rc = upload(&n_tcols, tile_cols.data(), tile_cols.size())) ||
+        (rc = upload(&n_lidx, lidx.data(), lidx.size())) ||
+        (rc = upload(&n_rows, chunk_rows.data(), chunk_rows.size())) ||
         (rc = upload(&n_inst, chunk_inst.data(), chunk_inst.size())) ||
         (rc = upload(&n_ichunks, inst_chunks.data(), inst_chunks.size())) ||
         (rc = upload(&n_tiles, chunk_tiles.data(), chunk_tiles.size())) ||
-        (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 4 * (size_t)n_chunks)))
+        (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 2 * (size_t)NPART * n_chunks)))
       return rc;
     for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
-                    (void*)d_chunk_tiles, (void*)d_tile_rows}) {
+                    (void*)d_chunk_tiles, (void*)d_tile_rows, (void*)d_tile_cptr, (void*)d_tile_cols,
+                    (void*)d_lidx}) {
       if (p) {
         (void)hipFree(p);
         allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
@@ -1195,6 +1338,9 @@ struct fem_system {
     d_inst_chunks = n_ichunks;
     d_chunk_tiles = n_tiles;
     d_tile_rows = n_trows;
+    d_tile_cptr = n_tcptr;
+    d_tile_cols = n_tcols;
+    d_lidx = n_lidx;
     d_part = n_part;
     nchunks = n_chunks;
     cg_k = k_rows;
@@ -1294,7 +1440,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_anchor, d->particle_q, 3 * (size_t)P)) ||
       (rc = s->upload(&s->d_row_fixed, fixed_mask.data(), fixed_mask.size())) ||
       (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
-      (rc = s->alloc(&s->d_z, 3 * (size_t)P)) || (rc = s->alloc(&s->d_p, 3 * (size_t)P)) ||
+      (rc = s->alloc(&s->d_xw, 2 * (size_t)XW * P)) ||
       (rc = s->alloc(&s->d_rhs_max, (size_t)RHS_MAX_SLOTS * s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
       (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)))
     return cleanup(rc);
@@ -1461,11 +1607,13 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
   ca.qd_out = qd_out;
   ca.anchor = s->d_anchor;
   ca.h = dt;
-  ca.zg = s->d_z;
-  ca.pg = s->d_p;
+  ca.xw = s->d_xw;
   ca.chunk_rows = s->d_chunk_rows;
   ca.chunk_tiles = s->d_chunk_tiles;
   ca.tile_rows = s->d_tile_rows;
+  ca.tile_cptr = s->d_tile_cptr;
+  ca.tile_cols = s->d_tile_cols;
+  ca.lidx = s->d_lidx;
   ca.chunk_inst = s->d_chunk_inst;
   ca.inst_chunks = s->d_inst_chunks;
   ca.rhs_max = s->d_rhs_max;

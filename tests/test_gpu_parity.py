@@ -266,3 +266,60 @@ def test_hex_cantilever_rollout():
     print(f"cantilever position rel err {err:.2e}, tip displacement {disp:.3e}", flush=True)
     assert err < 1e-4
     np.testing.assert_array_equal(np_(st0.particle_q)[sc.fixed], model.host["particle_q"][sc.fixed])
+
+
+def _large_step(s, model, dt, tol, maxiter, early_exit):
+    # Device-side residual exit follows linear_solver_use_cuda_graph (reference
+    # solver.py:1205-1216); the setters below re-send the parameters.
+    s.linear_solver_use_cuda_graph = early_exit
+    s.linear_solver_tolerance = tol
+    s.linear_solver_maxiter = maxiter
+    st0, st1 = model.state(), model.state()
+    s.velocity_increment.zero_()
+    s.refresh_tangent(st0.particle_q)
+    s.step(st0, st1, None, None, dt)
+    return st0, st1
+
+
+@pytest.mark.parametrize("early_exit", [True, False])
+def test_large_substep_matches_oracle(early_exit):
+    """1M-tet solid (x=0 face fixed): one substep on every XCD vs the FP64 oracle.
+
+    A converged solve (tol 1e-6) must reproduce the oracle's velocity increment;
+    the fixed 50-iteration budget (residual exit disabled) must agree with the
+    oracle's 50 iterations.  Either fails loudly if a workgroup reads a stale
+    exchange record or partial sum.
+    """
+    print(f"case large substep early_exit={early_exit}", flush=True)
+    sc = meshes.large_tet_scene()
+    model = build_tet_model(sc, device=DEV)
+    dt = 1 / 600
+    s = solver_for(model, dt=dt, fixed_particle_indices=sc.fixed, linear_solver_use_cuda_graph=early_exit)
+    tol, maxiter = (1e-6, 400) if early_exit else (1e-12, 50)
+    st0, st1 = _large_step(s, model, dt, tol, maxiter, early_exit)
+    ref = oracle_for(model, fixed=sc.fixed, dt=dt, linear_solver_tolerance=tol, linear_solver_maxiter=maxiter)
+    ref.refresh_tangent(np_(st0.particle_q))
+    out = ref.step(np_(st0.particle_q), np_(st0.particle_qd))
+    err_dv = rel(np_(s.dv), ref.dv)
+    print(f"large substep dv rel err {err_dv:.2e} (oracle iterations {out['iters']})", flush=True)
+    assert err_dv < (2e-3 if early_exit else 1e-3)
+    assert rel(np_(st1.particle_q), out["q"]) < 1e-6
+    np.testing.assert_array_equal(np_(st1.particle_q)[sc.fixed], model.host["particle_q"][sc.fixed])
+
+
+def test_cg_launch_shapes_agree():
+    """The persistent CG's result does not depend on how rows are split into workgroups."""
+    print("case launch shapes", flush=True)
+    sc = meshes.large_tet_scene()
+    model = build_tet_model(sc, device=DEV)
+    dt = 1 / 600
+    s = solver_for(model, dt=dt, fixed_particle_indices=sc.fixed)
+    base = None
+    for wg in (128, 256, 416, 512):
+        s.set_launch(0, 0, wg)
+        _large_step(s, model, dt, 1e-6, 400, True)
+        dv = np_(s.dv)
+        if base is None:
+            base = dv
+        else:
+            assert rel(dv, base) < 1e-3, wg
