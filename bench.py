@@ -78,6 +78,103 @@ def cpu_baseline(scene, model, substeps: int):
     }
 
 
+GROUND = dict(ground_plane=(0.0, 0.0, 1.0, 0.0), ground_ke=100.0, ground_kd=0.1, ground_kf=1.0, ground_mu=0.5)
+
+
+def _graph_two_substeps(solver, model, dt):
+    """HIP graph of two substeps (s0 -> s1 -> s0), each tangent refresh + step.
+
+    The graph writes into the returned states: the caller must keep them alive
+    as long as it replays the graph (freed tensors are reused by later
+    allocations, which the replays would then overwrite).
+    """
+    import torch
+
+    s0, s1 = model.state(), model.state()
+
+    def body():
+        solver.refresh_tangent(s0.particle_q, dt)
+        solver.step(s0, s1, None, None, dt)
+        solver.refresh_tangent(s1.particle_q, dt)
+        solver.step(s1, s0, None, None, dt)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        body()  # eager warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        body()
+    return g, (s0, s1)
+
+
+def bench_batched(dev, replays: int):
+    """256 uneven drop meshes: one batched solver vs 256 solvers stepped in turn.
+
+    Both sides run the same substep (tangent refresh + forces + PCG with the
+    per-instance residual exit at tol 1e-2, <= 50 iterations) captured in HIP
+    graphs of two substeps; a step advances all 256 meshes by one substep.
+    """
+    import numpy as np
+    import torch
+
+    from newton_solver_fem_amd import build_tet_model, merge_tet_scenes, meshes
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    scenes = meshes.batched_tet_scenes()
+    merged, offs = merge_tet_scenes(scenes)
+    kw = dict(dt=DT, solver_type="cg", preconditioner_type="block_diag", linear_solver_maxiter=CG_ITERS,
+              linear_solver_tolerance=1e-2, self_contact_ke=0.0, linear_solver_use_cuda_graph=True, **GROUND)
+    model = build_tet_model(merged, device=dev)
+    bs = SolverFEM(model, instance_offsets=offs, **kw)
+    tune = bs.autotune(model.state(), DT, repeats=2)
+    g, g_states = _graph_two_substeps(bs, model, DT)
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(replays):
+        g.replay()
+    torch.cuda.synchronize()
+    batched = 2 * replays / (time.perf_counter() - t0)
+    iters = bs.solve_stats()["iterations"]
+
+    graphs = []  # (graph, its states, solver, model): everything a replay touches stays referenced
+    for sc in scenes:
+        m = build_tet_model(sc, device=dev)
+        sk = SolverFEM(m, **kw)
+        gk, states = _graph_two_substeps(sk, m, DT)
+        graphs.append((gk, states, sk, m))
+    for gk, *_ in graphs:
+        gk.replay()
+    torch.cuda.synchronize()
+    lrep = max(1, replays // 4)
+    t0 = time.perf_counter()
+    for _ in range(lrep):
+        for gk, *_ in graphs:
+            gk.replay()
+    torch.cuda.synchronize()
+    looped = 2 * lrep / (time.perf_counter() - t0)
+    for _, (q_a, q_b), _, _ in graphs:
+        if not (torch.isfinite(q_a.particle_q).all() and torch.isfinite(q_b.particle_q).all()):
+            raise RuntimeError("non-finite looped state")
+    if not torch.isfinite(g_states[0].particle_q).all():
+        raise RuntimeError("non-finite batched state")
+    return {
+        "workload": "256 uneven Tet4 drop meshes (tets log-uniform in [500, 50000], seed 1), ground contact, "
+                    "tangent refresh + forces + per-instance PCG (tol 1e-2, <= 50 iterations) per substep",
+        "instances": len(scenes),
+        "tets": int(model.tet_count),
+        "nodes": int(model.particle_count),
+        "batched_steps_per_s": batched,
+        "looped_steps_per_s": looped,
+        "batched_over_looped": batched / looped,
+        "cg_iterations_mean": float(np.mean(iters)),
+        "launch": tune,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +189,8 @@ def main():
     ap.add_argument("--cg-workgroups", type=int, default=0)
     ap.add_argument("--assemble-rows", type=int, default=0)
     ap.add_argument("--elem-lanes", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-looped measurement")
+    ap.add_argument("--batched-replays", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -210,6 +309,8 @@ def main():
         "cg_iterations_at_tol": early_iters,
         "launch": solver.launch_config(),
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(dev, args.batched_replays)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, model, args.cpu_substeps)
         out["cpu_baseline"]["host_cpus"] = os.cpu_count()

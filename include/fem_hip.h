@@ -114,8 +114,11 @@ int fem_set_params(fem_system* sys, const fem_params* params);
  * force/RHS kernel and rows per workgroup of the assembly (powers of two in
  * [4, 64]), and the persistent CG workgroup count (0 = automatic). */
 int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups);
+/* Current launch shapes; cg_launch_groups > 1 when the instances' CG
+ * workgroups do not all fit on the device at once and the solve runs as
+ * consecutive launches over groups of whole instances. */
 int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread);
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups);
 
 /* FixedParticles.restore / restore_positions (fixed_particles.py:11-21,
  * 152-168): q[i] = anchor[i] (and qd[i] = 0 when qd_dev is not NULL). */

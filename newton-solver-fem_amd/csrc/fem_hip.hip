@@ -483,6 +483,8 @@ struct CgArgs {
   const int* tile_cols;    // distinct columns of each tile, ascending
   const uint16_t* lidx;    // [nnz] tile-local column number of each block
   const unsigned* rhs_max;
+  int chunk_base;     // first chunk of this launch group
+  int nchunks_total;  // chunks of all groups (stride of the double-buffered partials)
   float tolerance;
   int maxiter;
   int early_exit;
@@ -806,12 +808,12 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   __shared__ double red[4 * NPART];
   __shared__ int bflag;
   __shared__ int xinfo[3];
-  const int c = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int c = a.chunk_base + xcd_swizzle(blockIdx.x, gridDim.x);
   const int inst = a.chunk_inst[c];
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
   const int t0 = a.chunk_tiles[c], t1 = a.chunk_tiles[c + 1];
-  const size_t nch = (size_t)gridDim.x;
+  const size_t nch = (size_t)a.nchunks_total;
   unsigned* bar = a.bar + BAR_WORDS * inst;
   float m = 0.f;
   for (int k = 0; k < RHS_MAX_SLOTS; ++k) m = fmaxf(m, __uint_as_float(a.rhs_max[RHS_MAX_SLOTS * inst + k]));
@@ -1115,6 +1117,7 @@ struct fem_system {
       *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
   uint16_t* d_lidx = nullptr;
   int nchunks = 0, cg_k = 1;
+  std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups
   // bound
   int* b_offsets = nullptr;
   int* b_cols = nullptr;
@@ -1306,10 +1309,27 @@ struct fem_system {
     if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, BLOCK, 0));
-    if (n_chunks > nb * ncu) {
-      char buf[160];
-      std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups, device holds %d", n_chunks, nb * ncu);
-      return fail(FEM_ERR_UNSUPPORTED, buf);
+    // Launch groups of whole instances, each small enough that all of its
+    // workgroups are co-resident (instances never span two groups).
+    const int cap = nb * ncu;
+    std::vector<int32_t> groups(1, 0);
+    {
+      int in_group = 0;
+      for (int k = 0; k < ninst; ++k) {
+        const int nk = inst_chunks[k + 1] - inst_chunks[k];
+        if (nk > cap) {
+          char buf[160];
+          std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups for one instance, device holds %d", nk,
+                        cap);
+          return fail(FEM_ERR_UNSUPPORTED, buf);
+        }
+        if (in_group + nk > cap) {
+          groups.push_back(inst_chunks[k]);
+          in_group = 0;
+        }
+        in_group += nk;
+      }
+      groups.push_back(n_chunks);
     }
     int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr,
         *n_tcptr = nullptr, *n_tcols = nullptr;
@@ -1343,6 +1363,7 @@ struct fem_system {
     d_lidx = n_lidx;
     d_part = n_part;
     nchunks = n_chunks;
+    cg_groups = groups;
     cg_k = k_rows;
     max_resident = nb * ncu;
     return FEM_OK;
@@ -1529,8 +1550,9 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int
 }
 
 int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread) {
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (cg_launch_groups) *cg_launch_groups = s->cg_groups.empty() ? 0 : (int32_t)s->cg_groups.size() - 1;
   if (elem_lanes) *elem_lanes = s->elem_g;
   if (assemble_rows) *assemble_rows = s->asm_rb;
   if (cg_workgroups) *cg_workgroups = s->nchunks;
@@ -1629,8 +1651,13 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
   hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, s->d_bar,
                      s->ninst, s->d_err);
   HIP_TRY(hipGetLastError());
-  void* args[] = {&ca};
-  HIP_TRY(hipLaunchKernel(kp, dim3(s->nchunks), dim3(BLOCK), args, 0, (hipStream_t)stream));
+  ca.nchunks_total = s->nchunks;
+  for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
+    ca.chunk_base = s->cg_groups[g];
+    void* args[] = {&ca};
+    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(BLOCK), args, 0,
+                            (hipStream_t)stream));
+  }
   return FEM_OK;
 }
 

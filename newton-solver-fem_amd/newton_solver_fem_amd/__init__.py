@@ -17,6 +17,7 @@ from .model import (
     build_hex_model,
     build_tet_model,
     hex_reference_data,
+    merge_tet_scenes,
 )
 
 __all__ = [
@@ -32,6 +33,7 @@ __all__ = [
     "build_hex_model",
     "build_tet_model",
     "hex_reference_data",
+    "merge_tet_scenes",
 ]
 
 

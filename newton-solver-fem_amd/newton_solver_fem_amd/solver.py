@@ -301,9 +301,70 @@ class SolverFEM:
         _lib.check(self._lib.fem_set_launch(self._sys, elem_lanes, assemble_rows, cg_workgroups), "fem_set_launch")
 
     def launch_config(self) -> dict:
-        v = [ctypes.c_int32() for _ in range(4)]
+        v = [ctypes.c_int32() for _ in range(5)]
         _lib.check(self._lib.fem_get_launch(self._sys, *[ctypes.byref(x) for x in v]), "fem_get_launch")
-        return dict(zip(("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread"), (x.value for x in v)))
+        keys = ("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread", "cg_launch_groups")
+        return dict(zip(keys, (x.value for x in v)))
+
+    def autotune(self, state: State, dt: float | None = None, *, elem_lanes=(4, 8, 16), assemble_rows=(8, 16, 32),
+                 cg_workgroups=None, repeats: int = 3, control: Control | None = None) -> dict:
+        """Pick the fastest launch shapes for this model by timing real substeps.
+
+        Each candidate runs ``refresh_tangent`` + ``step`` on copies of
+        ``state`` (the caller's state is not modified), starting from the same
+        warm start every time; element lanes, assembly rows and the CG
+        workgroup count are tuned one after another and the fastest of each
+        is kept.  The warm start is restored afterwards.  Returns the chosen
+        ``launch_config()``.
+        """
+        dt = self.dt if dt is None else float(dt)
+        P = self.model.particle_count
+        dv0 = self.dv.clone()
+        s_in, s_out = self.model.state(), self.model.state()
+        if cg_workgroups is None:
+            base = max(1, P // 512)
+            cg_workgroups = sorted({0, 128, 256, 384, 512, base})
+
+        def timed() -> float:
+            best = float("inf")
+            for r in range(repeats + 1):
+                s_in.particle_q.copy_(state.particle_q)
+                s_in.particle_qd.copy_(state.particle_qd)
+                self.dv.copy_(dv0)
+                t0 = torch.cuda.Event(enable_timing=True)
+                t1 = torch.cuda.Event(enable_timing=True)
+                t0.record()
+                self.refresh_tangent(s_in.particle_q, dt)
+                self.step(s_in, s_out, control, None, dt)
+                t1.record()
+                t1.synchronize()
+                if r > 0:
+                    best = min(best, t0.elapsed_time(t1))
+            return best
+
+        for which, cands in ((0, elem_lanes), (1, assemble_rows), (2, cg_workgroups)):
+            keep = self.launch_config()
+            results = []
+            for c in cands:
+                args = [0, 0, 0]
+                args[which] = int(c)
+                try:
+                    self.set_launch(*args)
+                except (ValueError, _lib.FEMError):
+                    continue
+                results.append((timed(), c))
+            if results:
+                _, c = min(results)
+                args = [0, 0, 0]
+                args[which] = int(c)
+                self.set_launch(*args)
+            else:
+                restore = [keep["elem_lanes"], keep["assemble_rows"], keep["cg_workgroups"]]
+                args = [0, 0, 0]
+                args[which] = restore[which]
+                self.set_launch(*args)
+        self.dv.copy_(dv0)
+        return self.launch_config()
 
     # ------------------------------------------------------------------
     def reset(self, state=None, world_mask=None, flags=None):
