@@ -137,6 +137,56 @@ void clamp_stretch(Real F[9], Real min_s, Real max_s) {
     }
 }
 
+// clamp_deformation_stretch_grad, kernels.py:447-481: adjoint of the
+// principal-stretch clamp (identity at interior stretches, spectral
+// derivative of U clamp(S) V^T otherwise).
+inline void clamp_stretch_grad(const double F[9], double mn, double mx, const double adj[9], double out[9]) {
+  if (mn < 0.0 || mx < 0.0) {
+    for (int k = 0; k < 9; ++k) out[k] = adj[k];
+    return;
+  }
+  double U[9], S[3], V[9];
+  svd3_signed(F, U, S, V);
+  if (std::min(S[0], std::min(S[1], S[2])) > mn && std::max(S[0], std::max(S[1], S[2])) < mx) {
+    for (int k = 0; k < 9; ++k) out[k] = adj[k];
+    return;
+  }
+  double clipped[3], H[9], R[9];
+  for (int i = 0; i < 3; ++i) clipped[i] = std::max(std::min(S[i], mx), mn);
+  for (int i = 0; i < 3; ++i)  // H = U^T adj V
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) s += U[3 * a + i] * adj[3 * a + b] * V[3 * b + j];
+      H[3 * i + j] = s;
+    }
+  for (int k = 0; k < 9; ++k) R[k] = 0;
+  for (int i = 0; i < 3; ++i) {
+    double slope = (S[i] > mn && S[i] < mx) ? 1.0 : 0.0;
+    R[3 * i + i] = slope * H[3 * i + i];
+    for (int j = i + 1; j < 3; ++j) {
+      double difference = S[i] - S[j];
+      double divided = slope;
+      if (std::fabs(difference) > 1.0e-6) divided = (clipped[i] - clipped[j]) / difference;
+      double total = S[i] + S[j];
+      double safe_total = std::max(total, 1.0e-6);
+      if (total < 0.0) safe_total = std::min(total, -1.0e-6);
+      double summed = (clipped[i] + clipped[j]) / safe_total;
+      double symmetric = 0.5 * divided * (H[3 * i + j] + H[3 * j + i]);
+      double skew = 0.5 * summed * (H[3 * i + j] - H[3 * j + i]);
+      R[3 * i + j] = symmetric + skew;
+      R[3 * j + i] = symmetric - skew;
+    }
+  }
+  for (int i = 0; i < 3; ++i)  // out = U R V^T
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) s += U[3 * i + a] * R[3 * a + b] * V[3 * j + b];
+      out[3 * i + j] = s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Small fixed-size helpers.
 // ---------------------------------------------------------------------------
@@ -633,6 +683,7 @@ struct Solver {
   // Construction (solver.py:318-341): rest-pose tangent, pruned triplets.
   void construct(Real dt) {
     dv.assign(P, v3{0, 0, 0});
+    q_asm = anchor;
     write_triplets(anchor.data(), dt);
     set_from_triplets(true);
     refresh_preconditioner();
@@ -641,6 +692,7 @@ struct Solver {
   // refresh_tangent solver.py:769-857 (fixed positions restored first).
   void refresh(v3* q, Real dt) {
     for (int i : fixed_idx) q[i] = anchor[i];
+    q_asm.assign(q, q + P);
     write_triplets(q, dt);
     if (!have_slots) {
       set_from_triplets(false);
@@ -805,6 +857,394 @@ struct Solver {
       if (ext) f = f + ext[i] * dt;
       out[i] = f;
     }
+  }
+
+  // ---- differentiation (differentiation.py:289-394) ----------------------
+  // The matrix of a recorded step is a fixed input: it is the one assembled
+  // at q_asm by the last refresh.  Reverse-mode products of every force term
+  // with per-node adjoint weights zw (weights of the unscaled term buffers).
+  std::vector<v3> q_asm;
+
+  static void cofactor(const Real F[9], Real C[9]) {  // C = d det(F) / dF
+    C[0] = F[4] * F[8] - F[5] * F[7];
+    C[1] = F[5] * F[6] - F[3] * F[8];
+    C[2] = F[3] * F[7] - F[4] * F[6];
+    C[3] = F[2] * F[7] - F[1] * F[8];
+    C[4] = F[0] * F[8] - F[2] * F[6];
+    C[5] = F[1] * F[6] - F[0] * F[7];
+    C[6] = F[1] * F[5] - F[2] * F[4];
+    C[7] = F[2] * F[3] - F[0] * F[5];
+    C[8] = F[0] * F[4] - F[1] * F[3];
+  }
+  void clamp_grad(const Real F[9], const Real adj[9], Real out[9]) const {
+    double Fd[9], ad[9], od[9];
+    for (int k = 0; k < 9; ++k) {
+      Fd[k] = F[k];
+      ad[k] = adj[k];
+    }
+    clamp_stretch_grad(Fd, prm.min_stretch, prm.max_stretch, ad, od);
+    for (int k = 0; k < 9; ++k) out[k] = (Real)od[k];
+  }
+
+  // Reverse of eval_tets (+ eval_tet_activations): with t the tet term
+  // buffer (restoring forces), L = sum_i zw_i . t_i; accumulates dL/dx,
+  // dL/dv, dL/dact and dL/d(mu, lambda, k_damp) per tet.
+  void vjp_tets(const v3* x, const v3* v, const Real* act, const v3* zw, v3* xb, v3* vb, Real* actb, Real* matb) const {
+    for (int e = 0; e < T; ++e) {
+      int n[4] = {tet_idx[4 * e], tet_idx[4 * e + 1], tet_idx[4 * e + 2], tet_idx[4 * e + 3]};
+      const Real mu = tet_mat[3 * e], lam = tet_mat[3 * e + 1], kd = tet_mat[3 * e + 2];
+      const Real* Dm = &tet_pose[9 * e];
+      const Real s = det3(Dm);
+      const v3 x10 = x[n[1]] - x[n[0]], x20 = x[n[2]] - x[n[0]], x30 = x[n[3]] - x[n[0]];
+      const v3 w[3] = {zw[n[1]] - zw[n[0]], zw[n[2]] - zw[n[0]], zw[n[3]] - zw[n[0]]};
+      v3 dJb[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      Real cwb = 0;
+      const v3 dJ[3] = {cross(x20, x30) * s, cross(x30, x10) * s, cross(x10, x20) * s};
+      const Real cw = dot(w[0], dJ[0]) + dot(w[1], dJ[1]) + dot(w[2], dJ[2]);
+      v3 ub[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      Real Fb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      if (!(mu == Real(0) && lam == Real(0) && kd == Real(0))) {
+        const v3 u[3] = {v[n[1]] - v[n[0]], v[n[2]] - v[n[0]], v[n[3]] - v[n[0]]};
+        const Real V = Real(1) / (s * Real(6));
+        const Real alpha = Real(1) + mu / lam - mu / (Real(4) * lam);
+        const Real muV = mu * V, lamV = lam * V, kdV = kd * V;
+        Real F[9], Fc[9], dF[9], G[9];
+        const Real ds[3][3] = {{x10.x, x20.x, x30.x}, {x10.y, x20.y, x30.y}, {x10.z, x20.z, x30.z}};
+        const Real vs[3][3] = {{u[0].x, u[1].x, u[2].x}, {u[0].y, u[1].y, u[2].y}, {u[0].z, u[1].z, u[2].z}};
+        const Real ws[3][3] = {{w[0].x, w[1].x, w[2].x}, {w[0].y, w[1].y, w[2].y}, {w[0].z, w[1].z, w[2].z}};
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            F[3 * r + c] = ds[r][0] * Dm[c] + ds[r][1] * Dm[3 + c] + ds[r][2] * Dm[6 + c];
+            dF[3 * r + c] = vs[r][0] * Dm[c] + vs[r][1] * Dm[3 + c] + vs[r][2] * Dm[6 + c];
+            G[3 * r + c] = ws[r][0] * Dm[c] + ws[r][1] * Dm[3 + c] + ws[r][2] * Dm[6 + c];
+          }
+        for (int k = 0; k < 9; ++k) Fc[k] = F[k];
+        clamp_stretch(Fc, (Real)prm.min_stretch, (Real)prm.max_stretch);
+        Real Ic = 0, GF = 0, GdF = 0;
+        for (int k = 0; k < 9; ++k) {
+          Ic += Fc[k] * Fc[k];
+          GF += G[k] * Fc[k];
+          GdF += G[k] * dF[k];
+        }
+        const Real phi = Real(1) - Real(1) / (Ic + Real(1));
+        const Real dphi = Real(1) / ((Ic + Real(1)) * (Ic + Real(1)));
+        const Real detF = det3(Fc);
+        const Real J = std::max(detF, Real(0.01));
+        const Real cv = dot(u[0], dJ[0]) + dot(u[1], dJ[1]) + dot(u[2], dJ[2]);
+        const Real ft = (J - alpha) * lamV + kdV * cv;
+        // L = -<G, P> - ft * cw,  P = muV phi Fc + kdV dF.
+        const Real ftb = -cw;
+        cwb += -ft;
+        Real Fcb[9];
+        for (int k = 0; k < 9; ++k) Fcb[k] = -muV * phi * G[k] - Real(2) * muV * dphi * GF * Fc[k];
+        if (detF > Real(0.01)) {
+          Real C[9];
+          cofactor(Fc, C);
+          for (int k = 0; k < 9; ++k) Fcb[k] += ftb * lamV * C[k];
+        }
+        clamp_grad(F, Fcb, Fb);
+        // dF/dt = Dv Dm: dL/dDv = -kdV G Dm^T; damping part of ft through cv.
+        for (int a = 0; a < 3; ++a) {
+          v3 col = {0, 0, 0};
+          Real* cc = &col.x;
+          for (int r = 0; r < 3; ++r) {
+            Real acc = 0;
+            for (int c = 0; c < 3; ++c) acc += -kdV * G[3 * r + c] * Dm[3 * a + c];
+            cc[r] = acc;
+          }
+          ub[a] = col + dJ[a] * (ftb * kdV);
+          dJb[a] = dJb[a] + u[a] * (ftb * kdV);
+        }
+        const Real phib = -muV * GF;  // dL/dphi... folded below into mu
+        (void)phib;
+        // Materials (raw, per tet).
+        const Real lamVb = ftb * (J - alpha);
+        const Real alphab = -ftb * lamV;
+        const Real muVb = -phi * GF;
+        const Real kdVb = -GdF + ftb * cv;
+        matb[3 * e] += muVb * V + alphab * (Real(0.75) / lam);
+        matb[3 * e + 1] += lamVb * V + alphab * (Real(-0.75) * mu / (lam * lam));
+        matb[3 * e + 2] += kdVb * V;
+      }
+      // Activation term t_act = -(lam act / 6) (cw / s) in adjoint form.
+      if (prm.enable_activations && act) {
+        const Real sc = lam * act[e] / Real(6);
+        cwb += -sc / s;
+        actb[e] += -(lam / Real(6)) * cw / s;
+        matb[3 * e + 1] += -(act[e] / Real(6)) * cw / s;
+      }
+      for (int a = 0; a < 3; ++a) dJb[a] = dJb[a] + w[a] * cwb;
+      // dJ1 = s x20 x x30, dJ2 = s x30 x x10, dJ3 = s x10 x x20.
+      v3 e10 = cross(dJb[1], x30) + cross(x20, dJb[2]);
+      v3 e20 = cross(dJb[2], x10) + cross(x30, dJb[0]);
+      v3 e30 = cross(dJb[0], x20) + cross(x10, dJb[1]);
+      e10 = e10 * s;
+      e20 = e20 * s;
+      e30 = e30 * s;
+      // F = Ds Dm: dL/dDs = Fb Dm^T.
+      v3 dsb[3];
+      for (int a = 0; a < 3; ++a) {
+        Real* cc = &dsb[a].x;
+        for (int r = 0; r < 3; ++r) cc[r] = Fb[3 * r] * Dm[3 * a] + Fb[3 * r + 1] * Dm[3 * a + 1] + Fb[3 * r + 2] * Dm[3 * a + 2];
+      }
+      const v3 eb[3] = {e10 + dsb[0], e20 + dsb[1], e30 + dsb[2]};
+      for (int a = 0; a < 3; ++a) {
+        xb[n[a + 1]] = xb[n[a + 1]] + eb[a];
+        xb[n[0]] = xb[n[0]] - eb[a];
+        vb[n[a + 1]] = vb[n[a + 1]] + ub[a];
+        vb[n[0]] = vb[n[0]] - ub[a];
+      }
+    }
+  }
+
+  // Reverse of eval_hexes (hex materials are fixed inputs, as in the reference).
+  void vjp_hexes(const v3* x, const v3* v, const v3* zw, v3* xb, v3* vb) const {
+    for (int e = 0; e < H; ++e) {
+      const Real mu = hex_mat[3 * e], lam = hex_mat[3 * e + 1], kd = hex_mat[3 * e + 2];
+      const Real alpha = Real(1) + mu / lam - mu / (Real(4) * lam);
+      for (int g = 0; g < 8; ++g) {
+        v3 dN[8];
+        Real F[9], Fc[9];
+        hex_gp(e, g, x, dN, F);
+        for (int k = 0; k < 9; ++k) Fc[k] = F[k];
+        clamp_stretch(Fc, (Real)prm.min_stretch, (Real)prm.max_stretch);
+        const Real wg = hex_detJ0w[8 * e + g];
+        Real Pb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // -w G, G = sum_a zw_a dN_a^T
+        for (int a = 0; a < 8; ++a) {
+          const v3 z = zw[hex_idx[8 * e + a]];
+          const Real zs[3] = {z.x, z.y, z.z}, d[3] = {dN[a].x, dN[a].y, dN[a].z};
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) Pb[3 * r + c] -= wg * zs[r] * d[c];
+        }
+        Real Ic = 0, PF = 0;
+        for (int k = 0; k < 9; ++k) {
+          Ic += Fc[k] * Fc[k];
+          PF += Pb[k] * Fc[k];
+        }
+        const Real phi = Real(1) - Real(1) / (Ic + Real(1));
+        const Real dphi = Real(1) / ((Ic + Real(1)) * (Ic + Real(1)));
+        const Real detF = det3(Fc);
+        const Real J = std::max(detF, Real(0.01));
+        Real C[9];
+        const v3 f0 = {Fc[0], Fc[3], Fc[6]}, f1 = {Fc[1], Fc[4], Fc[7]}, f2 = {Fc[2], Fc[5], Fc[8]};
+        const v3 cc0 = cross(f1, f2), cc1 = cross(f2, f0), cc2 = cross(f0, f1);
+        const Real cof[9] = {cc0.x, cc1.x, cc2.x, cc0.y, cc1.y, cc2.y, cc0.z, cc1.z, cc2.z};
+        Real Pcof = 0;
+        for (int k = 0; k < 9; ++k) Pcof += Pb[k] * cof[k];
+        Real Fcb[9];
+        for (int k = 0; k < 9; ++k) Fcb[k] = mu * phi * Pb[k] + Real(2) * mu * dphi * PF * Fc[k];
+        if (detF > Real(0.01)) {
+          cofactor(Fc, C);
+          for (int k = 0; k < 9; ++k) Fcb[k] += lam * Pcof * C[k];
+        }
+        // d<Pb, cof(F)>/dF by columns.
+        const v3 p0 = {Pb[0], Pb[3], Pb[6]}, p1 = {Pb[1], Pb[4], Pb[7]}, p2 = {Pb[2], Pb[5], Pb[8]};
+        const Real sc = lam * (J - alpha);
+        const v3 g0 = (cross(p1, f2) + cross(f1, p2)) * sc, g1 = (cross(p2, f0) + cross(f2, p0)) * sc,
+                 g2 = (cross(p0, f1) + cross(f0, p1)) * sc;
+        Fcb[0] += g0.x; Fcb[3] += g0.y; Fcb[6] += g0.z;
+        Fcb[1] += g1.x; Fcb[4] += g1.y; Fcb[7] += g1.z;
+        Fcb[2] += g2.x; Fcb[5] += g2.y; Fcb[8] += g2.z;
+        Real Fb[9];
+        clamp_grad(F, Fcb, Fb);
+        for (int a = 0; a < 8; ++a) {
+          const int n = hex_idx[8 * e + a];
+          const v3 xa = {Fb[0] * dN[a].x + Fb[1] * dN[a].y + Fb[2] * dN[a].z,
+                         Fb[3] * dN[a].x + Fb[4] * dN[a].y + Fb[5] * dN[a].z,
+                         Fb[6] * dN[a].x + Fb[7] * dN[a].y + Fb[8] * dN[a].z};
+          const v3 va = {Pb[0] * dN[a].x + Pb[1] * dN[a].y + Pb[2] * dN[a].z,
+                         Pb[3] * dN[a].x + Pb[4] * dN[a].y + Pb[5] * dN[a].z,
+                         Pb[6] * dN[a].x + Pb[7] * dN[a].y + Pb[8] * dN[a].z};
+          xb[n] = xb[n] + xa;
+          vb[n] = vb[n] + va * kd;
+        }
+      }
+    }
+  }
+
+  // Reverse of eval_ground (branches follow the forward: min/max/abs give
+  // their derivative to the first argument only when it is selected
+  // strictly, as Warp's adjoints do).
+  void vjp_ground(const v3* x, const v3* v, const v3* zw, v3* xb, v3* vb) const {
+    if (!prm.has_ground) return;
+    const v3 n = {(Real)prm.ground[0], (Real)prm.ground[1], (Real)prm.ground[2]};
+    const Real gx = (Real)prm.gravity[0], gy = (Real)prm.gravity[1], gz = (Real)prm.gravity[2];
+    const Real gmag = std::sqrt(gx * gx + gy * gy + gz * gz);
+    for (int i = 0; i < P; ++i) {
+      if ((flags[i] & 1) == 0) continue;
+      const Real c = std::min(dot(n, x[i]) + (Real)prm.ground[3] - radius[i], Real(0));
+      if (c >= Real(0)) continue;
+      const Real vn = dot(n, v[i]);
+      const Real fn = c * (Real)prm.ke + std::min(vn, Real(0)) * (Real)prm.kd;
+      v3 vt = v[i] - n * vn;
+      const Real vs = std::sqrt(dot(vt, vt));
+      const v3 that = vs > Real(0) ? vt * (Real(1) / vs) : vt;
+      const Real inv_m = inv_mass[i];
+      const Real m = inv_m > Real(0) ? Real(1) / std::max(inv_m, Real(1e-9)) : Real(0);
+      const Real n_cap = m * gmag;
+      const Real afn = std::fabs(fn);
+      const Real fn_eff = std::min(afn, n_cap);
+      const Real ftf = std::min(vs * (Real)prm.kf, (Real)prm.mu * fn_eff);
+      // t = -n fn - that ft
+      const v3 z = zw[i];
+      Real fnb = -dot(z, n);
+      const Real ftb = -dot(z, that);
+      const v3 thatb = z * (-ftf);
+      Real vsb = 0, fneb = 0;
+      if (vs * (Real)prm.kf < (Real)prm.mu * fn_eff) vsb += ftb * (Real)prm.kf;
+      else fneb += ftb * (Real)prm.mu;
+      if (afn < n_cap) fnb += fneb * (fn < Real(0) ? Real(-1) : Real(1));
+      v3 vtb;
+      if (vs > Real(0)) {
+        const Real tt = dot(that, thatb);
+        vtb = (thatb - that * tt) * (Real(1) / vs) + that * vsb;
+      } else {
+        vtb = thatb;
+      }
+      Real vnb = -dot(n, vtb);
+      v3 vbi = vtb;
+      if (vn < Real(0)) vnb += fnb * (Real)prm.kd;
+      vbi = vbi + n * vnb;
+      xb[i] = xb[i] + n * (fnb * (Real)prm.ke);
+      vb[i] = vb[i] + vbi;
+    }
+  }
+
+  void vjp_damping(const v3* zw, v3* vb) const {
+    if (!(prm.linear_damping > 0.0)) return;
+    for (int i = 0; i < P; ++i) {
+      if ((flags[i] & 1) == 0) continue;
+      vb[i] = vb[i] + zw[i] * (-(Real)prm.linear_damping * mass[i]);
+    }
+  }
+
+  // VJP of eval_terms(mask) with per-node weights zw.
+  void vjp_terms(int mask, const v3* x, const v3* v, const Real* act, const v3* zw, v3* xb, v3* vb, Real* actb,
+                 Real* matb) const {
+    if ((mask & TERM_TET) && T) vjp_tets(x, v, (prm.enable_activations ? act : nullptr), zw, xb, vb, actb, matb);
+    if ((mask & TERM_HEX) && H) vjp_hexes(x, v, zw, xb, vb);
+    if (mask & TERM_GROUND) vjp_ground(x, v, zw, xb, vb);
+    if (mask & TERM_DAMPING) vjp_damping(zw, vb);
+  }
+
+  // -z^T (dA/d mu_e) y and -z^T (dA/d lambda_e) y per tet, with A assembled
+  // at q_asm (the element tangent's block rows sum to zero, so the
+  // predictor's -(dC/dmu) v folds into y = dv + v).
+  void matrix_material_terms(const v3* z, const v3* y, Real dt, Real* matb) const {
+    const v3* x = q_asm.data();
+    for (int e = 0; e < T; ++e) {
+      int n[4] = {tet_idx[4 * e], tet_idx[4 * e + 1], tet_idx[4 * e + 2], tet_idx[4 * e + 3]};
+      if (tet_mat[3 * e] == Real(0) && tet_mat[3 * e + 1] == Real(0) && tet_mat[3 * e + 2] == Real(0)) continue;
+      const Real* Dm = &tet_pose[9 * e];
+      const Real s = det3(Dm);
+      if (s * Real(6) <= Real(0)) continue;
+      const Real V = Real(1) / (s * Real(6));
+      const v3 x10 = x[n[1]] - x[n[0]], x20 = x[n[2]] - x[n[0]], x30 = x[n[3]] - x[n[0]];
+      Real F[9], Gz[9], Gy[9];
+      const v3 wz[3] = {z[n[1]] - z[n[0]], z[n[2]] - z[n[0]], z[n[3]] - z[n[0]]};
+      const v3 wy[3] = {y[n[1]] - y[n[0]], y[n[2]] - y[n[0]], y[n[3]] - y[n[0]]};
+      const Real ds[3][3] = {{x10.x, x20.x, x30.x}, {x10.y, x20.y, x30.y}, {x10.z, x20.z, x30.z}};
+      const Real zs[3][3] = {{wz[0].x, wz[1].x, wz[2].x}, {wz[0].y, wz[1].y, wz[2].y}, {wz[0].z, wz[1].z, wz[2].z}};
+      const Real ys[3][3] = {{wy[0].x, wy[1].x, wy[2].x}, {wy[0].y, wy[1].y, wy[2].y}, {wy[0].z, wy[1].z, wy[2].z}};
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          F[3 * r + c] = ds[r][0] * Dm[c] + ds[r][1] * Dm[3 + c] + ds[r][2] * Dm[6 + c];
+          Gz[3 * r + c] = zs[r][0] * Dm[c] + zs[r][1] * Dm[3 + c] + zs[r][2] * Dm[6 + c];
+          Gy[3 * r + c] = ys[r][0] * Dm[c] + ys[r][1] * Dm[3 + c] + ys[r][2] * Dm[6 + c];
+        }
+      clamp_stretch(F, (Real)prm.min_stretch, (Real)prm.max_stretch);
+      Real Ic = 0, zy = 0, zF = 0, yF = 0;
+      for (int k = 0; k < 9; ++k) {
+        Ic += F[k] * F[k];
+        zy += Gz[k] * Gy[k];
+        zF += Gz[k] * F[k];
+        yF += Gy[k] * F[k];
+      }
+      const Real Ic1 = std::max(Ic + Real(1), Real(1e-6));
+      const Real sdev = Real(1) - Real(1) / Ic1;
+      const v3 dJ[3] = {cross(x20, x30) * s, cross(x30, x10) * s, cross(x10, x20) * s};
+      const Real cz = dot(wz[0], dJ[0]) + dot(wz[1], dJ[1]) + dot(wz[2], dJ[2]);
+      const Real cy = dot(wy[0], dJ[0]) + dot(wy[1], dJ[1]) + dot(wy[2], dJ[2]);
+      const Real h2 = dt * dt;
+      matb[3 * e] += -h2 * V * (sdev * zy + Real(2) / (Ic1 * Ic1) * zF * yF);
+      matb[3 * e + 1] += -h2 * V * cz * cy;
+    }
+  }
+
+  // Differentiable step forward (differentiation.py step): rhs = h sum terms
+  // + h ext - sum_j A_ij (v_j - v_i), A dv = rhs solved from zero to `tol`.
+  void diff_forward(const v3* q, const v3* qd, const v3* ext, const Real* act, Real dt, double tol, v3* q_out,
+                    v3* qd_out, v3* dv_out) {
+    std::vector<v3> f(P);
+    forces_scaled(q, qd, ext, act, dt, f.data());
+    if (T || H) {
+      std::vector<v3> c(P);
+      predictor(qd, c.data());
+      for (int i = 0; i < P; ++i) f[i] = f[i] - c[i];
+    }
+    std::vector<v3> x(P, v3{0, 0, 0});
+    const double keep_tol = prm.tol;
+    const int keep_max = prm.maxiter;
+    prm.tol = tol;
+    prm.maxiter = 100000;
+    cg(f, x);
+    prm.tol = keep_tol;
+    prm.maxiter = keep_max;
+    for (int i = 0; i < P; ++i) {
+      dv_out[i] = x[i];
+      qd_out[i] = qd[i] + x[i];
+      q_out[i] = q[i] + qd_out[i] * dt;
+    }
+  }
+  void predictor(const v3* qd, v3* c) const {
+    for (int i = 0; i < P; ++i) {
+      Real acc[3] = {0, 0, 0};
+      for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
+        const Real* b = &values[9 * (size_t)k];
+        v3 d = qd[columns[k]] - qd[i];
+        for (int r = 0; r < 3; ++r) acc[r] += b[3 * r] * d.x + b[3 * r + 1] * d.y + b[3 * r + 2] * d.z;
+      }
+      c[i] = {acc[0], acc[1], acc[2]};
+    }
+  }
+
+  // Reverse of diff_forward for output adjoints (qb_out, vb_out).
+  void diff_backward(const v3* q, const v3* qd, const Real* act, const v3* dv, const v3* qb_out, const v3* vb_out,
+                     Real dt, double tol, v3* qb_in, v3* vb_in, Real* actb, v3* extb, Real* matb) {
+    std::vector<v3> u(P), z(P, v3{0, 0, 0}), zw(P);
+    for (int i = 0; i < P; ++i) u[i] = vb_out[i] + qb_out[i] * dt;
+    const double keep_tol = prm.tol;
+    const int keep_max = prm.maxiter;
+    prm.tol = tol;
+    prm.maxiter = 100000;
+    cg(u, z);  // A symmetric: the adjoint operator is A
+    prm.tol = keep_tol;
+    prm.maxiter = keep_max;
+    for (int i = 0; i < P; ++i) {
+      zw[i] = z[i] * dt;
+      qb_in[i] = qb_out[i];
+      vb_in[i] = u[i];
+      extb[i] = z[i] * dt;
+    }
+    vjp_terms(TERM_TET | TERM_HEX | TERM_GROUND | TERM_DAMPING, q, qd, act, zw.data(), qb_in, vb_in, actb, matb);
+    if (T || H) {  // predictor: vb_j += -(A z)_j + (sum_k A_jk)^T z_j
+      for (int i = 0; i < P; ++i) {
+        Real az[3] = {0, 0, 0}, rs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
+          const Real* b = &values[9 * (size_t)k];
+          v3 zc = z[columns[k]];
+          for (int r = 0; r < 3; ++r) az[r] += b[3 * r] * zc.x + b[3 * r + 1] * zc.y + b[3 * r + 2] * zc.z;
+          for (int q9 = 0; q9 < 9; ++q9) rs[q9] += b[q9];
+        }
+        const v3 zi = z[i];
+        const v3 rt = {rs[0] * zi.x + rs[3] * zi.y + rs[6] * zi.z, rs[1] * zi.x + rs[4] * zi.y + rs[7] * zi.z,
+                       rs[2] * zi.x + rs[5] * zi.y + rs[8] * zi.z};
+        vb_in[i] = vb_in[i] - v3{az[0], az[1], az[2]} + rt;
+      }
+    }
+    std::vector<v3> y(P);
+    for (int i = 0; i < P; ++i) y[i] = dv[i] + qd[i];
+    matrix_material_terms(z.data(), y.data(), dt, matb);
   }
 
   void eval_terms(int mask, const v3* q, const v3* qd, const Real* act, v3* out) {
@@ -1054,6 +1494,72 @@ void orc_block_inverse(int bits, const double* B, double* out) {
   } else {
     inverse_diagonal_block<double>(B, out);
   }
+}
+
+// ---- differentiation (test infrastructure for the adjoint path) ----------
+void orc_vjp_terms(void* p, int mask, const double* q, const double* qd, const double* act, const double* zw,
+                   double* xb, double* vb, double* actb, double* matb) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    int P = S.P;
+    auto x = to_v3<Real>(q, P), v = to_v3<Real>(qd, P), z = to_v3<Real>(zw, P);
+    std::vector<V3<Real>> xo(P, V3<Real>{0, 0, 0}), vo(P, V3<Real>{0, 0, 0});
+    std::vector<Real> a, ab(S.T, Real(0)), mb(3 * (size_t)S.T, Real(0));
+    if (act) a.assign(act, act + S.T);
+    S.vjp_terms(mask, x.data(), v.data(), act ? a.data() : nullptr, z.data(), xo.data(), vo.data(), ab.data(),
+                mb.data());
+    from_v3(xo, xb);
+    from_v3(vo, vb);
+    for (int e = 0; e < S.T; ++e) actb[e] = ab[e];
+    for (size_t k = 0; k < mb.size(); ++k) matb[k] = mb[k];
+  });
+}
+
+void orc_diff_forward(void* p, const double* q, const double* qd, const double* ext, const double* act, double dt,
+                      double tol, double* q_out, double* qd_out, double* dv_out) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    int P = S.P;
+    auto x = to_v3<Real>(q, P), v = to_v3<Real>(qd, P);
+    std::vector<V3<Real>> e, qo(P), vo(P), d(P);
+    if (ext) e = to_v3<Real>(ext, P);
+    std::vector<Real> a;
+    if (act) a.assign(act, act + S.T);
+    S.diff_forward(x.data(), v.data(), ext ? e.data() : nullptr, act ? a.data() : nullptr, (Real)dt, tol, qo.data(),
+                   vo.data(), d.data());
+    from_v3(qo, q_out);
+    from_v3(vo, qd_out);
+    from_v3(d, dv_out);
+  });
+}
+
+void orc_diff_backward(void* p, const double* q, const double* qd, const double* act, const double* dv,
+                       const double* qb_out, const double* vb_out, double dt, double tol, double* qb_in, double* vb_in,
+                       double* actb, double* extb, double* matb) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    int P = S.P;
+    auto x = to_v3<Real>(q, P), v = to_v3<Real>(qd, P), d = to_v3<Real>(dv, P);
+    auto qo = to_v3<Real>(qb_out, P), vo = to_v3<Real>(vb_out, P);
+    std::vector<V3<Real>> qi(P), vi(P), eb(P);
+    std::vector<Real> a, ab(S.T, Real(0)), mb(3 * (size_t)S.T, Real(0));
+    if (act) a.assign(act, act + S.T);
+    S.diff_backward(x.data(), v.data(), act ? a.data() : nullptr, d.data(), qo.data(), vo.data(), (Real)dt, tol,
+                    qi.data(), vi.data(), ab.data(), eb.data(), mb.data());
+    from_v3(qi, qb_in);
+    from_v3(vi, vb_in);
+    from_v3(eb, extb);
+    for (int e = 0; e < S.T; ++e) actb[e] = ab[e];
+    for (size_t k = 0; k < mb.size(); ++k) matb[k] = mb[k];
+  });
+}
+
+// Replace the tet materials (mu, lambda, k_damp per tet); callers re-assemble.
+void orc_set_tet_materials(void* p, const double* mat) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    for (size_t k = 0; k < S.tet_mat.size(); ++k) S.tet_mat[k] = (Real)mat[k];
+  });
 }
 
 }  // extern "C"

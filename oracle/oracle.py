@@ -56,6 +56,11 @@ def lib():
         _lib.orc_clamp_stretch.argtypes = [ctypes.c_int, d, ctypes.c_double, ctypes.c_double, d]
         _lib.orc_svd3.argtypes = [d, d, d, d]
         _lib.orc_block_inverse.argtypes = [ctypes.c_int, d, d]
+        vp = ctypes.c_void_p
+        _lib.orc_vjp_terms.argtypes = [vp, ctypes.c_int, d, d, vp, d, d, d, d, d]
+        _lib.orc_diff_forward.argtypes = [vp, d, d, vp, vp, ctypes.c_double, ctypes.c_double, d, d, d]
+        _lib.orc_diff_backward.argtypes = [vp, d, d, vp, d, d, d, ctypes.c_double, ctypes.c_double, d, d, d, d, d]
+        _lib.orc_set_tet_materials.argtypes = [vp, d]
     return _lib
 
 
@@ -229,3 +234,60 @@ class OracleSolver:
         lib().orc_eval_terms(self.h, int(mask), q.ctypes.data_as(dp), qd.ctypes.data_as(dp),
                              None if act is None else act.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(dp))
         return out
+
+    # ---- differentiation (fixed-matrix linearisation, differentiation.py) ----
+    @staticmethod
+    def _arr(a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        return a if shape is None else a.reshape(shape)
+
+    def vjp_terms(self, mask, q, qd, weights, activations=None):
+        """Reverse-mode product of eval_terms(mask) with per-node weights.
+
+        Returns (dL/dq, dL/dqd, dL/dact [T], dL/d(mu, lambda, k_damp) [T, 3]) for
+        L = sum_i weights_i . eval_terms(mask)_i.
+        """
+        P, T = self.P, self.T
+        dp = ctypes.POINTER(ctypes.c_double)
+        q, qd, w = self._arr(q, (P, 3)), self._arr(qd, (P, 3)), self._arr(weights, (P, 3))
+        act = None if activations is None else self._arr(activations)
+        xb, vb, ab, mb = np.zeros((P, 3)), np.zeros((P, 3)), np.zeros(max(T, 1)), np.zeros((max(T, 1), 3))
+        lib().orc_vjp_terms(self.h, int(mask), q.ctypes.data_as(dp), qd.ctypes.data_as(dp),
+                            None if act is None else act.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(dp),
+                            xb.ctypes.data_as(dp), vb.ctypes.data_as(dp), ab.ctypes.data_as(dp), mb.ctypes.data_as(dp))
+        return xb, vb, ab[:T], mb[:T]
+
+    def diff_step(self, q, qd, external_forces=None, activations=None, tol=1e-13):
+        """Differentiable step with the current (fixed) matrix: returns (q_out, qd_out, dv)."""
+        P = self.P
+        dp = ctypes.POINTER(ctypes.c_double)
+        q, qd = self._arr(q, (P, 3)), self._arr(qd, (P, 3))
+        ext = None if external_forces is None else self._arr(external_forces, (P, 3))
+        act = None if activations is None else self._arr(activations)
+        qo, vo, dv = np.zeros((P, 3)), np.zeros((P, 3)), np.zeros((P, 3))
+        lib().orc_diff_forward(self.h, q.ctypes.data_as(dp), qd.ctypes.data_as(dp),
+                               None if ext is None else ext.ctypes.data_as(ctypes.c_void_p),
+                               None if act is None else act.ctypes.data_as(ctypes.c_void_p), self.dt, float(tol),
+                               qo.ctypes.data_as(dp), vo.ctypes.data_as(dp), dv.ctypes.data_as(dp))
+        return qo, vo, dv
+
+    def diff_step_backward(self, q, qd, dv, q_out_grad, qd_out_grad, activations=None, tol=1e-13):
+        """Adjoint of diff_step: returns dict(q, qd, act, ext, mat) input gradients."""
+        P, T = self.P, self.T
+        dp = ctypes.POINTER(ctypes.c_double)
+        q, qd, dv = self._arr(q, (P, 3)), self._arr(qd, (P, 3)), self._arr(dv, (P, 3))
+        go, gv = self._arr(q_out_grad, (P, 3)), self._arr(qd_out_grad, (P, 3))
+        act = None if activations is None else self._arr(activations)
+        qb, vb, eb = np.zeros((P, 3)), np.zeros((P, 3)), np.zeros((P, 3))
+        ab, mb = np.zeros(max(T, 1)), np.zeros((max(T, 1), 3))
+        lib().orc_diff_backward(self.h, q.ctypes.data_as(dp), qd.ctypes.data_as(dp),
+                                None if act is None else act.ctypes.data_as(ctypes.c_void_p), dv.ctypes.data_as(dp),
+                                go.ctypes.data_as(dp), gv.ctypes.data_as(dp), self.dt, float(tol),
+                                qb.ctypes.data_as(dp), vb.ctypes.data_as(dp), ab.ctypes.data_as(dp),
+                                eb.ctypes.data_as(dp), mb.ctypes.data_as(dp))
+        return {"q": qb, "qd": vb, "act": ab[:T], "ext": eb, "mat": mb[:T]}
+
+    def set_tet_materials(self, materials):
+        """Replace per-tet (mu, lambda, k_damp); re-assemble with refresh_tangent afterwards."""
+        m = self._arr(materials, (self.T, 3))
+        lib().orc_set_tet_materials(self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
