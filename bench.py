@@ -175,6 +175,84 @@ def bench_batched(dev, replays: int):
     }
 
 
+def bench_adjoint(dev, repeats: int = 3):
+    """Differentiable bar: gradient of a tip-target loss w.r.t. per-tet activations.
+
+    40x8x8-cell Tet4 bar (12,800 tets), activations U(0, 0.3) (seed 2), base
+    face pinned the way the reference pins differentiable supports (zero mass
+    + projection after every substep), 60 substeps with the fixed matrix,
+    checked solves (tol 1e-8).  forward = the rollout without a tape;
+    gradient = rollout with checkpoints every 10 substeps + backward
+    (recomputation + adjoint solves + reverse products).
+    """
+    import numpy as np
+    import torch
+
+    from newton_solver_fem_amd import build_tet_model, meshes
+    from newton_solver_fem_amd.differentiation import rollout
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    sc = meshes.tet_bar_scene()
+    base = np.zeros(len(sc.points), bool)
+    base[sc.fixed] = True
+    sc.masses = np.where(base, 0.0, sc.masses)
+    sc.fixed = np.zeros(0, np.int32)
+    model = build_tet_model(sc, device=dev)
+    solver = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0,
+                       differentiable=True, enable_tet_activations=True, defer_differentiation_checks=True,
+                       linear_solver_maxiter=256)
+    rest = model.particle_q.clone()
+    mask = torch.tensor(base, device=dev)[:, None]
+
+    def project(q, v):
+        return torch.where(mask, rest, q), torch.where(mask, torch.zeros_like(v), v)
+
+    tip = rest[:, 0] > rest[:, 0].max() - 1e-6
+    target = rest[tip].mean(0) + torch.tensor(np.random.default_rng(3).uniform(-0.01, 0.01, 3),
+                                              dtype=torch.float32, device=dev)
+    act0 = torch.tensor(sc.activations, dtype=torch.float32, device=dev)
+    steps, k = 60, 10
+
+    def forward():
+        with torch.no_grad():
+            q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps, activations=act0, after_step=project)
+            return ((q[tip].mean(0) - target) ** 2).sum()
+
+    def gradient():
+        a = act0.clone().requires_grad_(True)
+        q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps, activations=a, checkpoint_every=k,
+                       after_step=project)
+        loss = ((q[tip].mean(0) - target) ** 2).sum()
+        loss.backward()
+        return a.grad
+
+    forward()
+    gradient()
+    torch.cuda.synchronize()
+    tf, tg = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        forward()
+        torch.cuda.synchronize()
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        g = gradient()
+        torch.cuda.synchronize()
+        tg.append(time.perf_counter() - t0)
+    res = solver.check_differentiation()
+    if not torch.isfinite(g).all():
+        raise RuntimeError("non-finite activation gradient")
+    fwd, grad = min(tf), min(tg)
+    return {
+        "workload": "differentiable Tet4 bar 40x8x8 cells (12,800 tets), d(tip-target loss)/d(activations), "
+                    "60 substeps, checkpoint every 10, checked solves tol 1e-8",
+        "forward_ms": fwd * 1e3,
+        "gradient_ms": grad * 1e3,
+        "adjoint_over_forward": grad / fwd,
+        "residuals": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +269,7 @@ def main():
     ap.add_argument("--elem-lanes", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-looped measurement")
     ap.add_argument("--batched-replays", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -311,6 +390,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = bench_batched(dev, args.batched_replays)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, model, args.cpu_substeps)
         out["cpu_baseline"]["host_cpus"] = os.cpu_count()

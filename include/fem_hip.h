@@ -166,6 +166,47 @@ int fem_eval_forces(fem_system* sys, int32_t terms, const float* q_dev, const fl
 int fem_solve_stats(fem_system* sys, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,
                     int32_t* error_flags);
 
+/* ---- Differentiable substep (differentiation.py:149-394) -----------------
+ * The matrix of a recorded step is the current assembly (a fixed input, as in
+ * the reference's MatrixSnapshot).  The forward right-hand side of a
+ * differentiable step is fem_step_rhs's (no fixed particles in this mode).
+ *
+ * MatrixSnapshot.solve (:208-232): x = A^-1 b by the block-Jacobi PCG from a
+ * zero start (maximum_rhs_component / normalize_linear_rhs /
+ * restore_linear_increment), relative tolerance `tolerance`, atol 0, device
+ * residual exit; then the FP64 residual ||A x - b|| / ||b|| and its ratio to
+ * the budget 2e-5 ||b|| + 1e-10 update running maxima for `kind`
+ * (0 = forward, 1 = adjoint; residual_norms / record_residual :118-146).
+ * A is symmetric, so the adjoint solve (bsr_transposed) uses the same call. */
+int fem_solve(fem_system* sys, const float* b_dev, float* x_dev, float tolerance, int32_t maxiter, int32_t kind,
+              void* stream);
+
+/* Reverse products of a differentiable step for the adjoint z = A^-1 dv_bar
+ * (what the reference's tape computes through sum_forces / form_rhs and the
+ * force kernels, :62-111, with clamp_deformation_stretch_grad kernels.py:447-481):
+ *   q_bar  = h (d terms / d q)^T z                       (written, P x 3)
+ *   qd_bar = h (d terms / d qd)^T z - A z + rowsum(A)^T z  (written, P x 3)
+ *   act_bar[e] = h (d t_tet / d act_e)^T z               (written if not NULL)
+ *   mat_bar[e] = d/d(mu, lambda, k_damp)_e of  h z^T terms  - z^T (dA/d(mu, lambda))(dv + qd),
+ *     with A assembled at q_asm (written if not NULL; needs dv and q_asm).
+ * The reference has no material gradients (tet_materials must not require
+ * gradients, differentiation.py:244-262); mat_bar extends it. */
+int fem_vjp(fem_system* sys, const float* q_dev, const float* qd_dev, const float* act_dev, const float* z_dev,
+            const float* dv_dev, const float* q_asm_dev, float dt, float* q_bar_dev, float* qd_bar_dev,
+            float* act_bar_dev, float* mat_bar_dev, void* stream);
+
+/* Replace the per-tet (mu, lambda, k_damp) copied at creation with the T x 3
+ * device array mat_dev (a material-optimisation step); the next fem_assemble
+ * uses them.  Tets must keep their zero / non-zero material status (the
+ * pattern excluded zero-material tets).  No reference counterpart: there the
+ * materials are fixed model inputs. */
+int fem_set_tet_materials(fem_system* sys, const float* mat_dev, void* stream);
+
+/* check() (:269-279): synchronizes and copies the running maxima
+ * {forward ratio, adjoint ratio, forward violation, adjoint violation};
+ * reset != 0 clears them. */
+int fem_diff_stats(fem_system* sys, void* stream, double* out4, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

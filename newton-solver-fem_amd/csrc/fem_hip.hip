@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "fem_hip.h"
+#include "fem_adjoint.hpp"
 #include "fem_math.hpp"
 #include "fem_pattern.hpp"
 
@@ -218,6 +219,73 @@ __device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* _
   for (int b = 0; b < 8; ++b)
 #pragma unroll
     for (int k = 0; k < 9; ++k) K[b][k] = K[b][k] * h2;
+}
+
+// Reverse of the Hex8 force for corner la: dL/dx_la and dL/dv_la for
+// L = sum_b zw_b . t_b over the element's restoring forces
+// (eval_hexahedra kernels.py:1481-1601 under the tape).
+__device__ __forceinline__ void hex_node_vjp(const RowArgs& a, const float* __restrict__ x,
+                                             const float* __restrict__ zw, float zscale, int e, int la, v3& xb,
+                                             v3& vb) {
+  float px[8], py[8], pz[8], zx[8], zy[8], zz[8];
+  hex_gather(a.hex_idx + 8 * (size_t)e, x, px, py, pz);
+  hex_gather(a.hex_idx + 8 * (size_t)e, zw, zx, zy, zz);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    zx[b] *= zscale;
+    zy[b] *= zscale;
+    zz[b] *= zscale;
+  }
+  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1], k_damp = a.hex_mat[3 * e + 2];
+  const float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
+  xb = v3{0.f, 0.f, 0.f};
+  vb = xb;
+  for (int g = 0; g < 8; ++g) {
+    float gx[8], gy[8], gz[8];
+    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
+    float F[9], Fc[9], G[9];
+    hex_F(px, py, pz, gx, gy, gz, F);
+    hex_F(zx, zy, zz, gx, gy, gz, G);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fc[k] = F[k];
+    clamp_stretch(Fc, a.mn, a.mx);
+    const float w = a.hex_detw[8 * (size_t)e + g];
+    float Pb[9], cof[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Pb[k] = -w * G[k];
+    cofactor(Fc, cof);
+    float ic = 0.f, PF = 0.f, Pcof = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      ic += Fc[k] * Fc[k];
+      PF += Pb[k] * Fc[k];
+      Pcof += Pb[k] * cof[k];
+    }
+    const float phi = 1.f - 1.f / (ic + 1.f);
+    const float dphi = 1.f / ((ic + 1.f) * (ic + 1.f));
+    const float detF = det3(Fc);
+    const float J = fmaxf(detF, 0.01f);
+    float Fcb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fcb[k] = k_mu * phi * Pb[k] + 2.f * k_mu * dphi * PF * Fc[k];
+    if (detF > 0.01f) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fcb[k] += k_lambda * Pcof * cof[k];
+    }
+    const v3 f0 = {Fc[0], Fc[3], Fc[6]}, f1 = {Fc[1], Fc[4], Fc[7]}, f2 = {Fc[2], Fc[5], Fc[8]};
+    const v3 p0 = {Pb[0], Pb[3], Pb[6]}, p1 = {Pb[1], Pb[4], Pb[7]}, p2 = {Pb[2], Pb[5], Pb[8]};
+    const float sc = k_lambda * (J - alpha);
+    const v3 g0 = (cross(p1, f2) + cross(f1, p2)) * sc, g1 = (cross(p2, f0) + cross(f2, p0)) * sc,
+             g2 = (cross(p0, f1) + cross(f0, p1)) * sc;
+    Fcb[0] += g0.x; Fcb[3] += g0.y; Fcb[6] += g0.z;
+    Fcb[1] += g1.x; Fcb[4] += g1.y; Fcb[7] += g1.z;
+    Fcb[2] += g2.x; Fcb[5] += g2.y; Fcb[8] += g2.z;
+    float Fb[9];
+    clamp_stretch_grad(F, a.mn, a.mx, Fcb, Fb);
+    const v3 dNa = hex_grad_corner(a.hex_invJ + 72 * (size_t)e + 9 * g, g, la);
+    xb = xb + mv(Fb, dNa);
+    vb = vb + mv(Pb, dNa) * k_damp;
+  }
 }
 
 template <int G>
@@ -456,6 +524,177 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the differentiable step (differentiation.py:289-394): for the
+// solved adjoint z = A^-1 dv_bar, the reverse products of
+// rhs = h * sum_terms(x, v, act) + h * ext - sum_j A_ij (v_j - v_i).
+// ---------------------------------------------------------------------------
+// Per node i (G lanes over its element incidences): xbar_i = h dT/dx^T z and
+// vbar_i = h dT/dv^T z - (A z)_i + (sum_k A_ik)^T z_i.  Written, not added.
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __restrict__ x,
+                                                    const float* __restrict__ v, const float* __restrict__ act,
+                                                    const float* __restrict__ z, const float* __restrict__ vals,
+                                                    float h, float* __restrict__ xbar, float* __restrict__ vbar) {
+  constexpr int NG = BLOCK / G;
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int i = blk * NG + grp;
+  const bool active = i < a.P;
+  v3 xb = {0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f}, az = {0.f, 0.f, 0.f};
+  float rs[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) rs[k] = 0.f;
+  if (active) {
+    const int t1 = a.tinc_ptr[i + 1];
+    for (int t = a.tinc_ptr[i] + lane; t < t1; t += G) {
+      const int code = a.tinc[t];
+      const int e = code >> 2, la = code & 3;
+      const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+      const TetVjp o = tet_vjp(ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w), ld3(v, nd.x), ld3(v, nd.y),
+                               ld3(v, nd.z), ld3(v, nd.w), ld3(z, nd.x) * h, ld3(z, nd.y) * h, ld3(z, nd.z) * h,
+                               ld3(z, nd.w) * h, a.tet_pose + 9 * (size_t)e, a.tet_mat[3 * e], a.tet_mat[3 * e + 1],
+                               a.tet_mat[3 * e + 2], a.use_act && act, act ? act[e] : 0.f, a.mn, a.mx);
+      const v3 x0 = (o.xe[0] + o.xe[1] + o.xe[2]) * -1.f, v0 = (o.ve[0] + o.ve[1] + o.ve[2]) * -1.f;
+      xb = xb + sel4(la, x0, o.xe[0], o.xe[1], o.xe[2]);
+      vb = vb + sel4(la, v0, o.ve[0], o.ve[1], o.ve[2]);
+    }
+    const int h1 = a.hinc_ptr[i + 1];
+    for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
+      const int code = a.hinc[t];
+      v3 hx, hv;
+      hex_node_vjp(a, x, z, h, code >> 3, code & 7, hx, hv);
+      xb = xb + hx;
+      vb = vb + hv;
+    }
+    const int k1 = a.offsets[i + 1];
+    for (int k = a.offsets[i] + lane; k < k1; k += G) {
+      const float* b = vals + 9 * (size_t)k;
+      const v3 zc = ld3(z, a.cols[k]);
+      az = az + v3{b[0] * zc.x + b[1] * zc.y + b[2] * zc.z, b[3] * zc.x + b[4] * zc.y + b[5] * zc.z,
+                   b[6] * zc.x + b[7] * zc.y + b[8] * zc.z};
+#pragma unroll
+      for (int q = 0; q < 9; ++q) rs[q] += b[q];
+    }
+  }
+  xb = {group_sum<G>(xb.x), group_sum<G>(xb.y), group_sum<G>(xb.z)};
+  vb = {group_sum<G>(vb.x), group_sum<G>(vb.y), group_sum<G>(vb.z)};
+  az = {group_sum<G>(az.x), group_sum<G>(az.y), group_sum<G>(az.z)};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rs[q] = group_sum<G>(rs[q]);
+  if (active && lane == 0) {
+    const v3 zi = ld3(z, i);
+    if ((a.flags[i] & 1) != 0) {
+      if (a.has_ground) {
+        v3 gx, gv;
+        ground_vjp(ld3(x, i), ld3(v, i), zi * h, a.radius[i], a.inv_mass[i], a.gn, a.ke, a.kd, a.kf, a.gmu, a.gmag,
+                   gx, gv);
+        xb = xb + gx;
+        vb = vb + gv;
+      }
+      if (a.lin_damp > 0.f) vb = vb + zi * (-a.lin_damp * a.mass[i] * h);
+    }
+    const v3 rt = {rs[0] * zi.x + rs[3] * zi.y + rs[6] * zi.z, rs[1] * zi.x + rs[4] * zi.y + rs[7] * zi.z,
+                   rs[2] * zi.x + rs[5] * zi.y + rs[8] * zi.z};
+    vb = vb - az + rt;
+    st3(xbar, i, xb);
+    st3(vbar, i, vb);
+  }
+}
+
+// Per Tet4: dL/d activation and dL/d(mu, lambda, k_damp), the latter with the
+// matrix terms -z^T (dA/dmu) (dv + v) of the fixed matrix assembled at xa.
+__global__ __launch_bounds__(BLOCK) void k_vjp_tets(RowArgs a, int T, const float* __restrict__ x,
+                                                    const float* __restrict__ v, const float* __restrict__ act,
+                                                    const float* __restrict__ z, const float* __restrict__ dv,
+                                                    const float* __restrict__ xa, float h,
+                                                    float* __restrict__ act_bar, float* __restrict__ mat_bar) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= T) return;
+  const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+  const float mu = a.tet_mat[3 * e], lam = a.tet_mat[3 * e + 1], kd = a.tet_mat[3 * e + 2];
+  const v3 z0 = ld3(z, nd.x), z1 = ld3(z, nd.y), z2 = ld3(z, nd.z), z3 = ld3(z, nd.w);
+  const v3 v0 = ld3(v, nd.x), v1 = ld3(v, nd.y), v2 = ld3(v, nd.z), v3_ = ld3(v, nd.w);
+  const TetVjp o = tet_vjp(ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w), v0, v1, v2, v3_, z0 * h, z1 * h,
+                           z2 * h, z3 * h, a.tet_pose + 9 * (size_t)e, mu, lam, kd, a.use_act && act,
+                           act ? act[e] : 0.f, a.mn, a.mx);
+  if (act_bar) act_bar[e] = o.act;
+  if (mat_bar) {
+    float mub = 0.f, lamb = 0.f;
+    if (!(mu == 0.f && lam == 0.f && kd == 0.f)) {
+      const v3 y0 = ld3(dv, nd.x) + v0, y1 = ld3(dv, nd.y) + v1, y2 = ld3(dv, nd.z) + v2, y3 = ld3(dv, nd.w) + v3_;
+      tet_matrix_material_terms(ld3(xa, nd.x), ld3(xa, nd.y), ld3(xa, nd.z), ld3(xa, nd.w), z1 - z0, z2 - z0, z3 - z0,
+                                y1 - y0, y2 - y0, y3 - y0, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, h * h, mub, lamb);
+    }
+    mat_bar[3 * (size_t)e] = o.mat[0] + mub;
+    mat_bar[3 * (size_t)e + 1] = o.mat[1] + lamb;
+    mat_bar[3 * (size_t)e + 2] = o.mat[2];
+  }
+}
+
+// max |b| per instance into the 64 spread words (maximum_rhs_component).
+__global__ __launch_bounds__(BLOCK) void k_absmax(const float* __restrict__ b, int P, const int* __restrict__ row_inst,
+                                                  unsigned* rhs_max) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  float m = 0.f;
+  if (i < P) {
+    const v3 bi = ld3(b, i);
+    m = fmaxf(fabsf(bi.x), fmaxf(fabsf(bi.y), fabsf(bi.z)));
+  }
+  if (row_inst) {
+    if (i < P && m > 0.f) atomicMax(rhs_max + RHS_MAX_SLOTS * row_inst[i] + (i & (RHS_MAX_SLOTS - 1)), __float_as_uint(m));
+    return;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0 && m > 0.f)
+    atomicMax(rhs_max + ((blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) & (RHS_MAX_SLOTS - 1)), __float_as_uint(m));
+}
+
+__global__ void k_zero_doubles(double* p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+// FP64 ||A x - b||^2 and ||b||^2 (residual_norms differentiation.py:118-133).
+__global__ __launch_bounds__(BLOCK) void k_resnorm(const int* __restrict__ offsets, const int* __restrict__ cols,
+                                                   const float* __restrict__ vals, const float* __restrict__ x,
+                                                   const float* __restrict__ b, int P, double* norms) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (i < P) {
+    v3 ax = {0.f, 0.f, 0.f};
+    for (int k = offsets[i]; k < offsets[i + 1]; ++k) ax = ax + mv(vals + 9 * (size_t)k, ld3(x, cols[k]));
+    const v3 bi = ld3(b, i);
+    const double r0 = (double)ax.x - bi.x, r1 = (double)ax.y - bi.y, r2 = (double)ax.z - bi.z;
+    rr = r0 * r0 + r1 * r1 + r2 * r2;
+    bb = (double)bi.x * bi.x + (double)bi.y * bi.y + (double)bi.z * bi.z;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o);
+    bb += __shfl_xor(bb, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(norms, rr);
+    atomicAdd(norms + 1, bb);
+  }
+}
+
+// record_residual differentiation.py:136-146: running max of the relative
+// residual (stats[kind]) and of residual / (2e-5 ||b|| + 1e-10) (stats[2+kind]).
+__global__ void k_record(const double* norms, int kind, double* stats) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double r = sqrt(norms[0]), sc = sqrt(norms[1]);
+  double ratio = r / fmax(sc, 1.0e-30);
+  double violation = r / (2.0e-5 * sc + 1.0e-10);
+  if (!isfinite(r) || !isfinite(sc)) ratio = violation = 1.0e30;
+  // Non-negative doubles order like their bit patterns.
+  atomicMax(reinterpret_cast<unsigned long long*>(stats) + kind, (unsigned long long)__double_as_longlong(ratio));
+  atomicMax(reinterpret_cast<unsigned long long*>(stats) + 2 + kind,
+            (unsigned long long)__double_as_longlong(violation));
+}
+
+// ---------------------------------------------------------------------------
 // Persistent PCG.
 // ---------------------------------------------------------------------------
 struct CgArgs {
@@ -471,6 +710,7 @@ struct CgArgs {
   const float* qd_in;
   float* q_out;
   float* qd_out;
+  float* x_out;  // solve mode: x = A^-1 rhs from a zero start, no state update
   const float* anchor;
   float h;
   float* xw;  // 2 x P exchange records (double buffered)
@@ -818,7 +1058,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   float m = 0.f;
   for (int k = 0; k < RHS_MAX_SLOTS; ++k) m = fmaxf(m, __uint_as_float(a.rhs_max[RHS_MAX_SLOTS * inst + k]));
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
-  const bool zero_start = (m == 0.f);
+  const bool zero_start = (m == 0.f) || a.x_out != nullptr;
 
   for (int r = threadIdx.x; r <= nr; r += BLOCK) off_l[r] = a.offsets[R0 + r];
 
@@ -994,7 +1234,9 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   // FixedParticles.restore(state_out) solver.py:1350-1351.
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (own[k]) {
+    if (own[k] && a.x_out) {
+      st3(a.x_out, R0 + threadIdx.x + BLOCK * k, v3{x[k][0], x[k][1], x[k][2]} * (1.f / s));
+    } else if (own[k]) {
       const int i = R0 + threadIdx.x + BLOCK * k;
       const v3 dvi = v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
       st3(a.dv, i, dvi);
@@ -1075,6 +1317,17 @@ struct LaunchRhs {
   }
 };
 
+template <int G>
+struct LaunchVjpRows {
+  static int run(const RowArgs& ra, const float* x, const float* v, const float* act, const float* z,
+                 const float* vals, float h, float* xbar, float* vbar, hipStream_t st) {
+    int nb = padded_blocks(ra.P, BLOCK / G);
+    hipLaunchKernelGGL(k_vjp_rows<G>, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, z, vals, h, xbar, vbar);
+    HIP_TRY(hipGetLastError());
+    return FEM_OK;
+  }
+};
+
 const void* cg_kernel_ptr(int k) {
   switch (k) {
     case 1: return (const void*)k_cg<1>;
@@ -1111,6 +1364,7 @@ struct fem_system {
   // solve scratch
   float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_bar = nullptr;
+  double *d_norms = nullptr, *d_dstats = nullptr;  // checked-solve residual norms and running maxima
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
@@ -1463,7 +1717,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
       (rc = s->alloc(&s->d_xw, 2 * (size_t)XW * P)) ||
       (rc = s->alloc(&s->d_rhs_max, (size_t)RHS_MAX_SLOTS * s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
-      (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)))
+      (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)) ||
+      (rc = s->alloc(&s->d_norms, 2)) || (rc = s->alloc(&s->d_dstats, 4)))
     return cleanup(rc);
   double avg_inc = P ? (double)(s->pt.tinc_ptr[P] + s->pt.hinc_ptr[P]) / P : 1.0;
   double avg_nnz = P ? (double)s->pt.offsets[P] / P : 1.0;
@@ -1608,27 +1863,20 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }
 
-int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
-                   float* qd_out, float dt, void* stream) {
-  if (!s || !q_in || !qd_in || !f_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
-  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
-  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
-  if (s->P == 0) return FEM_OK;
+}  // extern "C"
+
+namespace {
+CgArgs cg_args(const fem_system* s) {
   CgArgs ca;
+  std::memset(&ca, 0, sizeof ca);
   ca.P = s->P;
   ca.offsets = s->d_offsets;
   ca.cols = s->d_cols;
   ca.vals = s->b_vals;
   ca.minv = s->b_minv;
   ca.row_fixed = s->d_row_fixed;
-  ca.rhs = f_in;
   ca.dv = s->b_dv;
-  ca.q_in = q_in;
-  ca.qd_in = qd_in;
-  ca.q_out = q_out;
-  ca.qd_out = qd_out;
   ca.anchor = s->d_anchor;
-  ca.h = dt;
   ca.xw = s->d_xw;
   ca.chunk_rows = s->d_chunk_rows;
   ca.chunk_tiles = s->d_chunk_tiles;
@@ -1646,17 +1894,111 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
   ca.bar = s->d_bar;
   ca.stats = s->d_stats;
   ca.err = s->d_err;
+  ca.nchunks_total = s->nchunks;
+  return ca;
+}
+
+// Barrier reset + one persistent launch per group of whole instances.
+int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
   const void* kp = cg_kernel_ptr(s->cg_k);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
-  hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, s->d_bar,
-                     s->ninst, s->d_err);
+  hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_bar, s->ninst,
+                     s->d_err);
   HIP_TRY(hipGetLastError());
-  ca.nchunks_total = s->nchunks;
   for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
     ca.chunk_base = s->cg_groups[g];
     void* args[] = {&ca};
-    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(BLOCK), args, 0,
-                            (hipStream_t)stream));
+    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(BLOCK), args, 0, st));
+  }
+  return FEM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
+                   float* qd_out, float dt, void* stream) {
+  if (!s || !q_in || !qd_in || !f_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  CgArgs ca = cg_args(s);
+  ca.rhs = f_in;
+  ca.q_in = q_in;
+  ca.qd_in = qd_in;
+  ca.q_out = q_out;
+  ca.qd_out = qd_out;
+  ca.h = dt;
+  return launch_cg(s, ca, (hipStream_t)stream);
+}
+
+int fem_solve(fem_system* s, const float* b, float* x, float tolerance, int32_t maxiter, int32_t kind,
+              void* stream) {
+  if (!s || !b || !x) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(tolerance > 0.f) || !std::isfinite(tolerance)) return fail(FEM_ERR_ARG, "tolerance must be finite and positive");
+  if (maxiter < 0) return fail(FEM_ERR_ARG, "maxiter must be >= 0");
+  if (kind != 0 && kind != 1) return fail(FEM_ERR_ARG, "kind must be 0 (forward) or 1 (adjoint)");
+  if (s->P == 0) return FEM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int nz = std::max(s->ninst * RHS_MAX_SLOTS, 1);
+  hipLaunchKernelGGL(k_prepare, dim3((nz + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const int*)nullptr, 0,
+                     (const float*)nullptr, (float*)nullptr, (float*)nullptr, s->d_rhs_max, s->ninst);
+  hipLaunchKernelGGL(k_absmax, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b, s->P,
+                     s->ninst > 1 ? (const int*)s->d_row_inst : (const int*)nullptr, s->d_rhs_max);
+  HIP_TRY(hipGetLastError());
+  CgArgs ca = cg_args(s);
+  ca.rhs = b;
+  ca.x_out = x;
+  ca.tolerance = tolerance;
+  ca.maxiter = maxiter;
+  ca.early_exit = 1;
+  int rc = launch_cg(s, ca, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, s->d_norms, 2);
+  hipLaunchKernelGGL(k_resnorm, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const int*)s->d_offsets,
+                     (const int*)s->d_cols, (const float*)s->b_vals, (const float*)x, b, s->P, s->d_norms);
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, st, (const double*)s->d_norms, (int)kind, s->d_dstats);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, const float* z, const float* dv,
+            const float* q_asm, float dt, float* q_bar, float* qd_bar, float* act_bar, float* mat_bar,
+            void* stream) {
+  if (!s || !q || !qd || !z || !q_bar || !qd_bar) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (mat_bar && (!dv || !q_asm)) return fail(FEM_ERR_ARG, "material gradients need dv and the assembly positions");
+  if (s->P == 0) return FEM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  RowArgs ra = s->row_args();
+  int rc = dispatch_g<LaunchVjpRows>(s->elem_g, ra, q, qd, act, z, (const float*)s->b_vals, dt, q_bar, qd_bar, st);
+  if (rc) return rc;
+  if (s->T > 0 && (act_bar || mat_bar)) {
+    hipLaunchKernelGGL(k_vjp_tets, dim3((s->T + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ra, s->T, q, qd, act, z, dv,
+                       q_asm, dt, act_bar, mat_bar);
+    HIP_TRY(hipGetLastError());
+  }
+  return FEM_OK;
+}
+
+int fem_set_tet_materials(fem_system* s, const float* mat, void* stream) {
+  if (!s || !mat) return fail(FEM_ERR_ARG, "null argument");
+  if (s->T == 0) return FEM_OK;
+  HIP_TRY(hipMemcpyAsync(s->d_tet_mat, mat, 3 * (size_t)s->T * sizeof(float), hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream));
+  return FEM_OK;
+}
+
+int fem_diff_stats(fem_system* s, void* stream, double* out, int32_t reset) {
+  if (!s || !out) return fail(FEM_ERR_ARG, "null argument");
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  HIP_TRY(hipMemcpy(out, s->d_dstats, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  if (reset) {
+    hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, (hipStream_t)stream, s->d_dstats, 4);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
   }
   return FEM_OK;
 }

@@ -78,6 +78,12 @@ SIGNATURES = {
     "fem_step_solve": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_eval_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 5),
     "fem_solve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _ip, _fp, _fp, _ip]),
+    "fem_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                                 ctypes.c_int32, ctypes.c_void_p]),
+    "fem_vjp": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6 + [ctypes.c_float] + [ctypes.c_void_p] * 5),
+    "fem_set_tet_materials": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fem_diff_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_int32]),
 }
 
 _lib = None

@@ -238,6 +238,11 @@ class SolverFEM:
         # Construction-time rest tangent (solver.py:324-341).
         _lib.check(self._lib.fem_assemble(self._sys, model.particle_q.data_ptr(), self.dt, _stream(dev)), "fem_assemble")
         self._anchor = model.particle_q.clone()
+        # Per-tet (mu, lambda, k_damp) used by the library; set requires_grad on
+        # it to receive material gradients from differentiable steps.
+        self.tet_materials = model.tet_materials.detach().clone()
+        self._q_asm = model.particle_q.detach().clone() if self.differentiable else None
+        self._diff_revision = None
         self._initial_volume = self._compute_volume(model.particle_q)
 
     # ------------------------------------------------------------------
@@ -399,6 +404,8 @@ class SolverFEM:
         _lib.check(self._lib.fem_assemble(self._sys, particle_q.data_ptr(), float(dt), st), "fem_assemble")
         self.dt = dt
         self._revision += 1
+        if self.differentiable:
+            self._q_asm = particle_q.detach().clone()
 
     def rebuild_element_blocks(self, particle_q, dt, *, update_preconditioner=True):
         """Reassemble the deformation-dependent tangent (reference solver.py:506-508)."""
@@ -529,6 +536,25 @@ class SolverFEM:
             return 1.0
         ratio = current / initial
         return ratio if math.isfinite(ratio) else 1.0
+
+    def set_tet_materials(self, materials: torch.Tensor) -> None:
+        """Replace the per-tet (mu, lambda, k_damp) and re-assemble at the last assembly positions.
+
+        No reference counterpart (there the materials are fixed model inputs);
+        for material optimisation with ``differentiable=True``.  Tets keep their
+        zero / non-zero status (zero-material tets are not in the pattern).
+        """
+        m = torch.as_tensor(materials, dtype=torch.float32, device=self.device).reshape(-1, 3)
+        if m.shape[0] != self.model.tet_count:
+            raise ValueError("materials must have shape (tet_count, 3)")
+        with torch.no_grad():
+            self.tet_materials.copy_(m)
+        st = _stream(self.device)
+        _lib.check(self._lib.fem_set_tet_materials(self._sys, self.tet_materials.data_ptr(), st),
+                   "fem_set_tet_materials")
+        q = self._q_asm if self._q_asm is not None else self.model.particle_q
+        _lib.check(self._lib.fem_assemble(self._sys, q.data_ptr(), float(self.dt), st), "fem_assemble")
+        self._revision += 1
 
     # Differentiation entry points (reference solver.py:445-469).
     def prepare_differentiation(self):

@@ -159,8 +159,8 @@ class OracleSolver:
         self._push_params()
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_destroy(self.h)
             self.h = None
 
     def refresh_tangent(self, q: np.ndarray, dt=None) -> np.ndarray:

@@ -1,0 +1,237 @@
+"""Differentiable substep on the HIP path vs the FP64 oracle (fixed-matrix linearisation).
+
+Forward: one differentiable step equals the oracle's (checked solve to 1e-8;
+positions 1e-6, velocities 1e-4 relative).  Backward: every input gradient
+(positions, velocities, activations, external forces, Tet4 materials) of a
+step equals the oracle's reverse product within 1e-3 relative (fp32 vs fp64).
+Rollouts: directional derivatives of a 12-step loss agree with FP64 central
+differences of the oracle rollout within 1e-3; checkpointed and full-tape
+gradients are identical; d v / d f_ext = h / m for a free particle
+(reference test_solver.py:319-331).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import cube
+from newton_solver_fem_amd import ModelBuilder, build_tet_model, meshes
+from newton_solver_fem_amd.differentiation import rollout
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+DT = 0.002
+GROUND = dict(ground_plane=(0.0, 0.0, 1.0, -0.302), ground_ke=40.0, ground_kd=0.2, ground_kf=1.0, ground_mu=0.5)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def np_(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def make(kind, **extra):
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    model = cube(kind, material=(500.0, 1000.0, 0.2), gravity=(0.0, 0.0, -9.81), device=DEV)
+    kw = dict(dt=DT, min_stretch=-1.0, linear_damping=0.3, enable_tet_activations=(kind == "tet"), **GROUND)
+    kw.update(extra)
+    s = SolverFEM(model, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0, differentiable=True,
+                  linear_solver_maxiter=128, **kw)
+    names = dict(dt="dt", min_stretch="min_stretch", linear_damping="linear_damping",
+                 enable_tet_activations="enable_tet_activations", ground_plane="ground_plane", ground_ke="ground_ke",
+                 ground_kd="ground_kd", ground_kf="ground_kf", ground_mu="ground_mu", max_stretch="max_stretch")
+    o = oracle.OracleSolver(model.host, bits=64, gravity=model.gravity,
+                            **{names[k]: v for k, v in kw.items() if k in names})
+    return model, s, o
+
+
+def initial(model, seed):
+    rng = np.random.default_rng(seed)
+    q0 = model.host["particle_q"].astype(np.float64).copy()
+    q0[:, 0] *= 1.03
+    v0 = np.tile((0.02, -0.01, 0.03), (len(q0), 1)) + rng.normal(0, 0.01, q0.shape)
+    return rng, q0, v0
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_step_and_its_gradients_match_oracle(kind):
+    print(f"case diff step {kind}", flush=True)
+    model, s, o = make(kind)
+    rng, q0, v0 = initial(model, 3)
+    ext = rng.normal(0, 0.02, q0.shape)
+    act = rng.uniform(0, 0.3, model.tet_count) if kind == "tet" else None
+    qo, vo, dv = o.diff_step(q0, v0, ext, act)
+    gq_out, gv_out = rng.normal(size=q0.shape), rng.normal(size=q0.shape)
+    ref = o.diff_step_backward(q0, v0, dv, gq_out, gv_out, act)
+
+    tq = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    tv = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    te = torch.tensor(ext, dtype=torch.float32, device=DEV, requires_grad=True)
+    ctrl = model.control()
+    if act is not None:
+        ctrl.tet_activations = torch.tensor(act, dtype=torch.float32, device=DEV, requires_grad=True)
+        s.tet_materials.requires_grad_(True)
+    st_in = model.state()
+    st_in.particle_q, st_in.particle_qd = tq, tv
+    st_out = model.state()
+    s.step(st_in, st_out, ctrl, None, DT, external_forces=te)
+    assert rel(np_(st_out.particle_q), qo) < 1e-6
+    assert rel(np_(st_out.particle_qd), vo) < 1e-4
+    loss = (st_out.particle_q * torch.tensor(gq_out, dtype=torch.float32, device=DEV)).sum() + (
+        st_out.particle_qd * torch.tensor(gv_out, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    res = s.check_differentiation()
+    assert res["forward_relative_residual"] < 2e-5 and res["backward_relative_residual"] < 2e-5
+    assert rel(np_(tq.grad), ref["q"]) < 1e-3
+    assert rel(np_(tv.grad), ref["qd"]) < 1e-3
+    # The reverse products alone (without the identity pass-through of the update).
+    assert rel(np_(tq.grad) - gq_out, ref["q"] - gq_out) < 2e-3
+    u = gv_out + DT * gq_out
+    assert rel(np_(tv.grad) - u, ref["qd"] - u) < 2e-3
+    assert rel(np_(te.grad), ref["ext"]) < 1e-3
+    if act is not None:
+        assert rel(np_(ctrl.tet_activations.grad), ref["act"]) < 1e-3
+        for col in range(3):
+            assert rel(np_(s.tet_materials.grad)[:, col], ref["mat"][:, col]) < 2e-3, col
+
+
+def _loss_np(q, v, w):
+    return float(np.sum(w * q) + 0.03 * np.sum(v * v))
+
+
+@pytest.mark.parametrize("checkpoint_every", [0, 4])
+def test_rollout_gradients_match_oracle_finite_differences(checkpoint_every):
+    """12 substeps, per-step activations and forces; FD of the FP64 oracle rollout (tol 1e-3)."""
+    print(f"case diff rollout k={checkpoint_every}", flush=True)
+    model, s, o = make("tet", defer_differentiation_checks=True)
+    rng, q0, v0 = initial(model, 5)
+    steps = 12
+    exts = rng.normal(0, 0.02, (steps, *q0.shape))
+    acts = rng.uniform(0, 0.3, (steps, model.tet_count))
+    n = len(q0)
+    w = np.stack([np.arange(1, n + 1) / n, np.full(n, 0.13), np.full(n, -0.41)], axis=1)
+
+    def oracle_loss(q, v, e, a):
+        for k in range(steps):
+            q, v, _ = o.diff_step(q, v, e[k], a[k])
+        return _loss_np(q, v, w)
+
+    tq = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    tv = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    te = torch.tensor(exts, dtype=torch.float32, device=DEV, requires_grad=True)
+    ta = torch.tensor(acts, dtype=torch.float32, device=DEV, requires_grad=True)
+    qT, vT = rollout(s, tq, tv, steps, activations=ta, external_forces=te, checkpoint_every=checkpoint_every)
+    tw = torch.tensor(w, dtype=torch.float32, device=DEV)
+    loss = (tw * qT).sum() + 0.03 * (vT * vT).sum()
+    loss.backward()
+    s.check_differentiation()
+    assert abs(loss.item() - oracle_loss(q0, v0, exts, acts)) < 1e-5 * abs(loss.item()) + 1e-7
+    # fp64 FD with a small step: ground contact and friction have kinks that a
+    # 1e-4 m perturbation can cross within 12 substeps.
+    eps = 1e-6
+    for name, g, base in (("q0", tq.grad, q0), ("v0", tv.grad, v0), ("ext", te.grad, exts), ("act", ta.grad, acts)):
+        d = rng.normal(size=base.shape)
+        args = {"q0": [q0, v0, exts, acts], "v0": [q0, v0, exts, acts], "ext": [q0, v0, exts, acts],
+                "act": [q0, v0, exts, acts]}[name]
+        idx = ("q0", "v0", "ext", "act").index(name)
+        plus, minus = list(args), list(args)
+        plus[idx] = base + eps * d
+        minus[idx] = base - eps * d
+        fd = (oracle_loss(*plus) - oracle_loss(*minus)) / (2 * eps)
+        exact = float(np.sum(np_(g) * d))
+        assert abs(exact - fd) <= 1e-3 * abs(fd) + 1e-6, (name, exact, fd)
+
+
+def test_checkpointed_equals_full_tape():
+    """Reference test_bear_training.py:76-138: checkpointed recomputation == full tape."""
+    print("case checkpoint == full tape", flush=True)
+    model, s, _ = make("tet", defer_differentiation_checks=True)
+    rng, q0, v0 = initial(model, 9)
+    acts = torch.tensor(rng.uniform(0, 0.3, model.tet_count), dtype=torch.float32, device=DEV)
+    grads = []
+    for k in (0, 3, 5):
+        a = acts.clone().requires_grad_(True)
+        tq = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+        tv = torch.tensor(v0, dtype=torch.float32, device=DEV)
+        qT, vT = rollout(s, tq, tv, 15, activations=a, checkpoint_every=k)
+        (qT[:, 2].sum() + (vT * vT).sum()).backward()
+        grads.append((np_(a.grad), np_(tq.grad)))
+    s.check_differentiation()
+    for g in grads[1:]:
+        np.testing.assert_array_equal(g[0], grads[0][0])
+        np.testing.assert_array_equal(g[1], grads[0][1])
+
+
+def test_velocity_derivative_of_external_force_is_h_over_m():
+    """Reference test_solver.py:319-331: one free particle of mass 2, d v_x / d f_x = h / m."""
+    print("case free particle", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    b = ModelBuilder(gravity=(0.0, 0.0, 0.0))
+    b.add_particle((0, 0, 1), (0, 0, 0), mass=2.0)
+    model = b.finalize(DEV)
+    s = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0,
+                  differentiable=True, linear_solver_maxiter=128)
+    st_in, st_out = model.state(), model.state()
+    ext = torch.zeros(1, 3, device=DEV, requires_grad=True)
+    s.step(st_in, st_out, None, None, DT, external_forces=ext)
+    st_out.particle_qd[0, 0].backward()
+    np.testing.assert_allclose(np_(ext.grad), [[DT / 2.0, 0.0, 0.0]], atol=1e-8)
+
+
+def test_bar_with_pinned_base_gradient_matches_oracle():
+    """The differentiable-bar workload shape (zero-mass base + support projection), 20 substeps."""
+    print("case pinned bar", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    sc = meshes.tet_bar_scene(nx=12, ny=3, nz=3, fix_base=True)
+    base = np.zeros(len(sc.points), bool)
+    base[sc.fixed] = True
+    sc.masses = np.where(base, 0.0, sc.masses)
+    sc.fixed = np.zeros(0, np.int32)
+    model = build_tet_model(sc, device=DEV)
+    s = SolverFEM(model, dt=1 / 600, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0,
+                  differentiable=True, enable_tet_activations=True, defer_differentiation_checks=True,
+                  linear_solver_maxiter=256)
+    o = oracle.OracleSolver(model.host, bits=64, dt=1 / 600, gravity=model.gravity, enable_tet_activations=True)
+    rest = model.host["particle_q"].astype(np.float64)
+    mask = torch.tensor(base, device=DEV)[:, None]
+    rest_t = torch.tensor(rest, dtype=torch.float32, device=DEV)
+
+    def project(q, v):
+        return torch.where(mask, rest_t, q), torch.where(mask, torch.zeros_like(v), v)
+
+    act = torch.tensor(sc.activations, dtype=torch.float32, device=DEV, requires_grad=True)
+    q0 = model.particle_q.clone()
+    v0 = torch.zeros_like(q0)
+    steps = 20
+    qT, _ = rollout(s, q0, v0, steps, activations=act, checkpoint_every=5, after_step=project)
+    tip = torch.tensor(rest[:, 0] > rest[:, 0].max() - 1e-9, device=DEV)
+    target = rest_t[tip].mean(0) + torch.tensor([0.004, -0.002, 0.003], device=DEV)
+    loss = ((qT[tip].mean(0) - target) ** 2).sum()
+    loss.backward()
+    s.check_differentiation()
+
+    tipn = rest[:, 0] > rest[:, 0].max() - 1e-9
+    tgt = rest[tipn].mean(0) + np.array([0.004, -0.002, 0.003])
+
+    def oracle_loss(a):
+        q, v = rest.copy(), np.zeros_like(rest)
+        for _ in range(steps):
+            q, v, _ = o.diff_step(q, v, None, a)
+            q[base], v[base] = rest[base], 0.0
+        return float(np.sum((q[tipn].mean(0) - tgt) ** 2))
+
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=model.tet_count)
+    eps = 1e-4
+    a0 = sc.activations.astype(np.float64)
+    fd = (oracle_loss(a0 + eps * d) - oracle_loss(a0 - eps * d)) / (2 * eps)
+    exact = float(np.sum(np_(act.grad) * d))
+    assert abs(exact - fd) <= 1e-3 * abs(fd) + 1e-9, (exact, fd)
