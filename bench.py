@@ -178,12 +178,14 @@ def bench_batched(dev, replays: int):
 def bench_adjoint(dev, repeats: int = 3):
     """Differentiable bar: gradient of a tip-target loss w.r.t. per-tet activations.
 
-    40x8x8-cell Tet4 bar (12,800 tets), activations U(0, 0.3) (seed 2), base
-    face pinned the way the reference pins differentiable supports (zero mass
-    + projection after every substep), 60 substeps with the fixed matrix,
-    checked solves (tol 1e-8).  forward = the rollout without a tape;
-    gradient = rollout with checkpoints every 10 substeps + backward
-    (recomputation + adjoint solves + reverse products).
+    40x8x8-cell Tet4 bar (12,800 tets, E 1e5, nu 0.45), x = 0 face fixed
+    (projected forward and adjoint solves), activations a ~ U(0, 0.3)
+    (seed 2) scaling mu (mu_e = mu0_e (1 + a_e)), gravity, 60 substeps at
+    h = 1/600 with the matrix assembled once, checked solves (tol 1e-8).
+    Loss: squared distance of the tip-face centroid to rest + U(-0.01, 0.01)^3
+    (seed 3).  forward = the rollout without a tape; gradient = rollout with
+    a checkpoint every 10 substeps + backward (recomputation, adjoint solves,
+    reverse products, material matrix terms).
     """
     import numpy as np
     import torch
@@ -193,35 +195,31 @@ def bench_adjoint(dev, repeats: int = 3):
     from newton_solver_fem_amd.solver import SolverFEM
 
     sc = meshes.tet_bar_scene()
-    base = np.zeros(len(sc.points), bool)
-    base[sc.fixed] = True
-    sc.masses = np.where(base, 0.0, sc.masses)
-    sc.fixed = np.zeros(0, np.int32)
     model = build_tet_model(sc, device=dev)
     solver = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0,
-                       differentiable=True, enable_tet_activations=True, defer_differentiation_checks=True,
-                       linear_solver_maxiter=256)
+                       differentiable=True, defer_differentiation_checks=True, linear_solver_maxiter=256,
+                       fixed_particle_indices=sc.fixed, allow_fixed_in_differentiation=True)
     rest = model.particle_q.clone()
-    mask = torch.tensor(base, device=dev)[:, None]
-
-    def project(q, v):
-        return torch.where(mask, rest, q), torch.where(mask, torch.zeros_like(v), v)
-
+    mat0 = torch.tensor(sc.materials, dtype=torch.float32, device=dev)
     tip = rest[:, 0] > rest[:, 0].max() - 1e-6
     target = rest[tip].mean(0) + torch.tensor(np.random.default_rng(3).uniform(-0.01, 0.01, 3),
                                               dtype=torch.float32, device=dev)
     act0 = torch.tensor(sc.activations, dtype=torch.float32, device=dev)
     steps, k = 60, 10
 
+    def materials(a):
+        return mat0 * torch.stack([1.0 + a, torch.ones_like(a), torch.ones_like(a)], dim=1)
+
     def forward():
         with torch.no_grad():
-            q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps, activations=act0, after_step=project)
+            solver.set_tet_materials(materials(act0))
+            q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps)
             return ((q[tip].mean(0) - target) ** 2).sum()
 
     def gradient():
         a = act0.clone().requires_grad_(True)
-        q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps, activations=a, checkpoint_every=k,
-                       after_step=project)
+        solver.set_tet_materials(materials(a))
+        q, _ = rollout(solver, rest.clone(), torch.zeros_like(rest), steps, checkpoint_every=k)
         loss = ((q[tip].mean(0) - target) ** 2).sum()
         loss.backward()
         return a.grad
@@ -244,8 +242,8 @@ def bench_adjoint(dev, repeats: int = 3):
         raise RuntimeError("non-finite activation gradient")
     fwd, grad = min(tf), min(tg)
     return {
-        "workload": "differentiable Tet4 bar 40x8x8 cells (12,800 tets), d(tip-target loss)/d(activations), "
-                    "60 substeps, checkpoint every 10, checked solves tol 1e-8",
+        "workload": "differentiable Tet4 bar 40x8x8 cells (12,800 tets), x=0 face fixed, activations scaling mu, "
+                    "d(tip-target loss)/d(activations), 60 substeps, checkpoint every 10, checked solves tol 1e-8",
         "forward_ms": fwd * 1e3,
         "gradient_ms": grad * 1e3,
         "adjoint_over_forward": grad / fwd,

@@ -177,7 +177,10 @@ int fem_solve_stats(fem_system* sys, void* stream, int32_t* iterations, float* r
  * residual exit; then the FP64 residual ||A x - b|| / ||b|| and its ratio to
  * the budget 2e-5 ||b|| + 1e-10 update running maxima for `kind`
  * (0 = forward, 1 = adjoint; residual_norms / record_residual :118-146).
- * A is symmetric, so the adjoint solve (bsr_transposed) uses the same call. */
+ * A is symmetric, so the adjoint solve (bsr_transposed) uses the same call.
+ * With fixed particles the solve (and its residual) uses PAP + I - P, as the
+ * fast path does; the reference rejects fixed particles when differentiable,
+ * so that case is an extension (b must be zero on fixed rows). */
 int fem_solve(fem_system* sys, const float* b_dev, float* x_dev, float tolerance, int32_t maxiter, int32_t kind,
               void* stream);
 

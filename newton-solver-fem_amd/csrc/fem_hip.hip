@@ -655,15 +655,23 @@ __global__ void k_zero_doubles(double* p, int n) {
   if (t < n) p[t] = 0.0;
 }
 
-// FP64 ||A x - b||^2 and ||b||^2 (residual_norms differentiation.py:118-133).
+// FP64 ||A x - b||^2 and ||b||^2 (residual_norms differentiation.py:118-133),
+// with the solve's operator: PAP + I - P when there are fixed rows.
 __global__ __launch_bounds__(BLOCK) void k_resnorm(const int* __restrict__ offsets, const int* __restrict__ cols,
-                                                   const float* __restrict__ vals, const float* __restrict__ x,
-                                                   const float* __restrict__ b, int P, double* norms) {
+                                                   const float* __restrict__ vals,
+                                                   const unsigned char* __restrict__ row_fixed,
+                                                   const float* __restrict__ x, const float* __restrict__ b, int P,
+                                                   double* norms) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   double rr = 0.0, bb = 0.0;
   if (i < P) {
     v3 ax = {0.f, 0.f, 0.f};
-    for (int k = offsets[i]; k < offsets[i + 1]; ++k) ax = ax + mv(vals + 9 * (size_t)k, ld3(x, cols[k]));
+    if (row_fixed[i]) {
+      ax = ld3(x, i);
+    } else {
+      for (int k = offsets[i]; k < offsets[i + 1]; ++k)
+        if (!row_fixed[cols[k]]) ax = ax + mv(vals + 9 * (size_t)k, ld3(x, cols[k]));
+    }
     const v3 bi = ld3(b, i);
     const double r0 = (double)ax.x - bi.x, r1 = (double)ax.y - bi.y, r2 = (double)ax.z - bi.z;
     rr = r0 * r0 + r1 * r1 + r2 * r2;
@@ -1957,7 +1965,8 @@ int fem_solve(fem_system* s, const float* b, float* x, float tolerance, int32_t 
   if (rc) return rc;
   hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, st, s->d_norms, 2);
   hipLaunchKernelGGL(k_resnorm, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const int*)s->d_offsets,
-                     (const int*)s->d_cols, (const float*)s->b_vals, (const float*)x, b, s->P, s->d_norms);
+                     (const int*)s->d_cols, (const float*)s->b_vals, (const unsigned char*)s->d_row_fixed,
+                     (const float*)x, b, s->P, s->d_norms);
   hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, st, (const double*)s->d_norms, (int)kind, s->d_dstats);
   HIP_TRY(hipGetLastError());
   return FEM_OK;

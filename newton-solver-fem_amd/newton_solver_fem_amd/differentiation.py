@@ -43,15 +43,25 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def _fixed_mask(solver):
+    m = getattr(solver, "_fixed_mask_t", None)
+    if m is None:
+        m = torch.zeros(solver.model.particle_count, 1, dtype=torch.bool, device=solver.device)
+        if solver.fixed_particle_indices.size:
+            m[torch.as_tensor(solver.fixed_particle_indices, dtype=torch.long, device=solver.device)] = True
+        solver._fixed_mask_t = m
+    return m
+
+
 def prepare(solver):
     """Validate the model for differentiation and pin the current matrix revision.
 
     Reference ``prepare`` (differentiation.py:244-266): no streaming topology,
     no particle or surface contacts (only the analytic ground), fixed model
     arrays.  The fixed matrix of a recorded step is the current assembly.
+    Fixed particles (allowed only with allow_fixed_in_differentiation) use
+    the projected operator in both solves.
     """
-    if solver.fixed_particle_indices.size:
-        raise ValueError("fixed_particle_indices is not supported with differentiable=True")
     if torch.cuda.is_current_stream_capturing():
         raise RuntimeError("Call prepare_differentiation() before CUDA graph capture")
     if getattr(solver, "_q_asm", None) is None:
@@ -115,7 +125,10 @@ class _VelocityIncrement(torch.autograd.Function):
         q, v, act, dv = ctx.saved_tensors
         has_act, has_ext, has_mat = ctx.has
         act = act if has_act else None
-        z = _solve(solver, gdv.detach().contiguous(), 1)
+        g = gdv.detach()
+        if solver.fixed_particle_indices.size:  # dv is zero on fixed rows: project the adjoint
+            g = torch.where(_fixed_mask(solver), torch.zeros_like(g), g)
+        z = _solve(solver, g.contiguous(), 1)
         qb, vb = torch.empty_like(q), torch.empty_like(v)
         want_act = has_act and ctx.needs_input_grad[4]
         want_mat = has_mat and ctx.needs_input_grad[6]
@@ -148,6 +161,11 @@ def differentiable_step(solver, state_in: State, state_out: State, control, cont
         raise NotImplementedError("Differentiation supports only the analytic ground contact law")
     prepare(solver)
     q, v = state_in.particle_q, state_in.particle_qd
+    fixed = solver.fixed_particle_indices.size > 0
+    if fixed:  # FixedParticles.restore(state_in), differentiably
+        mask = _fixed_mask(solver)
+        q = torch.where(mask, solver._anchor, q)
+        v = torch.where(mask, torch.zeros_like(v), v)
     act = None
     if solver.enable_tet_activations:
         act = (control if control is not None else solver.model.control()).tet_activations
@@ -158,8 +176,12 @@ def differentiable_step(solver, state_in: State, state_out: State, control, cont
             raise ValueError("Use unique input/output states for every taped step")
         dv = _VelocityIncrement.apply(solver, dt, q, v, act, external_forces, mat)
         v_out = v + dv
+        q_out = q + dt * v_out
+        if fixed:
+            q_out = torch.where(mask, solver._anchor, q_out)
+            v_out = torch.where(mask, torch.zeros_like(v_out), v_out)
         state_out.particle_qd = v_out
-        state_out.particle_q = q + dt * v_out
+        state_out.particle_q = q_out
     else:
         with torch.no_grad():
             dv = _forward_dv(solver, dt, q.contiguous(), v.contiguous(),
@@ -167,6 +189,9 @@ def differentiable_step(solver, state_in: State, state_out: State, control, cont
                              None if external_forces is None else external_forces.contiguous())
             torch.add(v, dv, out=state_out.particle_qd)
             torch.add(q, state_out.particle_qd, alpha=dt, out=state_out.particle_q)
+            if fixed:
+                state_out.particle_q.copy_(torch.where(mask, solver._anchor, state_out.particle_q))
+                state_out.particle_qd.masked_fill_(mask, 0.0)
     with torch.no_grad():
         solver.dv.copy_(dv)
     if not solver.defer_differentiation_checks:

@@ -112,6 +112,7 @@ class SolverFEM:
         linear_solver_tolerance: float = 1.0e-2,
         linear_solver_use_cuda_graph: bool = True,
         instance_offsets=None,
+        allow_fixed_in_differentiation: bool = False,
     ):
         # Argument validation, reference solver.py:241-261.
         if not math.isfinite(linear_solver_tolerance) or linear_solver_tolerance <= 0:
@@ -122,7 +123,9 @@ class SolverFEM:
         if fixed.size and (fixed.min() < 0 or fixed.max() >= model.particle_count):
             raise ValueError("fixed_particle_indices contains an out-of-range node")
         fixed = np.unique(fixed).astype(np.int32)
-        if fixed.size and differentiable:
+        if fixed.size and differentiable and not allow_fixed_in_differentiation:
+            # Reference behaviour (solver.py:250-251).  allow_fixed_in_differentiation
+            # enables the projected forward and adjoint solves instead.
             raise ValueError("fixed_particle_indices is not supported with differentiable=True")
         if not (isinstance(dt, (int, float)) and math.isfinite(dt) and dt > 0):
             raise ValueError("dt must be finite and positive")
@@ -544,13 +547,15 @@ class SolverFEM:
         for material optimisation with ``differentiable=True``.  Tets keep their
         zero / non-zero status (zero-material tets are not in the pattern).
         """
-        m = torch.as_tensor(materials, dtype=torch.float32, device=self.device).reshape(-1, 3)
-        if m.shape[0] != self.model.tet_count:
-            raise ValueError("materials must have shape (tet_count, 3)")
-        with torch.no_grad():
-            self.tet_materials.copy_(m)
+        m = torch.as_tensor(materials).reshape(-1, 3)
+        if m.shape[0] != self.model.tet_count or m.dtype != torch.float32 or m.device != self.device:
+            raise ValueError("materials must be a float32 (tet_count, 3) tensor on the solver's device")
+        # Kept by reference: when it requires a gradient (e.g. it is a function
+        # of optimisation parameters), differentiable steps propagate into it.
+        self.tet_materials = m
+        self._tet_materials_values = m.detach().contiguous()
         st = _stream(self.device)
-        _lib.check(self._lib.fem_set_tet_materials(self._sys, self.tet_materials.data_ptr(), st),
+        _lib.check(self._lib.fem_set_tet_materials(self._sys, self._tet_materials_values.data_ptr(), st),
                    "fem_set_tet_materials")
         q = self._q_asm if self._q_asm is not None else self.model.particle_q
         _lib.check(self._lib.fem_assemble(self._sys, q.data_ptr(), float(self.dt), st), "fem_assemble")

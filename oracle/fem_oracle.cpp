@@ -1173,8 +1173,18 @@ struct Solver {
 
   // Differentiable step forward (differentiation.py step): rhs = h sum terms
   // + h ext - sum_j A_ij (v_j - v_i), A dv = rhs solved from zero to `tol`.
-  void diff_forward(const v3* q, const v3* qd, const v3* ext, const Real* act, Real dt, double tol, v3* q_out,
-                    v3* qd_out, v3* dv_out) {
+  // With fixed particles (an extension: the reference rejects them when
+  // differentiable) the step restores their inputs, projects the right-hand
+  // side and solves with the projected operator PAP + I - P, as the fast path.
+  void diff_forward(const v3* q_in, const v3* qd_in, const v3* ext, const Real* act, Real dt, double tol,
+                    v3* q_out, v3* qd_out, v3* dv_out) {
+    std::vector<v3> qr(q_in, q_in + P), vr(qd_in, qd_in + P);
+    for (int i : fixed_idx) {
+      qr[i] = anchor[i];
+      vr[i] = v3{0, 0, 0};
+    }
+    const v3* q = qr.data();
+    const v3* qd = vr.data();
     std::vector<v3> f(P);
     forces_scaled(q, qd, ext, act, dt, f.data());
     if (T || H) {
@@ -1182,6 +1192,7 @@ struct Solver {
       predictor(qd, c.data());
       for (int i = 0; i < P; ++i) f[i] = f[i] - c[i];
     }
+    for (int i : fixed_idx) f[i] = v3{0, 0, 0};
     std::vector<v3> x(P, v3{0, 0, 0});
     const double keep_tol = prm.tol;
     const int keep_max = prm.maxiter;
@@ -1194,6 +1205,10 @@ struct Solver {
       dv_out[i] = x[i];
       qd_out[i] = qd[i] + x[i];
       q_out[i] = q[i] + qd_out[i] * dt;
+    }
+    for (int i : fixed_idx) {
+      q_out[i] = anchor[i];
+      qd_out[i] = v3{0, 0, 0};
     }
   }
   void predictor(const v3* qd, v3* c) const {
@@ -1209,10 +1224,18 @@ struct Solver {
   }
 
   // Reverse of diff_forward for output adjoints (qb_out, vb_out).
-  void diff_backward(const v3* q, const v3* qd, const Real* act, const v3* dv, const v3* qb_out, const v3* vb_out,
-                     Real dt, double tol, v3* qb_in, v3* vb_in, Real* actb, v3* extb, Real* matb) {
+  void diff_backward(const v3* q_in, const v3* qd_in, const Real* act, const v3* dv, const v3* qb_out,
+                     const v3* vb_out, Real dt, double tol, v3* qb_in, v3* vb_in, Real* actb, v3* extb, Real* matb) {
+    std::vector<v3> qr(q_in, q_in + P), vr(qd_in, qd_in + P);
+    for (int i : fixed_idx) {
+      qr[i] = anchor[i];
+      vr[i] = v3{0, 0, 0};
+    }
+    const v3* q = qr.data();
+    const v3* qd = vr.data();
     std::vector<v3> u(P), z(P, v3{0, 0, 0}), zw(P);
     for (int i = 0; i < P; ++i) u[i] = vb_out[i] + qb_out[i] * dt;
+    for (int i : fixed_idx) u[i] = v3{0, 0, 0};  // fixed outputs are constants
     const double keep_tol = prm.tol;
     const int keep_max = prm.maxiter;
     prm.tol = tol;
@@ -1245,6 +1268,10 @@ struct Solver {
     std::vector<v3> y(P);
     for (int i = 0; i < P; ++i) y[i] = dv[i] + qd[i];
     matrix_material_terms(z.data(), y.data(), dt, matb);
+    for (int i : fixed_idx) {  // fixed inputs are overwritten by their anchors
+      qb_in[i] = v3{0, 0, 0};
+      vb_in[i] = v3{0, 0, 0};
+    }
   }
 
   void eval_terms(int mask, const v3* q, const v3* qd, const Real* act, v3* out) {
