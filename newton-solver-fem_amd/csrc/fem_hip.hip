@@ -864,7 +864,7 @@ __device__ __forceinline__ double ld_shared(const double* p) { return __builtin_
 
 // Sum of N floats over the workgroup (fp32 within waves, fp64 across waves,
 // fixed order); every thread receives the result.
-template <int N>
+template <int NT, int N>
 __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* red) {
 #pragma unroll
   for (int n = 0; n < N; ++n)
@@ -876,19 +876,24 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
     for (int n = 0; n < N; ++n) red[w * N + n] = (double)v[n];
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < N; ++n) out[n] = ((red[n] + red[N + n]) + red[2 * N + n]) + red[3 * N + n];
+  for (int n = 0; n < N; ++n) {
+    double t = red[n];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) t += red[w * N + n];
+    out[n] = t;
+  }
   __syncthreads();
 }
 
 // Sum of part[c*NPART + slot[n]] over the instance's chunks, identical in every
 // workgroup (same data, same reduction tree).
-template <int N>
+template <int NT, int N>
 __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, const int (&slot)[N], double (&out)[N],
                                          double* red) {
   double acc[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) acc[n] = 0.0;
-  for (int c = cb + (int)threadIdx.x; c < ce; c += BLOCK)
+  for (int c = cb + (int)threadIdx.x; c < ce; c += NT)
 #pragma unroll
     for (int n = 0; n < N; ++n) acc[n] += ld_shared(part + NPART * (size_t)c + slot[n]);
 #pragma unroll
@@ -901,18 +906,22 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
     for (int n = 0; n < N; ++n) red[w * N + n] = acc[n];
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < N; ++n) out[n] = ((red[n] + red[N + n]) + red[2 * N + n]) + red[3 * N + n];
+  for (int n = 0; n < N; ++n) {
+    double t = red[n];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) t += red[w * N + n];
+    out[n] = t;
+  }
   __syncthreads();
 }
 
-constexpr int CG_J = 8;                   // blocks in flight per thread
-constexpr int CG_TILE = CG_J * BLOCK;     // blocks per row-aligned tile
+constexpr int CG_J = 8;  // blocks per thread per tile (a tile is CG_J * NT blocks)
 // Exchange record of one row: z(3) w(3) p(3) + pad, 48 bytes (three 16-byte
 // aligned loads per gathered column).
 constexpr int XW = 12;
 
-// q[k] = sum_j A_ij u_j for the thread's own rows i = R0 + t + 256 k.
-// Row-aligned tiles of <= CG_TILE blocks.  Per tile, the operand u of every
+// q[k] = sum_j A_ij u_j for the thread's own rows i = R0 + t + NT k.
+// Row-aligned tiles of <= (CG_J * NT) blocks.  Per tile, the operand u of every
 // distinct column the tile references is formed once into LDS (u_l, indexed
 // by the tile-local column number lidx), then CG_J coalesced blocks per
 // thread are multiplied against it; block products go to LDS and each row is
@@ -920,11 +929,12 @@ constexpr int XW = 12;
 // Otherwise u_j = p_j' = (z_j - alpha w_j) + beta p_j from the previous
 // phase's exchange records.  Fixed rows give q = 0.
 // Distinct columns of `tile` held by this thread (-1 past the end).
+template <int NT>
 __device__ __forceinline__ void load_tile_cols(const CgArgs& a, int tile, int (&ucol)[CG_J]) {
   const int c0 = a.tile_cptr[tile], nu = a.tile_cptr[tile + 1] - c0;
 #pragma unroll
   for (int j = 0; j < CG_J; ++j) {
-    const int u = threadIdx.x + BLOCK * j;
+    const int u = threadIdx.x + NT * j;
     ucol[j] = u < nu ? a.tile_cols[c0 + u] : -1;
   }
 }
@@ -932,7 +942,7 @@ __device__ __forceinline__ void load_tile_cols(const CgArgs& a, int tile, int (&
 // ucol holds the first tile's distinct columns on entry and again on return:
 // each tile prefetches the next one's list (the first tile's after the last)
 // so that the operand loads of a tile never wait for its column list.
-template <int K, bool PROLOGUE>
+template <int K, bool PROLOGUE, int NT>
 __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0, int t1, const int* off_l,
                                         float s, float alpha, float beta, const float* __restrict__ xw,
                                         float* part_l, float* u_l, int (&ucol)[CG_J], const bool (&fx)[K],
@@ -949,7 +959,7 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
     float bv[CG_J][9];
 #pragma unroll
     for (int j = 0; j < CG_J; ++j) {
-      const int k = kb + threadIdx.x + BLOCK * j;
+      const int k = kb + threadIdx.x + NT * j;
       li[j] = k < ke ? (int)a.lidx[k] : -1;
       if (k < ke) {
         // Three 12-byte loads per block: a wave's load then touches each
@@ -979,20 +989,20 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
           u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
                fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
         }
-        float* o = u_l + 3 * (threadIdx.x + BLOCK * j);
+        float* o = u_l + 3 * (threadIdx.x + NT * j);
         o[0] = u.x;
         o[1] = u.y;
         o[2] = u.z;
       }
     }
-    load_tile_cols(a, tile + 1 < t1 ? tile + 1 : t0, ucol);
+    load_tile_cols<NT>(a, tile + 1 < t1 ? tile + 1 : t0, ucol);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < CG_J; ++j) {
       if (li[j] >= 0) {
         const float* uu = u_l + 3 * li[j];
         const float ux = uu[0], uy = uu[1], uz = uu[2];
-        float* o = part_l + 3 * (threadIdx.x + BLOCK * j);
+        float* o = part_l + 3 * (threadIdx.x + NT * j);
         o[0] = bv[j][0] * ux + bv[j][1] * uy + bv[j][2] * uz;
         o[1] = bv[j][3] * ux + bv[j][4] * uy + bv[j][5] * uz;
         o[2] = bv[j][6] * ux + bv[j][7] * uy + bv[j][8] * uz;
@@ -1001,7 +1011,7 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int lr = threadIdx.x + BLOCK * k;
+      const int lr = threadIdx.x + NT * k;
       if (lr >= ta && lr < tb && !fx[k]) {
         float s0 = 0.f, s1 = 0.f, s2 = 0.f;
         const int e = off_l[lr + 1] - kb;
@@ -1048,12 +1058,12 @@ __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-
 // k+1, so recurrence error does not accumulate.  Exchange records and
 // partial sums are double buffered: phase k writes buffer (k+1)&1, which no
 // workgroup reads after the barrier that ended phase k-1.
-template <int K>
-__global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
-  __shared__ float part_l[3 * CG_TILE];
-  __shared__ float u_l[3 * CG_TILE];
-  __shared__ int off_l[K * BLOCK + 1];
-  __shared__ double red[4 * NPART];
+template <int K, int NT>
+__global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
+  __shared__ float part_l[3 * (CG_J * NT)];
+  __shared__ float u_l[3 * (CG_J * NT)];
+  __shared__ int off_l[K * NT + 1];
+  __shared__ double red[(NT / 64) * NPART];
   __shared__ int bflag;
   __shared__ int xinfo[3];
   const int c = a.chunk_base + xcd_swizzle(blockIdx.x, gridDim.x);
@@ -1068,7 +1078,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
   const bool zero_start = (m == 0.f) || a.x_out != nullptr;
 
-  for (int r = threadIdx.x; r <= nr; r += BLOCK) off_l[r] = a.offsets[R0 + r];
+  for (int r = threadIdx.x; r <= nr; r += NT) off_l[r] = a.offsets[R0 + r];
 
   // Register on this workgroup's XCC and learn the instance's XCC layout.
   XcdGroup xg{0, 1, 1};
@@ -1101,7 +1111,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   bool own[K], fx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int lr = threadIdx.x + BLOCK * k;
+    const int lr = threadIdx.x + NT * k;
     own[k] = lr < nr;
     fx[k] = own[k] ? (a.row_fixed[R0 + lr] != 0) : true;
     if (MRES) {
@@ -1117,12 +1127,12 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     return mv(m, v);
   };
   int ucol[CG_J];
-  load_tile_cols(a, t0, ucol);
+  load_tile_cols<NT>(a, t0, ucol);
 
   // Prologue: r0 = b - A x0, z0 = M^-1 r0; records (z0, 0, 0) so that the
   // first phase gathers p0 = z0.
   if (!zero_start) {
-    cg_spmv<K, true>(a, R0, nr, t0, t1, off_l, s, 0.f, 0.f, a.xw, part_l, u_l, ucol, fx, qv);
+    cg_spmv<K, true, NT>(a, R0, nr, t0, t1, off_l, s, 0.f, 0.f, a.xw, part_l, u_l, ucol, fx, qv);
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k][0] = qv[k][1] = qv[k][2] = 0.f;
@@ -1133,7 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = w[k][d] = 0.f;
     if (own[k]) {
-      const int i = R0 + threadIdx.x + BLOCK * k;
+      const int i = R0 + threadIdx.x + NT * k;
       const v3 b = ld3(a.rhs, i) * s;
       const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : ld3(a.dv, i) * s;
       const v3 rr0 = b - v3{qv[k][0], qv[k][1], qv[k][2]};
@@ -1151,7 +1161,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   }
   {
     double wsum[3];
-    wg_sum<3>(pb, wsum, red);
+    wg_sum<NT, 3>(pb, wsum, red);
     if (threadIdx.x == 0) {
       a.part[NPART * (size_t)c + PT_BB] = wsum[0];
       a.part[NPART * (size_t)c + PT_RR] = wsum[1];
@@ -1163,7 +1173,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
   double tot[3];
   {
     const int sl[3] = {PT_BB, PT_RR, PT_RZ};
-    inst_sum<3>(a.part, cb, ce, sl, tot, red);
+    inst_sum<NT, 3>(a.part, cb, ce, sl, tot, red);
   }
   double rr = tot[1], rz = tot[2];
   double atol = fmax(0.0, (double)a.tolerance * sqrt(tot[0]));
@@ -1177,7 +1187,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
       double g[7];
       {
         const int sl[7] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR};
-        inst_sum<7>(a.part + NPART * nch * cur, cb, ce, sl, g, red);
+        inst_sum<NT, 7>(a.part + NPART * nch * cur, cb, ce, sl, g, red);
       }
       rz = g[5];
       rr = g[6];
@@ -1204,14 +1214,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
       }
     }
     if (ph == a.maxiter) break;
-    cg_spmv<K, false>(a, R0, nr, t0, t1, off_l, s, alpha, beta, a.xw + XW * (size_t)a.P * cur, part_l, u_l,
+    cg_spmv<K, false, NT>(a, R0, nr, t0, t1, off_l, s, alpha, beta, a.xw + XW * (size_t)a.P * cur, part_l, u_l,
                       ucol, fx, qv);
     float pp[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float* xw_next = a.xw + XW * (size_t)a.P * (cur ^ 1);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (own[k]) {
-        const int i = R0 + threadIdx.x + BLOCK * k;
+        const int i = R0 + threadIdx.x + NT * k;
         const v3 qq = {qv[k][0], qv[k][1], qv[k][2]};
         const v3 ww = minv_mul(k, i, qq);
         w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
@@ -1229,7 +1239,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
     }
     {
       double ws[7];
-      wg_sum<7>(pp, ws, red);
+      wg_sum<NT, 7>(pp, ws, red);
       if (threadIdx.x == 0) {
         double* dst = a.part + NPART * (nch * (cur ^ 1) + c);
 #pragma unroll
@@ -1243,9 +1253,9 @@ __global__ __launch_bounds__(BLOCK) void k_cg(CgArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (own[k] && a.x_out) {
-      st3(a.x_out, R0 + threadIdx.x + BLOCK * k, v3{x[k][0], x[k][1], x[k][2]} * (1.f / s));
+      st3(a.x_out, R0 + threadIdx.x + NT * k, v3{x[k][0], x[k][1], x[k][2]} * (1.f / s));
     } else if (own[k]) {
-      const int i = R0 + threadIdx.x + BLOCK * k;
+      const int i = R0 + threadIdx.x + NT * k;
       const v3 dvi = v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
       st3(a.dv, i, dvi);
       v3 vel = ld3(a.qd_in, i) + dvi;
@@ -1336,12 +1346,20 @@ struct LaunchVjpRows {
   }
 };
 
-const void* cg_kernel_ptr(int k) {
+// Rows per thread K and threads per workgroup NT of the persistent CG.
+const void* cg_kernel_ptr(int k, int nt) {
+  if (nt == 512) {
+    switch (k) {
+      case 1: return (const void*)k_cg<1, 512>;
+      case 2: return (const void*)k_cg<2, 512>;
+    }
+    return nullptr;
+  }
   switch (k) {
-    case 1: return (const void*)k_cg<1>;
-    case 2: return (const void*)k_cg<2>;
-    case 4: return (const void*)k_cg<4>;
-    case 8: return (const void*)k_cg<8>;
+    case 1: return (const void*)k_cg<1, 256>;
+    case 2: return (const void*)k_cg<2, 256>;
+    case 4: return (const void*)k_cg<4, 256>;
+    case 8: return (const void*)k_cg<8, 256>;
   }
   return nullptr;
 }
@@ -1378,7 +1396,7 @@ struct fem_system {
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
       *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
   uint16_t* d_lidx = nullptr;
-  int nchunks = 0, cg_k = 1;
+  int nchunks = 0, cg_k = 1, cg_nt = 256;
   std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups
   // bound
   int* b_offsets = nullptr;
@@ -1463,14 +1481,26 @@ struct fem_system {
     return std::max<size_t>(worst, 16);
   }
 
-  // Row partition of the persistent CG: per instance, contiguous chunks
-  // (at most 8*BLOCK rows each), row-aligned tiles inside every chunk.  All
-  // state changes are committed only after every check has passed.
+  // Row partition of the persistent CG: per instance, contiguous chunks,
+  // row-aligned tiles inside every chunk.  The 512-thread kernel (<= 2 rows
+  // per thread, one workgroup per CU) is preferred; instances too large for
+  // it use the 256-thread kernel (<= 8 rows per thread).  All state changes
+  // are committed only after every check has passed.
   int plan_cg() {
+    const int rc = plan_cg_nt(512);
+    if (rc == FEM_OK) return rc;
+    const std::string why = g_error;
+    const int rc2 = plan_cg_nt(256);
+    if (rc2 != FEM_OK) g_error = why + "; " + g_error;
+    return rc2;
+  }
+
+  int plan_cg_nt(const int nt) {
     int dev = 0, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const long max_rows_chunk = 8L * BLOCK;
+    const int kmax = nt == 512 ? 2 : 8;
+    const long max_rows_chunk = (long)kmax * nt;
     std::vector<int> per(ninst, 1);
     std::vector<double> cost(ninst);
     long need = 0;
@@ -1482,9 +1512,11 @@ struct fem_system {
       cost[k] = (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]) + 8.0 * rows;
       total += cost[k];
     }
-    // Default chunk count: about 2 rows per thread for large meshes, one
-    // workgroup for small ones; cg_wg_request overrides.
-    long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, (long)P / (2 * BLOCK));
+    // Default chunk count: about 512 rows per workgroup for small meshes, at
+    // most one 512-thread workgroup per CU for large ones (2 rows per thread
+    // of the 256-thread kernel); cg_wg_request overrides.
+    const long natural = nt == 512 ? std::min<long>(P / 512, ncu) : (long)P / 512;
+    long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, natural);
     target = std::max(target, need);
     if (target > need && total > 0) {
       long extra = target - need;
@@ -1503,7 +1535,7 @@ struct fem_system {
     int maxrows = 0;
     for (int attempt = 0;; ++attempt) {
       long n = count();
-      if (n > 8 && (n & 7)) {  // pad to a multiple of 8 (XCD swizzle) on the largest instance
+      if (cg_wg_request == 0 && n > 8 && (n & 7)) {  // automatic count: pad to a multiple of 8 (XCD swizzle)
         int big = (int)(std::max_element(cost.begin(), cost.end()) - cost.begin());
         long rows = inst_off[big + 1] - inst_off[big];
         long pad = 8 - (n & 7);
@@ -1528,7 +1560,7 @@ struct fem_system {
         inst_chunks.push_back((int)chunk_rows.size());
       }
       if (maxrows <= max_rows_chunk) break;
-      if (attempt > 64) return fail(FEM_ERR_UNSUPPORTED, "cannot split the CG rows into chunks of <= 2048 rows");
+      if (attempt > 64) return fail(FEM_ERR_UNSUPPORTED, "cannot split the CG rows into small enough chunks");
       per[worst] += 1;
     }
     chunk_rows.push_back(P);
@@ -1541,7 +1573,7 @@ struct fem_system {
       while (r < r1) {
         tile_rows.push_back(r);
         int e = r;
-        while (e < r1 && pt.offsets[e + 1] - pt.offsets[r] <= CG_TILE) ++e;
+        while (e < r1 && pt.offsets[e + 1] - pt.offsets[r] <= (int64_t)CG_J * nt) ++e;
         if (e == r) return fail(FEM_ERR_UNSUPPORTED, "row longer than a CG tile");
         r = e;
       }
@@ -1566,11 +1598,11 @@ struct fem_system {
         tile_cptr.push_back((int32_t)tile_cols.size());
       }
     }
-    const int k_rows = maxrows <= BLOCK ? 1 : maxrows <= 2 * BLOCK ? 2 : maxrows <= 4 * BLOCK ? 4 : 8;
-    const void* kp = cg_kernel_ptr(k_rows);
+    const int k_rows = maxrows <= nt ? 1 : maxrows <= 2 * nt ? 2 : maxrows <= 4 * nt ? 4 : 8;
+    const void* kp = cg_kernel_ptr(k_rows, nt);
     if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
     int nb = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, BLOCK, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, nt, 0));
     // Launch groups of whole instances, each small enough that all of its
     // workgroups are co-resident (instances never span two groups).
     const int cap = nb * ncu;
@@ -1627,6 +1659,7 @@ struct fem_system {
     nchunks = n_chunks;
     cg_groups = groups;
     cg_k = k_rows;
+    cg_nt = nt;
     max_resident = nb * ncu;
     return FEM_OK;
   }
@@ -1908,7 +1941,7 @@ CgArgs cg_args(const fem_system* s) {
 
 // Barrier reset + one persistent launch per group of whole instances.
 int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
-  const void* kp = cg_kernel_ptr(s->cg_k);
+  const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
   hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_bar, s->ninst,
                      s->d_err);
@@ -1916,7 +1949,7 @@ int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
   for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
     ca.chunk_base = s->cg_groups[g];
     void* args[] = {&ca};
-    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(BLOCK), args, 0, st));
+    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(s->cg_nt), args, 0, st));
   }
   return FEM_OK;
 }

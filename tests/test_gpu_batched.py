@@ -61,21 +61,23 @@ def test_batched_bit_identical_to_looped(batch):
     kw = dict(dt=dt, linear_solver_tolerance=1e-2, linear_solver_maxiter=50)
     model = build_tet_model(merged, device=DEV)
     s = solver_for(model, instance_offsets=offs, **kw)
-    s.set_launch(0, 0, len(scenes))
-    assert s.launch_config()["cg_workgroups"] == len(scenes)
+    s.set_launch(0, 0, len(scenes))  # the minimum: every instance gets the chunks it needs, no more
     out = rollout(s, model, steps, dt)
     it_b = s.solve_stats()["iterations"]
     assert len(it_b) == len(scenes) and min(it_b) > 0
     dv_b, q_b = np_(s.dv), np_(out.particle_q)
+    looped_chunks = 0
     for k, sc in enumerate(scenes):
         m = build_tet_model(sc, device=DEV)
         sk = solver_for(m, **kw)
         sk.set_launch(0, 0, 1)
+        looped_chunks += sk.launch_config()["cg_workgroups"]
         ok = rollout(sk, m, steps, dt)
         lo, hi = int(offs[k]), int(offs[k + 1])
         np.testing.assert_array_equal(dv_b[lo:hi], np_(sk.dv), err_msg=f"instance {k}")
         np.testing.assert_array_equal(q_b[lo:hi], np_(ok.particle_q), err_msg=f"instance {k}")
         assert sk.solve_stats()["iterations"][0] == it_b[k]
+    assert s.launch_config()["cg_workgroups"] == looped_chunks
 
 
 def test_batched_launch_groups(batch):
