@@ -116,9 +116,20 @@ int fem_set_params(fem_system* sys, const fem_params* params);
 int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups);
 /* Current launch shapes; cg_launch_groups > 1 when the instances' CG
  * workgroups do not all fit on the device at once and the solve runs as
- * consecutive launches over groups of whole instances. */
+ * consecutive launches over groups of whole instances; cg_variant is
+ * FEM_CG_STREAMED or FEM_CG_RESIDENT. */
 int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups);
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups, int32_t* cg_variant);
+
+/* Persistent CG kernel variant (the same iteration as warp.optim.linear.cg's
+ * block-Jacobi PCG, linear.py cg, as called by SolverFEM._solve_linear_system
+ * solver.py:1206-1286; the choice changes only the fp32 summation order):
+ * FEM_CG_STREAMED reads the matrix from HBM in every iteration;
+ * FEM_CG_RESIDENT keeps each workgroup's blocks in registers and LDS for the
+ * whole solve (one workgroup per CU; FEM_ERR_UNSUPPORTED when a chunk does
+ * not fit); FEM_CG_AUTO (default) picks resident whenever it fits. */
+enum { FEM_CG_AUTO = 0, FEM_CG_STREAMED = 1, FEM_CG_RESIDENT = 2 };
+int fem_set_cg_variant(fem_system* sys, int32_t variant);
 
 /* FixedParticles.restore / restore_positions (fixed_particles.py:11-21,
  * 152-168): q[i] = anchor[i] (and qd[i] = 0 when qd_dev is not NULL). */

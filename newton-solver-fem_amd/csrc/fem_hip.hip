@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fem_hip.h"
@@ -730,6 +731,12 @@ struct CgArgs {
   const int* tile_cptr;    // [ntiles+1] offsets into tile_cols
   const int* tile_cols;    // distinct columns of each tile, ascending
   const uint16_t* lidx;    // [nnz] tile-local column number of each block
+  // Resident variant (one tile per chunk): per thread {first block within the
+  // chunk, block count, first row run} and its run-end mask; per row {first
+  // run within the chunk, run count}.
+  const int4* res_thr;
+  const unsigned long long* res_mask;
+  const int2* res_row;
   const unsigned* rhs_max;
   int chunk_base;     // first chunk of this launch group
   int nchunks_total;  // chunks of all groups (stride of the double-buffered partials)
@@ -1030,6 +1037,159 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
   }
 }
 
+// Resident variant of the SpMV: the chunk's blocks stay on chip for the whole
+// solve (RES_JR per thread in registers, up to RES_JL more in LDS; at most
+// RES_JS more per thread are streamed from HBM every iteration, loaded before
+// the operands are formed so that their latency overlaps that step).  Thread
+// t holds a contiguous run of the chunk's blocks; it accumulates their
+// products row by row and writes one partial per row run (a row split
+// between threads has two or more runs), and the owner thread of each row
+// adds its runs in order.
+constexpr int RES_NT = 256;  // one wave per SIMD: 512 registers per lane
+constexpr int RES_K = 4;     // rows per thread (<= 1024 rows per chunk)
+constexpr int RES_JR = 28;   // blocks per thread held in registers
+constexpr int RES_JL = 10;   // blocks per thread held in LDS
+constexpr int RES_JS = 4;    // blocks per thread streamed every iteration
+constexpr int RES_UC = 11;   // distinct columns per thread (< 2816 per chunk)
+constexpr int RES_SEG = RES_K * RES_NT + RES_NT;  // row runs per chunk
+// Chunk-local column number of padding blocks: an operand slot that stays zero.
+constexpr int RES_ZERO = RES_UC * RES_NT - 1;
+
+struct ResThread {
+  int k0;   // first block of the thread (global block index)
+  int cnt;  // blocks of the thread
+  int run0;
+  unsigned long long mask;  // bit j: block j ends a row run
+};
+static_assert(RES_JR <= 32 && RES_JR % 4 == 0 && RES_JR + RES_JL + RES_JS <= 64, "run-end mask layout");
+
+template <int K, bool PROLOGUE>
+__device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, const float (&A)[RES_JR][9],
+                                         const uint32_t (&L)[RES_JR / 2], const int (&ucol)[RES_UC],
+                                         const float* mat_l, const uint16_t* lidx_l, f4v* u_l, f4v* seg_l,
+                                         float s, float alpha, float beta, const float* __restrict__ xw,
+                                         const int (&rf)[K], const int (&rn)[K], const bool (&fx)[K],
+                                         float (&q)[K][3]) {
+  constexpr int NT = RES_NT;
+  // Streamed tier.
+  float B[RES_JS][9];
+  uint32_t bl[RES_JS];
+#pragma unroll
+  for (int i = 0; i < RES_JS; ++i) {
+    const int j = RES_JR + RES_JL + i;
+    bl[i] = RES_ZERO;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) B[i][e] = 0.f;
+    if (j < rt.cnt) {
+      const float* b = a.vals + 9 * (size_t)(rt.k0 + j);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const f3v v = *reinterpret_cast<const f3v*>(b + 3 * r);
+        B[i][3 * r] = v.x;
+        B[i][3 * r + 1] = v.y;
+        B[i][3 * r + 2] = v.z;
+      }
+      bl[i] = a.lidx[rt.k0 + j];
+    }
+  }
+  // Operands of the chunk's distinct columns, four columns at a time (bounds
+  // the registers the gathered records occupy).  ucol[i] < 0 marks a slot
+  // past the chunk's columns: it loads column 0 and stores a zero operand
+  // (such slots include the zero slot RES_ZERO).
+#pragma unroll
+  for (int i0 = 0; i0 < RES_UC; i0 += 4) {
+#pragma unroll
+    for (int i = i0; i < i0 + 4 && i < RES_UC; ++i) {
+      const bool live = ucol[i] >= 0;
+      const uint32_t c = live ? (uint32_t)ucol[i] : 0u;
+      v3 u;
+      if (PROLOGUE) {
+        u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, (int)c) * s;
+      } else {
+        const float* rec = xw + (size_t)(c * (uint32_t)XW);
+        const f4v* e = reinterpret_cast<const f4v*>(rec);
+        const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
+        const float p2 = ld_shared(rec + 8);
+        u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
+             fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
+      }
+      u_l[threadIdx.x + NT * i] = live ? f4v{u.x, u.y, u.z, 0.f} : f4v{0.f, 0.f, 0.f, 0.f};
+    }
+    asm volatile("" ::: "memory");
+  }
+  __syncthreads();
+  // Block products accumulated per row run.  Every block stores the running
+  // partial of its run (no divergent branches); the slot advances after a
+  // run's last block, so each slot ends up holding its run's full partial.
+  // Padding blocks (past the thread's count) store to the unused slot RES_SEG.
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  int run = rt.run0;
+  uint32_t mlo = (uint32_t)rt.mask, mhi = (uint32_t)(rt.mask >> 32);
+  int cnt = rt.cnt;
+  // Opaque copies: keeps the compiler from hoisting per-block addresses and
+  // run-end flags out of the solve loop (they would occupy registers for the
+  // whole solve); recomputing them is a few ALU operations per block.
+  uint32_t Lw[RES_JR / 2];
+#pragma unroll
+  for (int i = 0; i < RES_JR / 2; ++i) {
+    Lw[i] = L[i];
+    asm volatile("" : "+v"(Lw[i]));
+  }
+  asm volatile("" : "+v"(mlo), "+v"(mhi), "+v"(run), "+v"(cnt));
+  auto block = [&](int j, const f4v& u, float a0, float a1, float a2, float a3, float a4, float a5, float a6,
+                   float a7, float a8) {
+    acc.x += a0 * u.x + a1 * u.y + a2 * u.z;
+    acc.y += a3 * u.x + a4 * u.y + a5 * u.z;
+    acc.z += a6 * u.x + a7 * u.y + a8 * u.z;
+    seg_l[j < cnt ? run : RES_SEG] = acc;
+    const bool e = ((((uint64_t)mhi << 32) | mlo) >> j) & 1u;
+    run += e;
+    acc = e ? f4v{0.f, 0.f, 0.f, 0.f} : acc;
+  };
+  // Register tier; operand loads are issued four blocks ahead of their products.
+  constexpr int G = 4;
+#pragma unroll
+  for (int g = 0; g < RES_JR; g += G) {
+    f4v u[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int j = g + i;
+      u[i] = u_l[(Lw[j >> 1] >> (16 * (j & 1))) & 0xffffu];
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int j = g + i;
+      block(j, u[i], A[j][0], A[j][1], A[j][2], A[j][3], A[j][4], A[j][5], A[j][6], A[j][7], A[j][8]);
+    }
+  }
+  // LDS tier.
+  constexpr int S = RES_JL * NT;  // element e of an LDS block at m[e * S]
+  const int jle = min(cnt, RES_JR + RES_JL);
+  for (int j = RES_JR; j < jle; ++j) {
+    const int jl = j - RES_JR;
+    const f4v u = u_l[lidx_l[jl * NT + threadIdx.x]];
+    const float* m = mat_l + jl * NT + threadIdx.x;
+    block(j, u, m[0], m[S], m[2 * S], m[3 * S], m[4 * S], m[5 * S], m[6 * S], m[7 * S], m[8 * S]);
+  }
+  // Streamed tier.
+#pragma unroll
+  for (int i = 0; i < RES_JS; ++i) {
+    const f4v u = u_l[bl[i]];
+    block(RES_JR + RES_JL + i, u, B[i][0], B[i][1], B[i][2], B[i][3], B[i][4], B[i][5], B[i][6], B[i][7], B[i][8]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    f4v t = {0.f, 0.f, 0.f, 0.f};
+    if (!fx[k])
+      for (int m = rf[k]; m < rf[k] + rn[k]; ++m) t += seg_l[m];
+    q[k][0] = t.x;
+    q[k][1] = t.y;
+    q[k][2] = t.z;
+  }
+  // The next call writes u_l first and seg_l only after its own barrier.
+}
+
 __device__ __forceinline__ void st_xw(float* xw, int i, const float (&z)[3], const float (&w)[3],
                                       const float (&p)[3]) {
   float4* e = reinterpret_cast<float4*>(xw + XW * (size_t)i);
@@ -1058,11 +1218,25 @@ __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-
 // k+1, so recurrence error does not accumulate.  Exchange records and
 // partial sums are double buffered: phase k writes buffer (k+1)&1, which no
 // workgroup reads after the barrier that ended phase k-1.
-template <int K, int NT>
+//
+// RES selects the resident SpMV (res_spmv: blocks kept on chip, RES_NT
+// threads, RES_K rows per thread) instead of the streamed tiles (cg_spmv).
+template <int K, int NT, bool RES>
 __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
-  __shared__ float part_l[3 * (CG_J * NT)];
-  __shared__ float u_l[3 * (CG_J * NT)];
-  __shared__ int off_l[K * NT + 1];
+  static_assert(!RES || (NT == RES_NT && K == RES_K), "resident CG shape");
+  constexpr size_t SM_STREAM = sizeof(float) * (6 * CG_J * NT + K * NT + 1);
+  constexpr size_t SM_RES = sizeof(f4v) * (RES_UC * NT + RES_SEG + 1) + sizeof(float) * 9 * RES_JL * NT +
+                            sizeof(uint16_t) * RES_JL * NT;
+  __shared__ __align__(16) unsigned char smem[RES ? SM_RES : SM_STREAM];
+  // streamed tiles
+  float* part_l = reinterpret_cast<float*>(smem);
+  float* u_l = part_l + 3 * CG_J * NT;
+  int* off_l = reinterpret_cast<int*>(u_l + 3 * CG_J * NT);
+  // resident blocks
+  f4v* ures_l = reinterpret_cast<f4v*>(smem);
+  f4v* seg_l = ures_l + RES_UC * NT;
+  float* mat_l = reinterpret_cast<float*>(seg_l + RES_SEG + 1);
+  uint16_t* lidx_l = reinterpret_cast<uint16_t*>(mat_l + 9 * RES_JL * NT);
   __shared__ double red[(NT / 64) * NPART];
   __shared__ int bflag;
   __shared__ int xinfo[3];
@@ -1078,7 +1252,8 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
   const bool zero_start = (m == 0.f) || a.x_out != nullptr;
 
-  for (int r = threadIdx.x; r <= nr; r += NT) off_l[r] = a.offsets[R0 + r];
+  if (!RES)
+    for (int r = threadIdx.x; r <= nr; r += NT) off_l[r] = a.offsets[R0 + r];
 
   // Register on this workgroup's XCC and learn the instance's XCC layout.
   XcdGroup xg{0, 1, 1};
@@ -1127,12 +1302,79 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     return mv(m, v);
   };
   int ucol[CG_J];
-  load_tile_cols<NT>(a, t0, ucol);
+  // Resident blocks, their chunk-local column numbers (two per word), the
+  // distinct columns this thread forms operands for, and the row runs of the
+  // own rows.
+  constexpr int JR = RES ? RES_JR : 1;
+  float A[RES_JR][9];
+  uint32_t L[RES_JR / 2];
+  int rcol[RES_UC];
+  int rf[K], rn[K];
+  ResThread rt{0, 0, 0, 0ull};
+  if constexpr (RES) {
+    const int4 th = a.res_thr[(size_t)c * NT + threadIdx.x];
+    rt.k0 = a.offsets[R0] + th.x;
+    rt.cnt = th.y;
+    rt.run0 = th.z;
+    rt.mask = a.res_mask[(size_t)c * NT + threadIdx.x];
+    const size_t k0 = (size_t)rt.k0;
+#pragma unroll
+    for (int j = 0; j < JR; ++j) {
+      uint32_t lid = RES_ZERO;  // padding: zero block against the zero operand
+      if (j < rt.cnt) {
+        const float* b = a.vals + 9 * (k0 + j);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const f3v v = *reinterpret_cast<const f3v*>(b + 3 * r);
+          A[j][3 * r] = v.x;
+          A[j][3 * r + 1] = v.y;
+          A[j][3 * r + 2] = v.z;
+        }
+        lid = a.lidx[k0 + j];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) A[j][e] = 0.f;
+      }
+      if (j & 1) L[j >> 1] |= lid << 16;
+      else L[j >> 1] = lid;
+    }
+    for (int j = RES_JR; j < min(rt.cnt, RES_JR + RES_JL); ++j) {
+      const int jl = j - RES_JR;
+      const float* b = a.vals + 9 * (k0 + j);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) mat_l[(e * RES_JL + jl) * NT + threadIdx.x] = b[e];
+      lidx_l[jl * NT + threadIdx.x] = a.lidx[k0 + j];
+    }
+    const int c0 = a.tile_cptr[c], nu = a.tile_cptr[c + 1] - c0;
+#pragma unroll
+    for (int i = 0; i < RES_UC; ++i) {
+      const int u = threadIdx.x + NT * i;
+      rcol[i] = u < nu ? a.tile_cols[c0 + u] : -1;
+    }
+    if (threadIdx.x == 0) ures_l[RES_ZERO] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int lr = threadIdx.x + NT * k;
+      const int2 rr2 = own[k] ? a.res_row[R0 + lr] : make_int2(0, 0);
+      rf[k] = rr2.x;
+      rn[k] = rr2.y;
+    }
+    __syncthreads();
+  } else {
+    load_tile_cols<NT>(a, t0, ucol);
+  }
+  auto spmv = [&](auto prologue, float alpha, float beta, const float* xw) {
+    constexpr bool PRO = decltype(prologue)::value;
+    if constexpr (RES)
+      res_spmv<K, PRO>(a, rt, A, L, rcol, mat_l, lidx_l, ures_l, seg_l, s, alpha, beta, xw, rf, rn, fx, qv);
+    else
+      cg_spmv<K, PRO, NT>(a, R0, nr, t0, t1, off_l, s, alpha, beta, xw, part_l, u_l, ucol, fx, qv);
+  };
 
   // Prologue: r0 = b - A x0, z0 = M^-1 r0; records (z0, 0, 0) so that the
   // first phase gathers p0 = z0.
   if (!zero_start) {
-    cg_spmv<K, true, NT>(a, R0, nr, t0, t1, off_l, s, 0.f, 0.f, a.xw, part_l, u_l, ucol, fx, qv);
+    spmv(std::true_type{}, 0.f, 0.f, a.xw);
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k][0] = qv[k][1] = qv[k][2] = 0.f;
@@ -1214,8 +1456,7 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       }
     }
     if (ph == a.maxiter) break;
-    cg_spmv<K, false, NT>(a, R0, nr, t0, t1, off_l, s, alpha, beta, a.xw + XW * (size_t)a.P * cur, part_l, u_l,
-                      ucol, fx, qv);
+    spmv(std::false_type{}, alpha, beta, a.xw + XW * (size_t)a.P * cur);
     float pp[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float* xw_next = a.xw + XW * (size_t)a.P * (cur ^ 1);
 #pragma unroll
@@ -1346,20 +1587,22 @@ struct LaunchVjpRows {
   }
 };
 
-// Rows per thread K and threads per workgroup NT of the persistent CG.
-const void* cg_kernel_ptr(int k, int nt) {
+// Rows per thread K and threads per workgroup NT of the persistent CG
+// (res: the resident variant, RES_K rows and RES_NT threads).
+const void* cg_kernel_ptr(int k, int nt, bool res) {
+  if (res) return (k == RES_K && nt == RES_NT) ? (const void*)k_cg<RES_K, RES_NT, true> : nullptr;
   if (nt == 512) {
     switch (k) {
-      case 1: return (const void*)k_cg<1, 512>;
-      case 2: return (const void*)k_cg<2, 512>;
+      case 1: return (const void*)k_cg<1, 512, false>;
+      case 2: return (const void*)k_cg<2, 512, false>;
     }
     return nullptr;
   }
   switch (k) {
-    case 1: return (const void*)k_cg<1, 256>;
-    case 2: return (const void*)k_cg<2, 256>;
-    case 4: return (const void*)k_cg<4, 256>;
-    case 8: return (const void*)k_cg<8, 256>;
+    case 1: return (const void*)k_cg<1, 256, false>;
+    case 2: return (const void*)k_cg<2, 256, false>;
+    case 4: return (const void*)k_cg<4, 256, false>;
+    case 8: return (const void*)k_cg<8, 256, false>;
   }
   return nullptr;
 }
@@ -1396,7 +1639,11 @@ struct fem_system {
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
       *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
   uint16_t* d_lidx = nullptr;
+  int *d_res_thr = nullptr, *d_res_row = nullptr;
+  uint64_t* d_res_mask = nullptr;
   int nchunks = 0, cg_k = 1, cg_nt = 256;
+  bool cg_res = false;
+  int cg_variant_request = 0;  // 0 any, 1 streamed, 2 resident
   std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups
   // bound
   int* b_offsets = nullptr;
@@ -1482,24 +1729,32 @@ struct fem_system {
   }
 
   // Row partition of the persistent CG: per instance, contiguous chunks,
-  // row-aligned tiles inside every chunk.  The 512-thread kernel (<= 2 rows
-  // per thread, one workgroup per CU) is preferred; instances too large for
-  // it use the 256-thread kernel (<= 8 rows per thread).  All state changes
-  // are committed only after every check has passed.
+  // row-aligned tiles inside every chunk.  Preference order: the resident
+  // kernel (every chunk's blocks fit on chip, one workgroup per CU), the
+  // streamed 512-thread kernel (<= 2 rows per thread, one workgroup per CU),
+  // the streamed 256-thread kernel (<= 8 rows per thread).  cg_variant_request
+  // 1 / 2 restricts the choice to streamed / resident.  All state changes are
+  // committed only after every check has passed.
   int plan_cg() {
-    const int rc = plan_cg_nt(512);
+    std::string why;
+    if (cg_variant_request != 1) {
+      const int rc = plan_cg_nt(RES_NT, true);
+      if (rc == FEM_OK || cg_variant_request == 2) return rc;
+      why = g_error + "; ";
+    }
+    const int rc = plan_cg_nt(512, false);
     if (rc == FEM_OK) return rc;
-    const std::string why = g_error;
-    const int rc2 = plan_cg_nt(256);
+    why += g_error;
+    const int rc2 = plan_cg_nt(256, false);
     if (rc2 != FEM_OK) g_error = why + "; " + g_error;
     return rc2;
   }
 
-  int plan_cg_nt(const int nt) {
+  int plan_cg_nt(const int nt, const bool res) {
     int dev = 0, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int kmax = nt == 512 ? 2 : 8;
+    const int kmax = res ? RES_K : nt == 512 ? 2 : 8;
     const long max_rows_chunk = (long)kmax * nt;
     std::vector<int> per(ninst, 1);
     std::vector<double> cost(ninst);
@@ -1514,8 +1769,12 @@ struct fem_system {
     }
     // Default chunk count: about 512 rows per workgroup for small meshes, at
     // most one 512-thread workgroup per CU for large ones (2 rows per thread
-    // of the 256-thread kernel); cg_wg_request overrides.
-    const long natural = nt == 512 ? std::min<long>(P / 512, ncu) : (long)P / 512;
+    // of the 256-thread kernel); the resident kernel also needs enough chunks
+    // for every chunk's blocks to fit on chip.  cg_wg_request overrides.
+    const long res_blocks = (long)RES_NT * (RES_JR + RES_JL + RES_JS);  // blocks one resident chunk holds
+    long natural = (nt == 512 || res) ? std::min<long>(P / 512, ncu) : (long)P / 512;
+    if (res)
+      natural = std::min<long>(ncu, std::max<long>(natural, (long)std::ceil((double)pt.offsets[P] / (0.95 * res_blocks))));
     long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, natural);
     target = std::max(target, need);
     if (target > need && total > 0) {
@@ -1532,6 +1791,7 @@ struct fem_system {
       return n;
     };
     std::vector<int32_t> chunk_rows, chunk_inst, inst_chunks;
+    std::vector<int32_t> stamp(res ? P : 0, -1);
     int maxrows = 0;
     for (int attempt = 0;; ++attempt) {
       long n = count();
@@ -1546,9 +1806,11 @@ struct fem_system {
       inst_chunks.assign(1, 0);
       maxrows = 0;
       int worst = -1;
+      std::vector<int> over;  // instances with a chunk the kernel cannot hold
       for (int k = 0; k < ninst; ++k) {
         std::vector<int32_t> bnd;
         split_rows(pt.offsets, inst_off[k], inst_off[k + 1], per[k], bnd);
+        bool bad = false;
         for (int j = 0; j < per[k]; ++j) {
           chunk_rows.push_back(bnd[j]);
           chunk_inst.push_back(k);
@@ -1556,20 +1818,39 @@ struct fem_system {
             maxrows = bnd[j + 1] - bnd[j];
             worst = k;
           }
+          if (res && !bad) {  // on-chip capacity: blocks and distinct columns of the chunk
+            const int64_t kb = pt.offsets[bnd[j]], ke = pt.offsets[bnd[j + 1]];
+            int ncols = 0;
+            const int id = (int)chunk_rows.size();
+            for (int64_t q = kb; q < ke; ++q)
+              if (stamp[pt.columns[q]] != id) {
+                stamp[pt.columns[q]] = id;
+                ++ncols;
+              }
+            bad = ke - kb > res_blocks || ncols > RES_ZERO || bnd[j + 1] - bnd[j] > max_rows_chunk;
+          }
         }
+        if (bad) over.push_back(k);
         inst_chunks.push_back((int)chunk_rows.size());
       }
-      if (maxrows <= max_rows_chunk) break;
-      if (attempt > 64) return fail(FEM_ERR_UNSUPPORTED, "cannot split the CG rows into small enough chunks");
-      per[worst] += 1;
+      if (maxrows <= max_rows_chunk && over.empty()) break;
+      if (attempt > (res ? 256 : 64)) return fail(FEM_ERR_UNSUPPORTED, "cannot split the CG rows into small enough chunks");
+      if (over.empty()) over.push_back(worst);
+      for (int k : over) per[k] += res ? std::max(1, per[k] / 64) : 1;
+      std::fill(stamp.begin(), stamp.end(), -1);
     }
     chunk_rows.push_back(P);
     const int n_chunks = (int)chunk_inst.size();
-    // Row-aligned tiles of at most CG_TILE blocks inside every chunk.
+    // Row-aligned tiles of at most CG_J * nt blocks inside every chunk (the
+    // resident kernel: one tile per chunk).
     std::vector<int32_t> chunk_tiles(1, 0), tile_rows;
     for (int c = 0; c < n_chunks; ++c) {
       int r = chunk_rows[c];
       const int r1 = chunk_rows[c + 1];
+      if (res) {
+        tile_rows.push_back(r);
+        r = r1;
+      }
       while (r < r1) {
         tile_rows.push_back(r);
         int e = r;
@@ -1598,8 +1879,48 @@ struct fem_system {
         tile_cptr.push_back((int32_t)tile_cols.size());
       }
     }
-    const int k_rows = maxrows <= nt ? 1 : maxrows <= 2 * nt ? 2 : maxrows <= 4 * nt ? 4 : 8;
-    const void* kp = cg_kernel_ptr(k_rows, nt);
+    // Resident kernel: thread t of a chunk holds blocks [t nb / NT, (t+1) nb / NT)
+    // of the chunk's nb; a row run ends at a row's last block or at the
+    // thread's last block.  Runs are numbered in (thread, block) order.
+    std::vector<int32_t> res_thr, res_row;
+    std::vector<uint64_t> res_mask;
+    if (res) {
+      res_thr.assign((size_t)n_chunks * nt * 4, 0);
+      res_mask.assign((size_t)n_chunks * nt, 0);
+      res_row.assign((size_t)2 * P, 0);
+      for (int c = 0; c < n_chunks; ++c) {
+        const int r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
+        const int64_t kb = pt.offsets[r0], nb = pt.offsets[r1] - kb;
+        const int ncols = tile_cptr[c + 1] - tile_cptr[c];
+        if (ncols > RES_ZERO) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many distinct columns in a chunk");
+        int run = 0, row = r0;
+        for (int t = 0; t < nt; ++t) {
+          const int64_t b0 = nb * t / nt, b1 = nb * (t + 1) / nt;
+          if (b1 - b0 > RES_JR + RES_JL + RES_JS)
+            return fail(FEM_ERR_UNSUPPORTED, "resident CG: chunk exceeds the on-chip capacity");
+          int32_t* th = &res_thr[((size_t)c * nt + t) * 4];
+          th[0] = (int32_t)b0;
+          th[1] = (int32_t)(b1 - b0);
+          th[2] = run;
+          uint64_t mask = 0;
+          for (int64_t b = b0; b < b1; ++b) {
+            while (pt.offsets[row + 1] <= kb + b) ++row;
+            if (b == b0 || pt.offsets[row] == kb + b) {  // a run of `row` starts here
+              if (res_row[2 * row + 1] == 0) res_row[2 * row] = run;
+              ++res_row[2 * row + 1];
+            }
+            if (pt.offsets[row + 1] == kb + b + 1 || b + 1 == b1) {
+              mask |= 1ull << (b - b0);
+              ++run;
+            }
+          }
+          res_mask[(size_t)c * nt + t] = mask;
+        }
+        if (run > RES_SEG) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many row runs in a chunk");
+      }
+    }
+    const int k_rows = res ? RES_K : maxrows <= nt ? 1 : maxrows <= 2 * nt ? 2 : maxrows <= 4 * nt ? 4 : 8;
+    const void* kp = cg_kernel_ptr(k_rows, nt, res);
     if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, nt, 0));
@@ -1626,10 +1947,15 @@ struct fem_system {
       groups.push_back(n_chunks);
     }
     int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr,
-        *n_tcptr = nullptr, *n_tcols = nullptr;
+        *n_tcptr = nullptr, *n_tcols = nullptr, *n_rthr = nullptr, *n_rrow = nullptr;
+    uint64_t* n_rmask = nullptr;
     uint16_t* n_lidx = nullptr;
     double* n_part = nullptr;
     int rc;
+    if (res && ((rc = upload(&n_rthr, res_thr.data(), res_thr.size())) ||
+                (rc = upload(&n_rmask, res_mask.data(), res_mask.size())) ||
+                (rc = upload(&n_rrow, res_row.data(), res_row.size()))))
+      return rc;
     if ((rc = upload(&n_tcptr, tile_cptr.data(), tile_cptr.size())) ||
         (rc = upload(&n_tcols, tile_cols.data(), tile_cols.size())) ||
         (rc = upload(&n_lidx, lidx.data(), lidx.size())) ||
@@ -1641,7 +1967,7 @@ struct fem_system {
       return rc;
     for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
                     (void*)d_chunk_tiles, (void*)d_tile_rows, (void*)d_tile_cptr, (void*)d_tile_cols,
-                    (void*)d_lidx}) {
+                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row}) {
       if (p) {
         (void)hipFree(p);
         allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
@@ -1655,6 +1981,10 @@ struct fem_system {
     d_tile_cptr = n_tcptr;
     d_tile_cols = n_tcols;
     d_lidx = n_lidx;
+    d_res_thr = n_rthr;
+    d_res_mask = n_rmask;
+    d_res_row = n_rrow;
+    cg_res = res;
     d_part = n_part;
     nchunks = n_chunks;
     cg_groups = groups;
@@ -1846,13 +2176,29 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int
 }
 
 int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups) {
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups, int32_t* cg_variant) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (cg_variant) *cg_variant = s->cg_res ? FEM_CG_RESIDENT : FEM_CG_STREAMED;
   if (cg_launch_groups) *cg_launch_groups = s->cg_groups.empty() ? 0 : (int32_t)s->cg_groups.size() - 1;
   if (elem_lanes) *elem_lanes = s->elem_g;
   if (assemble_rows) *assemble_rows = s->asm_rb;
   if (cg_workgroups) *cg_workgroups = s->nchunks;
   if (cg_rows_per_thread) *cg_rows_per_thread = s->cg_k;
+  return FEM_OK;
+}
+
+int fem_set_cg_variant(fem_system* s, int32_t variant) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (variant < FEM_CG_AUTO || variant > FEM_CG_RESIDENT) return fail(FEM_ERR_ARG, "unknown CG variant");
+  const int old = s->cg_variant_request;
+  s->cg_variant_request = variant;
+  if (s->P > 0) {
+    const int rc = s->plan_cg();
+    if (rc) {  // a failed plan leaves the previous one in place
+      s->cg_variant_request = old;
+      return rc;
+    }
+  }
   return FEM_OK;
 }
 
@@ -1925,6 +2271,9 @@ CgArgs cg_args(const fem_system* s) {
   ca.tile_cptr = s->d_tile_cptr;
   ca.tile_cols = s->d_tile_cols;
   ca.lidx = s->d_lidx;
+  ca.res_thr = reinterpret_cast<const int4*>(s->d_res_thr);
+  ca.res_mask = reinterpret_cast<const unsigned long long*>(s->d_res_mask);
+  ca.res_row = reinterpret_cast<const int2*>(s->d_res_row);
   ca.chunk_inst = s->d_chunk_inst;
   ca.inst_chunks = s->d_inst_chunks;
   ca.rhs_max = s->d_rhs_max;
@@ -1941,7 +2290,7 @@ CgArgs cg_args(const fem_system* s) {
 
 // Barrier reset + one persistent launch per group of whole instances.
 int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
-  const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt);
+  const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt, s->cg_res);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
   hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_bar, s->ninst,
                      s->d_err);

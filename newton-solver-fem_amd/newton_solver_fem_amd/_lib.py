@@ -70,7 +70,8 @@ SIGNATURES = {
     "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "fem_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
-    "fem_get_launch": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip]),
+    "fem_get_launch": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "fem_set_cg_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "fem_restore_fixed": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "fem_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]),
     "fem_step": (ctypes.c_int, [ctypes.c_void_p] * 8 + [ctypes.c_float, ctypes.c_void_p]),

@@ -308,11 +308,25 @@ class SolverFEM:
         """Override kernel launch shapes (0 keeps the current value / auto)."""
         _lib.check(self._lib.fem_set_launch(self._sys, elem_lanes, assemble_rows, cg_workgroups), "fem_set_launch")
 
+    CG_VARIANTS = {"auto": 0, "streamed": 1, "resident": 2}
+
+    def set_cg_variant(self, variant: str = "auto"):
+        """Persistent CG kernel: "resident" keeps the matrix on chip for the whole
+        solve (one workgroup per CU; raises when a chunk does not fit),
+        "streamed" reads it from HBM every iteration, "auto" picks resident
+        whenever it fits.  Same iteration either way; only the fp32 summation
+        order differs."""
+        if variant not in self.CG_VARIANTS:
+            raise ValueError(f"cg variant must be one of {sorted(self.CG_VARIANTS)}")
+        _lib.check(self._lib.fem_set_cg_variant(self._sys, self.CG_VARIANTS[variant]), "fem_set_cg_variant")
+
     def launch_config(self) -> dict:
-        v = [ctypes.c_int32() for _ in range(5)]
+        v = [ctypes.c_int32() for _ in range(6)]
         _lib.check(self._lib.fem_get_launch(self._sys, *[ctypes.byref(x) for x in v]), "fem_get_launch")
         keys = ("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread", "cg_launch_groups")
-        return dict(zip(keys, (x.value for x in v)))
+        out = dict(zip(keys, (x.value for x in v[:5])))
+        out["cg_variant"] = {1: "streamed", 2: "resident"}[v[5].value]
+        return out
 
     def autotune(self, state: State, dt: float | None = None, *, elem_lanes=(4, 8, 16), assemble_rows=(8, 16, 32),
                  cg_workgroups=None, repeats: int = 3, control: Control | None = None) -> dict:

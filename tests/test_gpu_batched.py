@@ -53,15 +53,18 @@ def batch():
     return scenes, merged, offs
 
 
-def test_batched_bit_identical_to_looped(batch):
-    """One chunk per instance on both sides: per-instance arithmetic is identical, so is the result."""
+@pytest.mark.parametrize("variant", ["streamed", "resident"])
+def test_batched_bit_identical_to_looped(batch, variant):
+    """Same chunks per instance on both sides: per-instance arithmetic is identical, so is the result."""
     scenes, merged, offs = batch
-    print("case batched == looped", flush=True)
+    print(f"case batched == looped {variant}", flush=True)
     dt, steps = 1 / 600, 4
     kw = dict(dt=dt, linear_solver_tolerance=1e-2, linear_solver_maxiter=50)
     model = build_tet_model(merged, device=DEV)
     s = solver_for(model, instance_offsets=offs, **kw)
+    s.set_cg_variant(variant)
     s.set_launch(0, 0, len(scenes))  # the minimum: every instance gets the chunks it needs, no more
+    assert s.launch_config()["cg_variant"] == variant
     out = rollout(s, model, steps, dt)
     it_b = s.solve_stats()["iterations"]
     assert len(it_b) == len(scenes) and min(it_b) > 0
@@ -70,6 +73,7 @@ def test_batched_bit_identical_to_looped(batch):
     for k, sc in enumerate(scenes):
         m = build_tet_model(sc, device=DEV)
         sk = solver_for(m, **kw)
+        sk.set_cg_variant(variant)
         sk.set_launch(0, 0, 1)
         looped_chunks += sk.launch_config()["cg_workgroups"]
         ok = rollout(sk, m, steps, dt)
