@@ -35,10 +35,6 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
-// Barrier words of one CG instance (uint offsets inside its BAR_WORDS region;
-// every counter on its own 64-byte line).
-constexpr int BAR_WORDS = 512;
-constexpr int BAR_TOP_COUNT = 0, BAR_TOP_GEN = 16, BAR_HIER_TOP = 288, BAR_XCD = 32, BAR_REG = 320;
 // Per-instance words holding max|b| (reduced by the solve).
 constexpr int RHS_MAX_SLOTS = 64;
 // CG partial sums per chunk and phase.
@@ -118,18 +114,10 @@ __global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float
 // Per-solve reset of the CG barrier counters, XCC epochs and registration
 // and of the device error word (the flat barrier's generation word keeps
 // counting across launches).
-__global__ void k_cg_reset(unsigned* bar, int ninst, int* err) {
+// Clears the per-chunk publication flags of the persistent CG and the error word.
+__global__ void k_cg_reset(unsigned* flags, int nflags, int* err) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ninst) {
-    unsigned* b = bar + BAR_WORDS * (size_t)t;
-    b[BAR_TOP_COUNT] = 0u;
-    b[BAR_HIER_TOP] = 0u;
-    for (int x = 0; x < 8; ++x) {
-      b[BAR_XCD + 32 * x] = 0u;
-      b[BAR_XCD + 32 * x + 16] = 0u;
-      b[BAR_REG + x] = 0u;
-    }
-  }
+  if (t < nflags) flags[t] = 0u;
   if (t == 0) *err = 0;
 }
 
@@ -744,38 +732,24 @@ struct CgArgs {
   int maxiter;
   int early_exit;
   double* part;
-  unsigned* bar;
+  unsigned* flags;  // [nchunks_total] last phase each chunk has published
   int* stats;
   int* err;
 };
 
-__device__ __forceinline__ unsigned* bar_xcount(unsigned* b, int x) { return b + BAR_XCD + 32 * x; }
-__device__ __forceinline__ unsigned* bar_xgen(unsigned* b, int x) { return b + BAR_XCD + 32 * x + 16; }
-
-__device__ __forceinline__ unsigned ld_acq(unsigned* p) {
+__device__ __forceinline__ unsigned ld_acq(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Spin until *gen != g; bounded so that a launch whose workgroups are not all
-// resident ends with an error flag instead of hanging.
-__device__ __forceinline__ bool spin_gen(unsigned* gen, unsigned g, int* err) {
-  long spins = 0;
-  while (ld_acq(gen) == g) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > BARRIER_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-      __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-  }
-  return true;
-}
-
-// Spin until the monotonic counter *p reaches target (bounded like spin_gen).
-__device__ __forceinline__ bool spin_until(unsigned* p, unsigned target, int* err) {
+// Spin until the monotonic word *p reaches target; bounded so that a launch
+// whose workgroups are not all resident ends with an error flag instead of
+// hanging.
+__device__ __forceinline__ bool spin_until(const unsigned* p, unsigned target, int* err) {
   long spins = 0;
   while ((int)(ld_acq(p) - target) < 0) {
     __builtin_amdgcn_s_sleep(1);
-    if (++spins > BARRIER_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    if (++spins > BARRIER_SPIN_LIMIT ||
+        ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
       __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
@@ -783,91 +757,50 @@ __device__ __forceinline__ bool spin_until(unsigned* p, unsigned target, int* er
   return true;
 }
 
-// Flat barrier over n workgroups with full agent-scope release/acquire
-// (used once per launch, for the XCD registration).
-__device__ bool flat_barrier(unsigned* b, int n, int* err, int* lds_flag) {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  asm volatile("" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    if (n > 1) {
-      unsigned* count = b + BAR_TOP_COUNT;
-      unsigned* gen = b + BAR_TOP_GEN;
-      const unsigned g = ld_acq(gen);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (arrived == (unsigned)(n - 1)) {
-        __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        ok = spin_gen(gen, g, err);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    *lds_flag = ok;
-  }
-  __syncthreads();
-  return *lds_flag != 0;
-}
-
-struct XcdGroup {
-  int xcc;      // hardware XCC id of this workgroup
-  int nx;       // workgroups of this instance on that XCC
-  int ngroups;  // XCCs the instance spans
-};
-
-// Two-level barrier with monotonic counters (reset before every launch; the
-// caller passes the 1-based barrier index `epoch`).  Workgroups of one XCC
-// share its L2: each drains its own stores into L2 (s_waitcnt) and arrives on
-// the XCC counter.  The last arriver of an XCC (its leader) writes that L2
-// back, arrives on the top counter, waits for every XCC, invalidates its L2
-// once and releases its XCC through the XCC epoch word.  The other
-// workgroups only wait for that word: they read data published by other
-// workgroups with non-temporal loads, which bypass the vector L1 (see
-// ld_shared below), so no per-workgroup cache invalidate is needed.
-__device__ bool hier_barrier(unsigned* b, const XcdGroup& xg, int n, unsigned epoch, int* err, int* lds_flag) {
-  // s_waitcnt is per wave and the workgroup barrier's release does not wait
-  // for global stores: every wave drains its own stores into L2 first.
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  asm volatile("" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    if (n > 1) {
-      unsigned* xc = bar_xcount(b, xg.xcc);
-      unsigned* xgen = bar_xgen(b, xg.xcc);
-      const unsigned a = __hip_atomic_fetch_add(xc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a == (unsigned)xg.nx * epoch - 1u) {
-        if (xg.ngroups > 1) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          unsigned* tc = b + BAR_HIER_TOP;
-          __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = spin_until(tc, (unsigned)xg.ngroups * epoch, err);
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        asm volatile("" ::: "memory");
-        __hip_atomic_store(xgen, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        ok = spin_until(xgen, epoch, err);
-      }
-    }
-    *lds_flag = ok;
-  }
-  __syncthreads();
-  return *lds_flag != 0;
-}
-
-// Loads of data that other workgroups of the running CG publish between
-// barriers (exchange records, partial sums).  Non-temporal loads are served
-// from L2 and never from a line the vector L1 kept from an earlier phase.
+// Synchronisation of the persistent CG: publish-and-wait, no cache maintenance.
+// Everything a workgroup shares during a solve (exchange records, partial
+// sums) is written with write-through stores (`sc1`: the store is
+// acknowledged once it has reached memory, past the XCC's L2) and read by
+// the others with non-temporal loads, which on gfx950 are not served from a
+// stale L1 or L2 line of the reading XCC (checked with a cross-XCC probe:
+// write-through store + non-temporal load 0/2000 stale; a plain or `nt`
+// store stays invisible to other XCCs until an L2 write-back).  A phase
+// ends with every wave waiting for its own stores, then one thread raising
+// the chunk's flag to the phase's epoch; the next phase starts once every
+// chunk of the instance has raised its flag.  No L2 write-back or invalidate
+// is needed, so read-only data (inverse diagonal, streamed blocks, column
+// lists) stays cached across iterations.
+typedef double d2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f3v __attribute__((ext_vector_type(3), aligned(4)));
 __device__ __forceinline__ f4v ld_shared(const f4v* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ld_shared(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ double ld_shared(const double* p) { return __builtin_nontemporal_load(p); }
+// The stores carry no memory clobber, so that independent loads (inverse
+// diagonal blocks) may be scheduled across them; they stay ordered before
+// publish() because both are volatile asm, and nothing in the kernel reads
+// the stored data through a plain load.  The compiler's hazard recognizer
+// does not see into the asm: a store of more than 64 bits needs two wait
+// states before a VALU may overwrite its data registers (gfx940 family), so
+// the 128-bit stores end with s_nop 2.
+__device__ __forceinline__ void st_wt(float* p, f4v v) {
+  asm volatile("global_store_dwordx4 %0, %1, off sc1\n s_nop 2" ::"v"(p), "v"(v));
+}
+__device__ __forceinline__ void st_wt(float* p, float v) {
+  asm volatile("global_store_dword %0, %1, off sc1" ::"v"(p), "v"(v));
+}
+__device__ __forceinline__ void st_wt(double* p, d2v v) {
+  asm volatile("global_store_dwordx4 %0, %1, off sc1\n s_nop 2" ::"v"(p), "v"(v));
+}
+
+// End of a phase: wait for this workgroup's shared stores, then raise its flag.
+__device__ __forceinline__ void publish(unsigned* flag, unsigned epoch) {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Sum of N floats over the workgroup (fp32 within waves, fp64 across waves,
 // fixed order); every thread receives the result.
@@ -892,17 +825,27 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
   __syncthreads();
 }
 
-// Sum of part[c*NPART + slot[n]] over the instance's chunks, identical in every
-// workgroup (same data, same reduction tree).
+// Start of a phase: wait until every chunk of the instance has published
+// `epoch`, and sum part[c*NPART + slot[n]] over those chunks, identical in
+// every workgroup (same data, same reduction tree).  Thread t waits for
+// chunks cb+t, cb+t+NT, ... and reads their sums as soon as their flags are
+// up.  A single-chunk instance reads its own sums without waiting.  Returns
+// false (every thread) when a wait timed out.
 template <int NT, int N>
-__device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, const int (&slot)[N], double (&out)[N],
-                                         double* red) {
+__device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* part, int cb, int ce, unsigned epoch,
+                                         const int (&slot)[N], double (&out)[N], double* red, int* err) {
   double acc[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) acc[n] = 0.0;
-  for (int c = cb + (int)threadIdx.x; c < ce; c += NT)
+  int bad = 0;
+  for (int c = cb + (int)threadIdx.x; c < ce; c += NT) {
+    if (ce - cb > 1 && !spin_until(flags + c, epoch, err)) {
+      bad = 1;
+      break;
+    }
 #pragma unroll
     for (int n = 0; n < N; ++n) acc[n] += ld_shared(part + NPART * (size_t)c + slot[n]);
+  }
 #pragma unroll
   for (int n = 0; n < N; ++n)
 #pragma unroll
@@ -911,7 +854,7 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int n = 0; n < N; ++n) red[w * N + n] = acc[n];
-  __syncthreads();
+  bad = __syncthreads_or(bad);
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     double t = red[n];
@@ -920,6 +863,7 @@ __device__ __forceinline__ void inst_sum(const double* part, int cb, int ce, con
     out[n] = t;
   }
   __syncthreads();
+  return bad == 0;
 }
 
 constexpr int CG_J = 8;  // blocks per thread per tile (a tile is CG_J * NT blocks)
@@ -1059,6 +1003,10 @@ struct ResThread {
   int k0;   // first block of the thread (global block index)
   int cnt;  // blocks of the thread
   int run0;
+  int own_lo;  // operand slot of the chunk's first row (its rows take slots own_lo .. own_lo + nr - 1)
+  int nr;
+  int nhalo;   // distinct columns outside the chunk's rows
+  int R0;      // first row of the chunk
   unsigned long long mask;  // bit j: block j ends a row run
 };
 static_assert(RES_JR <= 32 && RES_JR % 4 == 0 && RES_JR + RES_JL + RES_JS <= 64, "run-end mask layout");
@@ -1069,9 +1017,63 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
                                          const float* mat_l, const uint16_t* lidx_l, f4v* u_l, f4v* seg_l,
                                          float s, float alpha, float beta, const float* __restrict__ xw,
                                          const int (&rf)[K], const int (&rn)[K], const bool (&fx)[K],
-                                         float (&q)[K][3]) {
+                                         const bool (&own)[K], const float (&p)[K][3], float (&q)[K][3]) {
   constexpr int NT = RES_NT;
-  // Streamed tier.
+  // Operands: owners write their own rows' slots (p, or the masked scaled
+  // warm start in the prologue; outside the prologue p is exactly what the
+  // record rebuild gives); the halo columns are gathered four per thread at a
+  // time (bounds the registers the records occupy), loads unconditional.
+  // ucol[i] < 0 marks a slot past the halo: it stores a zero operand into
+  // the zero slot RES_ZERO.
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (own[k]) {
+      f4v u = f4v{p[k][0], p[k][1], p[k][2], 0.f};
+      if (PROLOGUE) {
+        const v3 d = fx[k] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, rt.R0 + (int)threadIdx.x + NT * k) * s;
+        u = f4v{d.x, d.y, d.z, 0.f};
+      }
+      u_l[rt.own_lo + threadIdx.x + NT * k] = u;
+    }
+  }
+  // Opaque copies (as for the block metadata below): the per-column record
+  // offsets and operand slots are recomputed here instead of being hoisted
+  // out of the solve loop into registers.
+  int hc[RES_UC];
+  int own_lo = rt.own_lo, nrow = rt.nr;
+  asm volatile("" : "+v"(own_lo), "+v"(nrow));
+#pragma unroll
+  for (int i = 0; i < RES_UC; ++i) {
+    hc[i] = ucol[i];
+    asm volatile("" : "+v"(hc[i]));
+  }
+#pragma unroll
+  for (int i0 = 0; i0 < RES_UC; i0 += 4) {
+    if (i0 * NT >= rt.nhalo) break;
+#pragma unroll
+    for (int i = i0; i < i0 + 4 && i < RES_UC; ++i) {
+      const int h = (int)threadIdx.x + NT * i;
+      const bool live = hc[i] >= 0;
+      const uint32_t c = live ? (uint32_t)hc[i] : 0u;
+      v3 u;
+      if (PROLOGUE) {
+        u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, (int)c) * s;
+      } else {
+        const float* rec = xw + (size_t)(c * (uint32_t)XW);
+        const f4v* e = reinterpret_cast<const f4v*>(rec);
+        const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
+        const float p2 = ld_shared(rec + 8);
+        u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
+             fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
+      }
+      const int slot = !live ? RES_ZERO : h < own_lo ? h : h + nrow;
+      u_l[slot] = live ? f4v{u.x, u.y, u.z, 0.f} : f4v{0.f, 0.f, 0.f, 0.f};
+    }
+    asm volatile("" ::: "memory");
+  }
+  // Streamed tier: loaded once the operands are formed, so that these
+  // registers are free during the gathers; the loads complete while the
+  // register and LDS tiers compute.
   float B[RES_JS][9];
   uint32_t bl[RES_JS];
 #pragma unroll
@@ -1091,31 +1093,6 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
       }
       bl[i] = a.lidx[rt.k0 + j];
     }
-  }
-  // Operands of the chunk's distinct columns, four columns at a time (bounds
-  // the registers the gathered records occupy).  ucol[i] < 0 marks a slot
-  // past the chunk's columns: it loads column 0 and stores a zero operand
-  // (such slots include the zero slot RES_ZERO).
-#pragma unroll
-  for (int i0 = 0; i0 < RES_UC; i0 += 4) {
-#pragma unroll
-    for (int i = i0; i < i0 + 4 && i < RES_UC; ++i) {
-      const bool live = ucol[i] >= 0;
-      const uint32_t c = live ? (uint32_t)ucol[i] : 0u;
-      v3 u;
-      if (PROLOGUE) {
-        u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, (int)c) * s;
-      } else {
-        const float* rec = xw + (size_t)(c * (uint32_t)XW);
-        const f4v* e = reinterpret_cast<const f4v*>(rec);
-        const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
-        const float p2 = ld_shared(rec + 8);
-        u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
-             fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
-      }
-      u_l[threadIdx.x + NT * i] = live ? f4v{u.x, u.y, u.z, 0.f} : f4v{0.f, 0.f, 0.f, 0.f};
-    }
-    asm volatile("" ::: "memory");
   }
   __syncthreads();
   // Block products accumulated per row run.  Every block stores the running
@@ -1192,10 +1169,10 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
 
 __device__ __forceinline__ void st_xw(float* xw, int i, const float (&z)[3], const float (&w)[3],
                                       const float (&p)[3]) {
-  float4* e = reinterpret_cast<float4*>(xw + XW * (size_t)i);
-  e[0] = make_float4(z[0], z[1], z[2], w[0]);
-  e[1] = make_float4(w[1], w[2], p[0], p[1]);
-  xw[XW * (size_t)i + 8] = p[2];
+  float* e = xw + XW * (size_t)i;
+  st_wt(e, f4v{z[0], z[1], z[2], w[0]});
+  st_wt(e + 4, f4v{w[1], w[2], p[0], p[1]});
+  st_wt(e + 8, p[2]);
 }
 
 // Warp's absolute floor eps32^(9/4) on the residual norm (tol given, atol=0).
@@ -1238,15 +1215,12 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   float* mat_l = reinterpret_cast<float*>(seg_l + RES_SEG + 1);
   uint16_t* lidx_l = reinterpret_cast<uint16_t*>(mat_l + 9 * RES_JL * NT);
   __shared__ double red[(NT / 64) * NPART];
-  __shared__ int bflag;
-  __shared__ int xinfo[3];
   const int c = a.chunk_base + xcd_swizzle(blockIdx.x, gridDim.x);
   const int inst = a.chunk_inst[c];
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
   const int t0 = a.chunk_tiles[c], t1 = a.chunk_tiles[c + 1];
   const size_t nch = (size_t)a.nchunks_total;
-  unsigned* bar = a.bar + BAR_WORDS * inst;
   float m = 0.f;
   for (int k = 0; k < RHS_MAX_SLOTS; ++k) m = fmaxf(m, __uint_as_float(a.rhs_max[RHS_MAX_SLOTS * inst + k]));
   const float s = (m > 0.f && m < 1e-12f) ? fminf(1.f / m, 1e30f) : 1.f;  // _linear_rhs_scale kernels.py:1785-1793
@@ -1255,27 +1229,7 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   if (!RES)
     for (int r = threadIdx.x; r <= nr; r += NT) off_l[r] = a.offsets[R0 + r];
 
-  // Register on this workgroup's XCC and learn the instance's XCC layout.
-  XcdGroup xg{0, 1, 1};
   bool ok = true;
-  if (nci > 1) {
-    if (threadIdx.x == 0) {
-      unsigned x;
-      asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
-      x &= 7u;
-      __hip_atomic_fetch_add(bar + BAR_REG + x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      xinfo[0] = (int)x;
-    }
-    ok = flat_barrier(bar, nci, a.err, &bflag);
-    if (threadIdx.x == 0) {
-      int ng = 0;
-      for (int x = 0; x < 8; ++x) ng += ld_acq(bar + BAR_REG + x) != 0u;
-      xinfo[1] = (int)ld_acq(bar + BAR_REG + xinfo[0]);
-      xinfo[2] = ng;
-    }
-    __syncthreads();
-    xg = XcdGroup{xinfo[0], xinfo[1], xinfo[2]};
-  }
   __syncthreads();
 
   float x[K][3], r[K][3], z[K][3], p[K][3], qv[K][3], w[K][3];
@@ -1310,12 +1264,15 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   uint32_t L[RES_JR / 2];
   int rcol[RES_UC];
   int rf[K], rn[K];
-  ResThread rt{0, 0, 0, 0ull};
+  ResThread rt{0, 0, 0, 0, 0, 0, 0, 0ull};
   if constexpr (RES) {
     const int4 th = a.res_thr[(size_t)c * NT + threadIdx.x];
     rt.k0 = a.offsets[R0] + th.x;
     rt.cnt = th.y;
     rt.run0 = th.z;
+    rt.own_lo = th.w;
+    rt.nr = nr;
+    rt.R0 = R0;
     rt.mask = a.res_mask[(size_t)c * NT + threadIdx.x];
     const size_t k0 = (size_t)rt.k0;
 #pragma unroll
@@ -1345,11 +1302,14 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       for (int e = 0; e < 9; ++e) mat_l[(e * RES_JL + jl) * NT + threadIdx.x] = b[e];
       lidx_l[jl * NT + threadIdx.x] = a.lidx[k0 + j];
     }
+    // Halo columns only (the chunk's own rows are formed by their owners):
+    // halo number h maps to operand slot h below the own rows, h + nr above.
     const int c0 = a.tile_cptr[c], nu = a.tile_cptr[c + 1] - c0;
+    rt.nhalo = nu - nr;
 #pragma unroll
     for (int i = 0; i < RES_UC; ++i) {
-      const int u = threadIdx.x + NT * i;
-      rcol[i] = u < nu ? a.tile_cols[c0 + u] : -1;
+      const int h = threadIdx.x + NT * i;
+      rcol[i] = h < rt.nhalo ? a.tile_cols[c0 + (h < rt.own_lo ? h : h + nr)] : -1;
     }
     if (threadIdx.x == 0) ures_l[RES_ZERO] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -1366,7 +1326,7 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   auto spmv = [&](auto prologue, float alpha, float beta, const float* xw) {
     constexpr bool PRO = decltype(prologue)::value;
     if constexpr (RES)
-      res_spmv<K, PRO>(a, rt, A, L, rcol, mat_l, lidx_l, ures_l, seg_l, s, alpha, beta, xw, rf, rn, fx, qv);
+      res_spmv<K, PRO>(a, rt, A, L, rcol, mat_l, lidx_l, ures_l, seg_l, s, alpha, beta, xw, rf, rn, fx, own, p, qv);
     else
       cg_spmv<K, PRO, NT>(a, R0, nr, t0, t1, off_l, s, alpha, beta, xw, part_l, u_l, ucol, fx, qv);
   };
@@ -1404,18 +1364,18 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   {
     double wsum[3];
     wg_sum<NT, 3>(pb, wsum, red);
+    static_assert(PT_QQ == 4 && PT_RZ == 5 && PT_RR == 6 && PT_BB == 7, "partial-sum slots");
     if (threadIdx.x == 0) {
-      a.part[NPART * (size_t)c + PT_BB] = wsum[0];
-      a.part[NPART * (size_t)c + PT_RR] = wsum[1];
-      a.part[NPART * (size_t)c + PT_RZ] = wsum[2];
+      st_wt(a.part + NPART * (size_t)c + 4, d2v{0.0, wsum[2]});
+      st_wt(a.part + NPART * (size_t)c + 6, d2v{wsum[1], wsum[0]});
     }
   }
   unsigned epoch = 1;
-  ok = ok && hier_barrier(bar, xg, nci, epoch, a.err, &bflag);
+  publish(a.flags + c, epoch);
   double tot[3];
   {
     const int sl[3] = {PT_BB, PT_RR, PT_RZ};
-    inst_sum<NT, 3>(a.part, cb, ce, sl, tot, red);
+    ok = wait_sum<NT, 3>(a.flags, a.part, cb, ce, epoch, sl, tot, red, a.err);
   }
   double rr = tot[1], rz = tot[2];
   double atol = fmax(0.0, (double)a.tolerance * sqrt(tot[0]));
@@ -1429,7 +1389,7 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       double g[7];
       {
         const int sl[7] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR};
-        inst_sum<NT, 7>(a.part + NPART * nch * cur, cb, ce, sl, g, red);
+        if (!wait_sum<NT, 7>(a.flags, a.part + NPART * nch * cur, cb, ce, epoch, sl, g, red, a.err)) break;
       }
       rz = g[5];
       rr = g[6];
@@ -1459,13 +1419,33 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     spmv(std::false_type{}, alpha, beta, a.xw + XW * (size_t)a.P * cur);
     float pp[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float* xw_next = a.xw + XW * (size_t)a.P * (cur ^ 1);
+    // Inverse diagonal blocks not held in registers: all rows' loads are
+    // issued together (rows past the chunk load its first row, unused).
+    float Mi[MRES ? 1 : K][9];
+    if constexpr (!MRES) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const size_t i = own[k] ? (size_t)(R0 + threadIdx.x + NT * k) : (size_t)R0;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) Mi[MRES ? 0 : k][e] = a.minv[9 * i + e];
+      }
+    }
+    // w = M^-1 q for every row before any record store: a load waited for
+    // after a write-through store would also wait for that store (vector
+    // memory operations complete in issue order).
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (own[k]) {
+        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]};
+        const v3 ww = MRES ? minv_mul(k, R0 + threadIdx.x + NT * k, qq) : mv(Mi[MRES ? 0 : k], qq);
+        w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (own[k]) {
         const int i = R0 + threadIdx.x + NT * k;
-        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]};
-        const v3 ww = minv_mul(k, i, qq);
-        w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
+        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]}, ww = {w[k][0], w[k][1], w[k][2]};
         st_xw(xw_next, i, z[k], w[k], p[k]);
         const v3 pv = {p[k][0], p[k][1], p[k][2]}, zv = {z[k][0], z[k][1], z[k][2]},
                  rv = {r[k][0], r[k][1], r[k][2]};
@@ -1481,13 +1461,16 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     {
       double ws[7];
       wg_sum<NT, 7>(pp, ws, red);
+      static_assert(PT_PAP == 0 && PT_QZ == 1 && PT_QW == 2 && PT_QR == 3 && PT_RR == 6, "partial-sum slots");
       if (threadIdx.x == 0) {
         double* dst = a.part + NPART * (nch * (cur ^ 1) + c);
-#pragma unroll
-        for (int n = 0; n < 7; ++n) dst[n] = ws[n];
+        st_wt(dst, d2v{ws[0], ws[1]});
+        st_wt(dst + 2, d2v{ws[2], ws[3]});
+        st_wt(dst + 4, d2v{ws[4], ws[5]});
+        st_wt(dst + 6, d2v{ws[6], 0.0});
       }
     }
-    ok = hier_barrier(bar, xg, nci, ++epoch, a.err, &bflag);
+    publish(a.flags + c, ++epoch);
   }
   // restore_linear_increment kernels.py:1808-1811, update_state :115-128,
   // FixedParticles.restore(state_out) solver.py:1350-1351.
@@ -1632,7 +1615,7 @@ struct fem_system {
   int* d_row_inst = nullptr;
   // solve scratch
   float* d_xw = nullptr;
-  unsigned *d_rhs_max = nullptr, *d_bar = nullptr;
+  unsigned *d_rhs_max = nullptr, *d_cgflags = nullptr;
   double *d_norms = nullptr, *d_dstats = nullptr;  // checked-solve residual norms and running maxima
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
@@ -1902,6 +1885,8 @@ struct fem_system {
           th[0] = (int32_t)b0;
           th[1] = (int32_t)(b1 - b0);
           th[2] = run;
+          th[3] = (int32_t)(std::lower_bound(tile_cols.begin() + tile_cptr[c], tile_cols.begin() + tile_cptr[c + 1], r0) -
+                            (tile_cols.begin() + tile_cptr[c]));
           uint64_t mask = 0;
           for (int64_t b = b0; b < b1; ++b) {
             while (pt.offsets[row + 1] <= kb + b) ++row;
@@ -1951,6 +1936,7 @@ struct fem_system {
     uint64_t* n_rmask = nullptr;
     uint16_t* n_lidx = nullptr;
     double* n_part = nullptr;
+    unsigned* n_flags = nullptr;
     int rc;
     if (res && ((rc = upload(&n_rthr, res_thr.data(), res_thr.size())) ||
                 (rc = upload(&n_rmask, res_mask.data(), res_mask.size())) ||
@@ -1963,11 +1949,12 @@ struct fem_system {
         (rc = upload(&n_inst, chunk_inst.data(), chunk_inst.size())) ||
         (rc = upload(&n_ichunks, inst_chunks.data(), inst_chunks.size())) ||
         (rc = upload(&n_tiles, chunk_tiles.data(), chunk_tiles.size())) ||
-        (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 2 * (size_t)NPART * n_chunks)))
+        (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 2 * (size_t)NPART * n_chunks)) ||
+        (rc = alloc(&n_flags, (size_t)n_chunks)))
       return rc;
     for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
                     (void*)d_chunk_tiles, (void*)d_tile_rows, (void*)d_tile_cptr, (void*)d_tile_cols,
-                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row}) {
+                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row, (void*)d_cgflags}) {
       if (p) {
         (void)hipFree(p);
         allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
@@ -1986,6 +1973,7 @@ struct fem_system {
     d_res_row = n_rrow;
     cg_res = res;
     d_part = n_part;
+    d_cgflags = n_flags;
     nchunks = n_chunks;
     cg_groups = groups;
     cg_k = k_rows;
@@ -2087,7 +2075,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_row_fixed, fixed_mask.data(), fixed_mask.size())) ||
       (rc = s->upload(&s->d_fixed, d->fixed_indices, s->nfixed)) ||
       (rc = s->alloc(&s->d_xw, 2 * (size_t)XW * P)) ||
-      (rc = s->alloc(&s->d_rhs_max, (size_t)RHS_MAX_SLOTS * s->ninst)) || (rc = s->alloc(&s->d_bar, (size_t)BAR_WORDS * s->ninst)) ||
+      (rc = s->alloc(&s->d_rhs_max, (size_t)RHS_MAX_SLOTS * s->ninst)) || 
       (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)) ||
       (rc = s->alloc(&s->d_norms, 2)) || (rc = s->alloc(&s->d_dstats, 4)))
     return cleanup(rc);
@@ -2281,7 +2269,7 @@ CgArgs cg_args(const fem_system* s) {
   ca.maxiter = s->prm.maxiter;
   ca.early_exit = s->prm.early_exit;
   ca.part = s->d_part;
-  ca.bar = s->d_bar;
+  ca.flags = s->d_cgflags;
   ca.stats = s->d_stats;
   ca.err = s->d_err;
   ca.nchunks_total = s->nchunks;
@@ -2292,8 +2280,8 @@ CgArgs cg_args(const fem_system* s) {
 int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
   const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt, s->cg_res);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
-  hipLaunchKernelGGL(k_cg_reset, dim3((s->ninst + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_bar, s->ninst,
-                     s->d_err);
+  hipLaunchKernelGGL(k_cg_reset, dim3((s->nchunks + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_cgflags,
+                     s->nchunks, s->d_err);
   HIP_TRY(hipGetLastError());
   for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
     ca.chunk_base = s->cg_groups[g];

@@ -82,6 +82,12 @@ def test_batched_bit_identical_to_looped(batch, variant):
         np.testing.assert_array_equal(q_b[lo:hi], np_(ok.particle_q), err_msg=f"instance {k}")
         assert sk.solve_stats()["iterations"][0] == it_b[k]
     assert s.launch_config()["cg_workgroups"] == looped_chunks
+    if variant == "resident":  # both kernels agree on every instance's iteration count
+        t = solver_for(model, instance_offsets=offs, **kw)
+        t.set_cg_variant("streamed")
+        t.set_launch(0, 0, len(scenes))
+        rollout(t, model, steps, dt)
+        np.testing.assert_array_equal(np.asarray(t.solve_stats()["iterations"]), np.asarray(it_b))
 
 
 def test_batched_launch_groups(batch):

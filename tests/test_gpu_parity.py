@@ -154,6 +154,54 @@ def test_step_matches_oracle_tight_tolerance(kind, variant):
         st0, st1 = st1, st0
 
 
+@pytest.mark.parametrize("variant", ["resident", "streamed"])
+def test_multichunk_step_matches_oracle(variant):
+    """Tet4 drop mesh (1331 nodes) split into 2, 3 and 4 CG chunks: a converged
+    step (tol 1e-7) matches the FP64 oracle for every split, and both CG
+    kernels take the same iterations at the production tolerance.  Covers
+    chunks whose blocks fill the register, LDS and streamed tiers and chunks
+    with padded register slots (resident kernel)."""
+    print(f"case multichunk {variant}", flush=True)
+    sc = meshes.tet_drop_scene()
+    model = build_tet_model(sc, device=DEV)
+    kw = dict(dt=1 / 600, linear_solver_maxiter=400, linear_solver_tolerance=1e-7, ground_plane=(0, 0, 1, 0),
+              ground_ke=100.0, ground_kd=0.1, ground_kf=1.0, ground_mu=0.5)
+    ref = oracle_for(model, **kw)
+    st0 = model.state()
+    rng = np.random.default_rng(21)
+    st0.particle_qd.copy_(torch.tensor(rng.normal(0, 0.05, st0.particle_qd.shape), dtype=torch.float32))
+    q, v = np_(st0.particle_q), np_(st0.particle_qd)
+    ref.refresh_tangent(q)
+    out = ref.step(q, v)
+    iters = {}
+    for wg in (2, 3, 4):
+        s = solver_for(model, **kw)
+        s.set_launch(0, 0, wg)
+        s.set_cg_variant(variant)
+        cfg = s.launch_config()
+        assert (cfg["cg_variant"], cfg["cg_workgroups"]) == (variant, wg), cfg
+        st1 = model.state()
+        s.refresh_tangent(st0.particle_q)
+        s.step(st0, st1, None, None, 1 / 600)
+        err = rel(np_(s.dv), ref.dv)
+        print(f"  {wg} chunks: dv rel err {err:.2e}, iterations {s.solve_stats()['iterations'][0]}", flush=True)
+        assert err < 2e-4, (wg, err)
+        assert rel(np_(st1.particle_q), out["q"]) < 1e-6
+        s.linear_solver_tolerance = 1e-2
+        s.velocity_increment.zero_()
+        s.step(st0, st1, None, None, 1 / 600)
+        iters[wg] = s.solve_stats()["iterations"][0]
+    other = "streamed" if variant == "resident" else "resident"
+    s = solver_for(model, **kw)
+    s.set_launch(0, 0, 3)
+    s.set_cg_variant(other)
+    s.linear_solver_tolerance = 1e-2
+    st1 = model.state()
+    s.refresh_tangent(st0.particle_q)
+    s.step(st0, st1, None, None, 1 / 600)
+    assert abs(s.solve_stats()["iterations"][0] - iters[3]) <= 1, (iters, s.solve_stats()["iterations"])
+
+
 def test_fixed_particles_reduced_dense_solve():
     """Reference test_fixed_particles.py:40-83 on the HIP path."""
     print("case fixed", flush=True)
@@ -330,8 +378,8 @@ def test_cg_launch_shapes_agree():
     base = None
     for variant, wg in (("streamed", 128), ("streamed", 256), ("streamed", 416), ("streamed", 512),
                         ("resident", 256)):
-        s.set_cg_variant(variant)
         s.set_launch(0, 0, wg)
+        s.set_cg_variant(variant)
         cfg = s.launch_config()
         assert cfg["cg_variant"] == variant, cfg
         _large_step(s, model, dt, 1e-6, 400, True)
