@@ -111,8 +111,10 @@ int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float*
 int fem_set_params(fem_system* sys, const fem_params* params);
 
 /* Launch shapes (0 keeps the current value): lanes per block row of the
- * force/RHS kernel and rows per workgroup of the assembly (powers of two in
- * [4, 64]), and the persistent CG workgroup count (0 = automatic). */
+ * force/RHS kernel and the maximum rows per workgroup of the assembly (powers
+ * of two in [4, 64]; consecutive rows are packed up to that count while their
+ * element incidences fit one pass of the workgroup), and the persistent CG
+ * workgroup count (0 = automatic). */
 int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups);
 /* Current launch shapes; cg_launch_groups > 1 when the instances' CG
  * workgroups do not all fit on the device at once and the solve runs as

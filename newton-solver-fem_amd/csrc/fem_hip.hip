@@ -286,7 +286,10 @@ __device__ __forceinline__ float group_sum(float v) {
 
 // ---------------------------------------------------------------------------
 // k_assemble: A = M + h^2 K at positions x, and M^-1 = inv(diag blocks).
-// Atomic-free and deterministic.  A workgroup owns RB consecutive rows.
+// Atomic-free and deterministic.  A workgroup owns the consecutive rows
+// groups[blk] .. groups[blk + 1] - 1 (at most RB rows, packed on the host so
+// that their incidences fit one pass of the workgroup's threads and the
+// staging fits ASM_LDS_BUDGET bytes).
 // Phase 1: every incidence (element, corner) of those rows computes its
 // element-tangent row (4 blocks for a Tet4, 8 for a Hex8) into LDS staging
 // with plain stores.  Phase 2: one thread per BSR slot of the rows sums, in
@@ -297,17 +300,23 @@ __device__ __forceinline__ float group_sum(float v) {
 // Staging strides (floats) per incidence: odd so that consecutive incidences
 // fall in different LDS banks.
 constexpr int ST_TET = 37, ST_HEX = 73;
+// Dynamic LDS of one assembly workgroup: four fit on a CU beside the static
+// row arrays.
+constexpr size_t ASM_LDS_BUDGET = 38 * 1024;
 
-template <int RB>
-__global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const float* __restrict__ x, float h2,
+// EL_TET kernels carry no Hex8 code (and so none of its register pressure).
+enum { EL_TET = 1, EL_ANY = 3 };
+
+template <int RB, int EL>
+__global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __restrict__ groups,
+                                                    const float* __restrict__ x, float h2,
                                                     float* __restrict__ vals, float* __restrict__ minv) {
   extern __shared__ float lds[];
   __shared__ int roff[RB + 1], rtb[RB], rhb[RB], rdiag[RB];
   __shared__ float rmass[RB];
   const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int r0 = blk * RB;
-  if (r0 >= a.P) return;  // uniform over the workgroup
-  const int r1 = min(r0 + RB, a.P);
+  const int r0 = groups[blk], r1 = groups[blk + 1];
+  if (r0 >= r1) return;  // uniform over the workgroup (padding groups are empty)
   const int nr = r1 - r0;
   const int tb = a.tinc_ptr[r0], nt = a.tinc_ptr[r1] - tb;
   const int hb = a.hinc_ptr[r0], nh = a.hinc_ptr[r1] - hb;
@@ -341,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const float* __re
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[9 * b + c] = K[b][c];
   }
-  for (int t = threadIdx.x; t < nh; t += BLOCK) {
+  if (EL & 2) for (int t = threadIdx.x; t < nh; t += BLOCK) {
     const int code = a.hinc[hb + t];
     float K[8][9];
     hex_tangent_row(a, x, code >> 3, code & 7, h2, K);
@@ -396,7 +405,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const float* __re
 // ---------------------------------------------------------------------------
 enum { MODE_EVAL = 0, MODE_STEP = 1 };
 
-template <int G>
+template <int G, int EL>
 __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restrict__ x, const float* __restrict__ v,
                                                const float* __restrict__ act, const float* __restrict__ ext,
                                                const float* __restrict__ vals, float* __restrict__ out,
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
         if (a.use_act && act) ft = ft + tet_activation_force(x0, x1, x2, x3, lam, act[e], la);
       }
     }
-    if (terms & FEM_TERM_HEX) {
+    if ((EL & 2) && (terms & FEM_TERM_HEX)) {
       const int h1 = a.hinc_ptr[i + 1];
       for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
         const int code = a.hinc[t];
@@ -1532,17 +1541,18 @@ int padded_blocks(int rows, int rows_per_block) {
 
 template <int RB>
 struct LaunchAssemble {
-  static int run(const RowArgs& ra, const float* x, float h2, float* vals, float* minv, size_t lds, hipStream_t st) {
-    // The static roff[] array shares the 160 KiB with the dynamic staging.
-    const size_t cap = 160 * 1024 - 1024;
-    if (lds > cap) return fail(FEM_ERR_UNSUPPORTED, "assembly staging exceeds the LDS; use fewer rows per workgroup");
-    static size_t attr = 0;
-    if (lds > 48 * 1024 && lds > attr) {
-      HIP_TRY(hipFuncSetAttribute((const void*)k_assemble<RB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
-      attr = cap;
+  static int run(bool hex, const RowArgs& ra, const int* groups, int ngroups, const float* x, float h2, float* vals,
+                 float* minv, size_t lds, hipStream_t st) {
+    // The static row arrays share the 160 KiB with the dynamic staging.
+    const size_t cap = 160 * 1024 - 2048;
+    if (lds > cap) return fail(FEM_ERR_UNSUPPORTED, "assembly staging of one row exceeds the LDS");
+    auto kern = hex ? k_assemble<RB, EL_ANY> : k_assemble<RB, EL_TET>;
+    static size_t attr[2] = {0, 0};
+    if (lds > 48 * 1024 && lds > attr[hex]) {
+      HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
+      attr[hex] = cap;
     }
-    int nb = padded_blocks(ra.P, RB);
-    hipLaunchKernelGGL(k_assemble<RB>, dim3(nb), dim3(BLOCK), lds, st, ra, x, h2, vals, minv);
+    hipLaunchKernelGGL(kern, dim3(ngroups), dim3(BLOCK), lds, st, ra, groups, x, h2, vals, minv);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -1550,10 +1560,11 @@ struct LaunchAssemble {
 
 template <int G>
 struct LaunchRhs {
-  static int run(const RowArgs& ra, const float* x, const float* v, const float* act, const float* ext,
+  static int run(bool hex, const RowArgs& ra, const float* x, const float* v, const float* act, const float* ext,
                  const float* vals, float* out, unsigned* rmax, int terms, int mode, float h, hipStream_t st) {
     int nb = padded_blocks(ra.P, BLOCK / G);
-    hipLaunchKernelGGL(k_rhs<G>, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, ext, vals, out, rmax, terms, mode, h);
+    auto kern = hex ? k_rhs<G, EL_ANY> : k_rhs<G, EL_TET>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, ext, vals, out, rmax, terms, mode, h);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -1635,6 +1646,8 @@ struct fem_system {
   fem_params prm{};
   int elem_g = 4, asm_rb = 16, cg_wg_request = 0;
   size_t asm_lds = 0;
+  int* d_asm_groups = nullptr;  // [asm_ngroups + 1] first row of each assembly workgroup
+  int asm_ngroups = 0;
   int max_resident = 0;
   std::vector<void*> allocs;
 
@@ -1696,19 +1709,49 @@ struct fem_system {
     return a;
   }
 
-  // LDS staging bytes of the assembly for `rb` rows per workgroup.
-  size_t assemble_lds(int rb) const {
-    size_t worst = 0;
-    for (int r0 = 0; r0 < P; r0 += rb) {
-      const int r1 = std::min(r0 + rb, P);
-      const size_t nt = (size_t)(pt.tinc_ptr[r1] - pt.tinc_ptr[r0]);
-      const size_t nh = (size_t)(pt.hinc_ptr[r1] - pt.hinc_ptr[r0]);
-      const size_t ns = (size_t)(pt.offsets[r1] - pt.offsets[r0]);
-      const size_t nc = (size_t)(pt.slot_ptr[pt.offsets[r1]] - pt.slot_ptr[pt.offsets[r0]]);
-      worst = std::max(worst, ((size_t)ST_TET * nt + (size_t)ST_HEX * nh + ns + 1) * sizeof(float) +
-                                  nc * sizeof(uint16_t));
+  // Dynamic LDS bytes of the assembly of rows r0 .. r1 - 1.
+  size_t assemble_lds(int r0, int r1) const {
+    const size_t nt = (size_t)(pt.tinc_ptr[r1] - pt.tinc_ptr[r0]);
+    const size_t nh = (size_t)(pt.hinc_ptr[r1] - pt.hinc_ptr[r0]);
+    const size_t ns = (size_t)(pt.offsets[r1] - pt.offsets[r0]);
+    const size_t nc = (size_t)(pt.slot_ptr[pt.offsets[r1]] - pt.slot_ptr[pt.offsets[r0]]);
+    return ((size_t)ST_TET * nt + (size_t)ST_HEX * nh + ns + 1) * sizeof(float) + nc * sizeof(uint16_t);
+  }
+
+  // Row groups of the assembly: consecutive rows packed greedily, at most rb
+  // rows, at most BLOCK element incidences (one pass of phase 1) and at most
+  // ASM_LDS_BUDGET bytes of staging per group; a row that alone exceeds the
+  // limits forms its own group.  The group count is padded to a multiple of 8
+  // with empty groups (XCD swizzle).  Replaces the previous table on success.
+  int plan_assemble(int rb) {
+    std::vector<int32_t> g{0};
+    size_t worst = 16;
+    int r0 = 0;
+    while (r0 < P) {
+      int r1 = r0 + 1;
+      while (r1 < P && r1 - r0 < rb) {
+        const int inc = (pt.tinc_ptr[r1 + 1] - pt.tinc_ptr[r0]) + (pt.hinc_ptr[r1 + 1] - pt.hinc_ptr[r0]);
+        if (inc > BLOCK || assemble_lds(r0, r1 + 1) > ASM_LDS_BUDGET) break;
+        ++r1;
+      }
+      worst = std::max(worst, assemble_lds(r0, r1));
+      g.push_back(r1);
+      r0 = r1;
     }
-    return std::max<size_t>(worst, 16);
+    int n = (int)g.size() - 1;
+    const int padded = n > 8 ? (n + 7) & ~7 : std::max(n, 1);
+    while ((int)g.size() < padded + 1) g.push_back(P);
+    int* dg = nullptr;
+    if (int rc = upload(&dg, g.data(), g.size())) return rc;
+    if (d_asm_groups) {
+      (void)hipFree(d_asm_groups);
+      allocs.erase(std::remove(allocs.begin(), allocs.end(), (void*)d_asm_groups), allocs.end());
+    }
+    d_asm_groups = dg;
+    asm_ngroups = padded;
+    asm_rb = rb;
+    asm_lds = worst;
+    return FEM_OK;
   }
 
   // Row partition of the persistent CG: per instance, contiguous chunks,
@@ -2083,15 +2126,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
   double avg_nnz = P ? (double)s->pt.offsets[P] / P : 1.0;
   (void)avg_nnz;
   s->elem_g = std::min(8, pick_lanes(avg_inc / 4.0));
-  // Rows per assembly workgroup: the most that keep the staging <= 40 KiB.
-  s->asm_rb = 4;
-  for (int rb : {32, 16, 8}) {
-    if (s->assemble_lds(rb) <= 56 * 1024) {
-      s->asm_rb = rb;
-      break;
-    }
-  }
-  s->asm_lds = s->assemble_lds(s->asm_rb);
+  // Assembly row groups: the incidence and LDS budgets bind before 64 rows.
+  if (P > 0 && (rc = s->plan_assemble(64))) return cleanup(rc);
   s->prm.min_stretch = 0.05f;
   s->prm.max_stretch = 20.f;
   s->prm.tolerance = 1e-2f;
@@ -2143,8 +2179,6 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int
   auto ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
   if (elem_lanes && !ok(elem_lanes)) return fail(FEM_ERR_ARG, "elem_lanes must be a power of two in [4, 64]");
   if (assemble_rows && !ok(assemble_rows)) return fail(FEM_ERR_ARG, "assemble_rows must be a power of two in [4, 64]");
-  if (assemble_rows && s->assemble_lds(assemble_rows) > 159 * 1024)
-    return fail(FEM_ERR_UNSUPPORTED, "assembly staging for that many rows exceeds 160 KiB");
   if (cg_workgroups < 0) return fail(FEM_ERR_ARG, "cg_workgroups must be >= 0");
   const int old_request = s->cg_wg_request;
   s->cg_wg_request = cg_workgroups;
@@ -2156,9 +2190,10 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int
     }
   }
   if (elem_lanes) s->elem_g = elem_lanes;
-  if (assemble_rows) {
+  if (assemble_rows && s->P > 0) {
+    if (int rc = s->plan_assemble(assemble_rows)) return rc;
+  } else if (assemble_rows) {
     s->asm_rb = assemble_rows;
-    s->asm_lds = s->assemble_lds(assemble_rows);
   }
   return FEM_OK;
 }
@@ -2206,8 +2241,8 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchAssemble>(s->asm_rb, ra, q, dt * dt, s->b_vals, s->b_minv, s->asm_lds,
-                                    (hipStream_t)stream);
+  return dispatch_g<LaunchAssemble>(s->asm_rb, s->H > 0, ra, s->d_asm_groups, s->asm_ngroups, q, dt * dt, s->b_vals, s->b_minv,
+                                    s->asm_lds, (hipStream_t)stream);
 }
 
 int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* qd, const float* act, float* out,
@@ -2215,7 +2250,7 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
   if (!s || !q || !qd || !out) return fail(FEM_ERR_ARG, "null argument");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchRhs>(s->elem_g, ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
+  return dispatch_g<LaunchRhs>(s->elem_g, s->H > 0, ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
 
@@ -2234,7 +2269,7 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
   }
   RowArgs ra = s->row_args();
   const int terms = FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
-  return dispatch_g<LaunchRhs>(s->elem_g, ra, (const float*)q_in, (const float*)qd_in, act, ext,
+  return dispatch_g<LaunchRhs>(s->elem_g, s->H > 0, ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }
 

@@ -328,7 +328,7 @@ class SolverFEM:
         out["cg_variant"] = {1: "streamed", 2: "resident"}[v[5].value]
         return out
 
-    def autotune(self, state: State, dt: float | None = None, *, elem_lanes=(4, 8, 16), assemble_rows=(8, 16, 32),
+    def autotune(self, state: State, dt: float | None = None, *, elem_lanes=(4, 8, 16), assemble_rows=(16, 32, 64),
                  cg_workgroups=None, repeats: int = 3, control: Control | None = None) -> dict:
         """Pick the fastest launch shapes for this model by timing real substeps.
 
