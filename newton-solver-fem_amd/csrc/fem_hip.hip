@@ -528,7 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
 // ---------------------------------------------------------------------------
 // Per node i (G lanes over its element incidences): xbar_i = h dT/dx^T z and
 // vbar_i = h dT/dv^T z - (A z)_i + (sum_k A_ik)^T z_i.  Written, not added.
-template <int G>
+template <int G, int EL>
 __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __restrict__ x,
                                                     const float* __restrict__ v, const float* __restrict__ act,
                                                     const float* __restrict__ z, const float* __restrict__ vals,
@@ -556,13 +556,15 @@ __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __re
       xb = xb + sel4(la, x0, o.xe[0], o.xe[1], o.xe[2]);
       vb = vb + sel4(la, v0, o.ve[0], o.ve[1], o.ve[2]);
     }
-    const int h1 = a.hinc_ptr[i + 1];
-    for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
-      const int code = a.hinc[t];
-      v3 hx, hv;
-      hex_node_vjp(a, x, z, h, code >> 3, code & 7, hx, hv);
-      xb = xb + hx;
-      vb = vb + hv;
+    if (EL & 2) {
+      const int h1 = a.hinc_ptr[i + 1];
+      for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
+        const int code = a.hinc[t];
+        v3 hx, hv;
+        hex_node_vjp(a, x, z, h, code >> 3, code & 7, hx, hv);
+        xb = xb + hx;
+        vb = vb + hv;
+      }
     }
     const int k1 = a.offsets[i + 1];
     for (int k = a.offsets[i] + lane; k < k1; k += G) {
@@ -1572,10 +1574,11 @@ struct LaunchRhs {
 
 template <int G>
 struct LaunchVjpRows {
-  static int run(const RowArgs& ra, const float* x, const float* v, const float* act, const float* z,
+  static int run(bool hex, const RowArgs& ra, const float* x, const float* v, const float* act, const float* z,
                  const float* vals, float h, float* xbar, float* vbar, hipStream_t st) {
     int nb = padded_blocks(ra.P, BLOCK / G);
-    hipLaunchKernelGGL(k_vjp_rows<G>, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, z, vals, h, xbar, vbar);
+    auto kern = hex ? k_vjp_rows<G, EL_ANY> : k_vjp_rows<G, EL_TET>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, st, ra, x, v, act, z, vals, h, xbar, vbar);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -2387,7 +2390,7 @@ int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, co
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   RowArgs ra = s->row_args();
-  int rc = dispatch_g<LaunchVjpRows>(s->elem_g, ra, q, qd, act, z, (const float*)s->b_vals, dt, q_bar, qd_bar, st);
+  int rc = dispatch_g<LaunchVjpRows>(s->elem_g, s->H > 0, ra, q, qd, act, z, (const float*)s->b_vals, dt, q_bar, qd_bar, st);
   if (rc) return rc;
   if (s->T > 0 && (act_bar || mat_bar)) {
     hipLaunchKernelGGL(k_vjp_tets, dim3((s->T + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ra, s->T, q, qd, act, z, dv,
