@@ -40,7 +40,9 @@ enum {
   FEM_TERM_HEX = 2,
   FEM_TERM_GROUND = 4,
   FEM_TERM_GRAVITY = 8,
-  FEM_TERM_DAMPING = 16
+  FEM_TERM_DAMPING = 16,
+  FEM_TERM_SPRING = 32,
+  FEM_TERM_TRI = 64
 };
 
 /* Device-side solve status bits (fem_solve_stats.error_flags). */
@@ -49,10 +51,13 @@ enum { FEM_STATUS_BARRIER_TIMEOUT = 1 };
 /* Model arrays, all HOST pointers, copied at creation.  Mirrors the Model /
  * LinearFEMData fields SolverFEM.__init__ consumes (solver.py:212-341,
  * builder.py:12-20): particle q/mass/inv_mass/radius/flags, tet
- * indices/poses/materials, hex indices/inv_J0/det_J0_w/materials, and the
- * sorted unique fixed_particle_indices (solver.py:244-249).  Independent
- * meshes may be packed back to back: instance_offsets[k]..[k+1] are the
- * particles of instance k (instance_count 0 or 1 means one instance). */
+ * indices/poses/materials, hex indices/inv_J0/det_J0_w/materials, the
+ * sorted unique fixed_particle_indices (solver.py:244-249), linear springs
+ * (Model.spring_indices/rest_length/stiffness/damping, eval_springs
+ * kernels.py:176-219) and triangle membranes (Model.tri_indices/poses/
+ * materials, eval_triangles kernels.py:876-970).  Independent meshes may be
+ * packed back to back: instance_offsets[k]..[k+1] are the particles of
+ * instance k (instance_count 0 or 1 means one instance). */
 typedef struct fem_model_desc {
   int32_t particle_count;
   const float* particle_q;        /* [P][3] construction positions (anchors) */
@@ -73,6 +78,15 @@ typedef struct fem_model_desc {
   const int32_t* fixed_indices;   /* [fixed_count], sorted, unique */
   int32_t instance_count;
   const int32_t* instance_offsets; /* [instance_count+1] or NULL */
+  int32_t spring_count;
+  const int32_t* spring_indices;  /* [S][2]; springs with a -1 endpoint are ignored */
+  const float* spring_rest_length; /* [S] */
+  const float* spring_stiffness;  /* [S] */
+  const float* spring_damping;    /* [S] */
+  int32_t tri_count;
+  const int32_t* tri_indices;     /* [R][3] */
+  const float* tri_poses;         /* [R][2][2] inverse rest edge matrix, row-major */
+  const float* tri_materials;     /* [R][5] k_mu, k_lambda, k_damp, k_drag, k_lift */
 } fem_model_desc;
 
 /* Physical and solver parameters (SolverFEM.__init__ arguments,
@@ -206,7 +220,8 @@ int fem_solve(fem_system* sys, const float* b_dev, float* x_dev, float tolerance
  *   mat_bar[e] = d/d(mu, lambda, k_damp)_e of  h z^T terms  - z^T (dA/d(mu, lambda))(dv + qd),
  *     with A assembled at q_asm (written if not NULL; needs dv and q_asm).
  * The reference has no material gradients (tet_materials must not require
- * gradients, differentiation.py:244-262); mat_bar extends it. */
+ * gradients, differentiation.py:244-262); mat_bar extends it.  Models with
+ * springs or triangle membranes return FEM_ERR_UNSUPPORTED. */
 int fem_vjp(fem_system* sys, const float* q_dev, const float* qd_dev, const float* act_dev, const float* z_dev,
             const float* dv_dev, const float* q_asm_dev, float dt, float* q_bar_dev, float* qd_bar_dev,
             float* act_bar_dev, float* mat_bar_dev, void* stream);

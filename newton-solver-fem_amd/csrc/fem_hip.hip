@@ -87,6 +87,20 @@ struct RowArgs {
   const int* flags;
   const unsigned char* row_fixed;
   const int* row_inst;
+  // springs and triangle membranes (NULL / empty when the model has none)
+  const int* sinc_ptr;
+  const int* sinc;       // (spring << 1) | end
+  const int* sp_idx;     // [S][2]
+  const float* sp_rest;
+  const float* sp_ke;
+  const float* sp_kd;
+  const int* rinc_ptr;
+  const int* rinc;       // (tri << 2) | corner
+  const int* tri_idx;    // [R][3]
+  const float* tri_pose; // [R][4]
+  const float* tri_mat;  // [R][5]
+  const float2* slot_spring;  // per BSR slot (sum ke, sum kd) of its springs (row sums on the diagonal)
+  const float* row_tri;       // per row sum of (k_mu + k_lambda) * rest area of its triangles
   float mn, mx;
   int has_ground;
   float gn[4];
@@ -304,12 +318,13 @@ constexpr int ST_TET = 37, ST_HEX = 73;
 // row arrays.
 constexpr size_t ASM_LDS_BUDGET = 38 * 1024;
 
-// EL_TET kernels carry no Hex8 code (and so none of its register pressure).
-enum { EL_TET = 1, EL_ANY = 3 };
+// EL_TET kernels carry no Hex8, spring or triangle code (and so none of its
+// register pressure); EL_ANY kernels handle every element kind.
+enum { EL_TET = 1, EL_ANY = 7 };
 
 template <int RB, int EL>
 __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __restrict__ groups,
-                                                    const float* __restrict__ x, float h2,
+                                                    const float* __restrict__ x, float h, float h2,
                                                     float* __restrict__ vals, float* __restrict__ minv) {
   extern __shared__ float lds[];
   __shared__ int roff[RB + 1], rtb[RB], rhb[RB], rdiag[RB];
@@ -369,6 +384,22 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
     const float m = diag ? rmass[lr] : 0.f;
 #pragma unroll
     for (int c = 0; c < 9; ++c) acc[c] = (c == 0 || c == 4 || c == 8) ? m : 0.f;
+    if (EL & 4) {
+      // Constant blocks in the reference's triplet order after the mass:
+      // springs (diagonal m + sum(h kd + h^2 ke), off-diagonal -(h kd + h^2 ke),
+      // kernels.py:294-366), then the lumped triangle diagonal
+      // h^2 (k_mu + k_lambda) A / 3 (kernels.py:972-1030).
+      float cst = 0.f;
+      if (a.slot_spring) {
+        const float2 sk = a.slot_spring[kb + kk];
+        const float c = h * sk.y + h2 * sk.x;
+        cst = diag ? c : -c;
+      }
+      if (diag && a.row_tri) cst += h2 * a.row_tri[r0 + lr] / 3.f;
+      acc[0] += cst;
+      acc[4] += cst;
+      acc[8] += cst;
+    }
     const int tbase = rtb[lr], hbase = rhb[lr];
     const int j1 = sptr[kk + 1];
     for (int j = sptr[kk]; j < j1; ++j) {
@@ -416,7 +447,33 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
   const int i = blk * NG + grp;
   const bool active = i < a.P;
   v3 ft = {0.f, 0.f, 0.f}, fh = {0.f, 0.f, 0.f}, corr = {0.f, 0.f, 0.f};
+  v3 fsp = {0.f, 0.f, 0.f}, ftri = {0.f, 0.f, 0.f}, srest = {0.f, 0.f, 0.f};
   if (active) {
+    if ((EL & 4) && a.sinc_ptr) {
+      // eval_springs kernels.py:176-219 and restore_spring_damping_predictor
+      // implicit_predictor.py:26-41 (h kd (v_node - v_other) per spring).
+      const int s1 = a.sinc_ptr[i + 1];
+      for (int t = a.sinc_ptr[i] + lane; t < s1; t += G) {
+        const int code = a.sinc[t];
+        const int sp = code >> 1, end = code & 1;
+        const int p = a.sp_idx[2 * sp], q = a.sp_idx[2 * sp + 1];
+        const v3 vp = ld3(v, p), vq = ld3(v, q);
+        const float kd = a.sp_kd[sp];
+        if (terms & FEM_TERM_SPRING)
+          fsp = fsp + spring_end_force(ld3(x, p), ld3(x, q), vp, vq, a.sp_rest[sp], a.sp_ke[sp], kd, end);
+        srest = srest + (end == 0 ? vp - vq : vq - vp) * (h * kd);
+      }
+    }
+    if ((EL & 4) && a.rinc_ptr && (terms & FEM_TERM_TRI)) {
+      const int r1 = a.rinc_ptr[i + 1];
+      for (int t = a.rinc_ptr[i] + lane; t < r1; t += G) {
+        const int code = a.rinc[t];
+        const int e = code >> 2, c = code & 3;
+        const int n0 = a.tri_idx[3 * e], n1 = a.tri_idx[3 * e + 1], n2 = a.tri_idx[3 * e + 2];
+        ftri = ftri + tri_corner_force(ld3(x, n0), ld3(x, n1), ld3(x, n2), ld3(v, n0), ld3(v, n1), ld3(v, n2),
+                                       a.tri_pose + 4 * (size_t)e, a.tri_mat + 5 * (size_t)e, c);
+      }
+    }
     if (terms & FEM_TERM_TET) {
       const int t1 = a.tinc_ptr[i + 1];
       for (int t = a.tinc_ptr[i] + lane; t < t1; t += G) {
@@ -451,6 +508,11 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
   }
   ft = {group_sum<G>(ft.x), group_sum<G>(ft.y), group_sum<G>(ft.z)};
   fh = {group_sum<G>(fh.x), group_sum<G>(fh.y), group_sum<G>(fh.z)};
+  if (EL & 4) {
+    fsp = {group_sum<G>(fsp.x), group_sum<G>(fsp.y), group_sum<G>(fsp.z)};
+    ftri = {group_sum<G>(ftri.x), group_sum<G>(ftri.y), group_sum<G>(ftri.z)};
+    srest = {group_sum<G>(srest.x), group_sum<G>(srest.y), group_sum<G>(srest.z)};
+  }
   if (mode == MODE_STEP) corr = {group_sum<G>(corr.x), group_sum<G>(corr.y), group_sum<G>(corr.z)};
   float mval = 0.f;
   if (active && lane == 0) {
@@ -483,20 +545,25 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     if ((terms & FEM_TERM_DAMPING) && on && a.lin_damp > 0.f) fdm = vi * (-a.lin_damp * a.mass[i]);
     v3 f;
     if (mode == MODE_STEP) {
-      // _accumulate_forces order solver.py:570-589, then h*ext (:993-999),
-      // then subtract_elastic_predictor implicit_predictor.py:9-23.
+      // _accumulate_forces order solver.py:570-589 (spring, tet, hex,
+      // triangle, ground, gravity, damping), then h*ext (:993-999), then
+      // subtract_elastic_predictor and restore_spring_damping_predictor
+      // implicit_predictor.py:9-41.
       f = {0.f, 0.f, 0.f};
+      if (EL & 4) f = f + fsp * h;
       f = f + ft * h;
       f = f + fh * h;
+      if (EL & 4) f = f + ftri * h;
       f = f + fg * h;
       f = f + fgr * h;
       f = f + fdm * h;
       if (ext) f = f + ld3(ext, i) * h;
       f = f - corr;
+      if (EL & 4) f = f + srest;
       if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
       mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
     } else {
-      f = ft + fh + fg + fgr + fdm;
+      f = fsp + ft + fh + ftri + fg + fgr + fdm;
     }
     st3(out, i, f);
   }
@@ -1543,7 +1610,7 @@ int padded_blocks(int rows, int rows_per_block) {
 
 template <int RB>
 struct LaunchAssemble {
-  static int run(bool hex, const RowArgs& ra, const int* groups, int ngroups, const float* x, float h2, float* vals,
+  static int run(bool hex, const RowArgs& ra, const int* groups, int ngroups, const float* x, float h, float* vals,
                  float* minv, size_t lds, hipStream_t st) {
     // The static row arrays share the 160 KiB with the dynamic staging.
     const size_t cap = 160 * 1024 - 2048;
@@ -1554,7 +1621,7 @@ struct LaunchAssemble {
       HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
       attr[hex] = cap;
     }
-    hipLaunchKernelGGL(kern, dim3(ngroups), dim3(BLOCK), lds, st, ra, groups, x, h2, vals, minv);
+    hipLaunchKernelGGL(kern, dim3(ngroups), dim3(BLOCK), lds, st, ra, groups, x, h, h * h, vals, minv);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -1610,7 +1677,7 @@ const void* cg_kernel_ptr(int k, int nt, bool res) {
 // System object.
 // ---------------------------------------------------------------------------
 struct fem_system {
-  int P = 0, T = 0, H = 0, nfixed = 0, ninst = 1;
+  int P = 0, T = 0, H = 0, S = 0, R = 0, nfixed = 0, ninst = 1;
   Pattern pt;
   std::vector<int32_t> inst_off;
   // device copies (library-owned)
@@ -1627,6 +1694,13 @@ struct fem_system {
   unsigned char* d_row_fixed = nullptr;
   int* d_fixed = nullptr;
   int* d_row_inst = nullptr;
+  int *d_sinc_ptr = nullptr, *d_sinc = nullptr, *d_sp_idx = nullptr, *d_rinc_ptr = nullptr, *d_rinc = nullptr,
+      *d_tri_idx = nullptr;
+  float *d_sp_rest = nullptr, *d_sp_ke = nullptr, *d_sp_kd = nullptr, *d_tri_pose = nullptr, *d_tri_mat = nullptr,
+        *d_row_tri = nullptr;
+  float2* d_slot_spring = nullptr;
+  // Kernels with Hex8, spring or triangle code (EL_ANY) are needed.
+  bool full() const { return H > 0 || S > 0 || R > 0; }
   // solve scratch
   float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_cgflags = nullptr;
@@ -1696,6 +1770,19 @@ struct fem_system {
     a.flags = d_flags;
     a.row_fixed = d_row_fixed;
     a.row_inst = ninst > 1 ? d_row_inst : nullptr;
+    a.sinc_ptr = S > 0 ? d_sinc_ptr : nullptr;
+    a.sinc = d_sinc;
+    a.sp_idx = d_sp_idx;
+    a.sp_rest = d_sp_rest;
+    a.sp_ke = d_sp_ke;
+    a.sp_kd = d_sp_kd;
+    a.rinc_ptr = R > 0 ? d_rinc_ptr : nullptr;
+    a.rinc = d_rinc;
+    a.tri_idx = d_tri_idx;
+    a.tri_pose = d_tri_pose;
+    a.tri_mat = d_tri_mat;
+    a.slot_spring = S > 0 ? d_slot_spring : nullptr;
+    a.row_tri = R > 0 ? d_row_tri : nullptr;
     a.mn = prm.min_stretch;
     a.mx = prm.max_stretch;
     a.has_ground = prm.has_ground;
@@ -2048,7 +2135,8 @@ const char* fem_last_error(void) { return g_error.c_str(); }
 int fem_create(const fem_model_desc* d, fem_system** out) {
   if (!d || !out) return fail(FEM_ERR_ARG, "null argument");
   *out = nullptr;
-  if (d->particle_count < 0 || d->tet_count < 0 || d->hex_count < 0 || d->fixed_count < 0)
+  if (d->particle_count < 0 || d->tet_count < 0 || d->hex_count < 0 || d->fixed_count < 0 || d->spring_count < 0 ||
+      d->tri_count < 0)
     return fail(FEM_ERR_ARG, "negative count");
   auto* s = new fem_system();
   auto cleanup = [&](int rc) {
@@ -2058,6 +2146,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
   s->P = d->particle_count;
   s->T = d->tet_count;
   s->H = d->hex_count;
+  s->S = d->spring_count;
+  s->R = d->tri_count;
   s->nfixed = d->fixed_count;
   const int P = s->P;
   if (d->instance_count > 1) {
@@ -2071,8 +2161,57 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
     s->inst_off = {0, P};
   }
   std::string err;
-  int rc = build_pattern(P, s->T, d->tet_indices, d->tet_materials, s->H, d->hex_indices, s->pt, err);
+  int rc = build_pattern(P, s->T, d->tet_indices, d->tet_materials, s->H, d->hex_indices, s->S, d->spring_indices,
+                         s->R, d->tri_indices, s->pt, err);
   if (rc) return cleanup(fail(rc, err));
+  if (s->S > 0 || s->R > 0) {
+    // Constant blocks: per slot the (ke, kd) sums of its springs (the
+    // diagonal slot holds the row's sums), per row the triangles'
+    // (k_mu + k_lambda) * rest area; degenerate triangles contribute nothing
+    // (build_system_matrix_tri_kernel kernels.py:994-1008).
+    std::vector<float2> slot_spring(s->S > 0 ? s->pt.columns.size() : 0, make_float2(0.f, 0.f));
+    std::vector<float> row_tri(s->R > 0 ? P : 0, 0.f);
+    auto slot_of = [&](int r, int c) {
+      const int32_t* row = s->pt.columns.data() + s->pt.offsets[r];
+      return (size_t)(s->pt.offsets[r] + (std::lower_bound(row, row + (s->pt.offsets[r + 1] - s->pt.offsets[r]), c) - row));
+    };
+    for (int k = 0; k < s->S; ++k) {
+      const int p = d->spring_indices[2 * k], q = d->spring_indices[2 * k + 1];
+      if (p < 0 || q < 0) continue;
+      const float ke = d->spring_stiffness[k], kd = d->spring_damping[k];
+      for (size_t sl : {slot_of(p, q), slot_of(q, p), slot_of(p, p), slot_of(q, q)}) {
+        slot_spring[sl].x += ke;
+        slot_spring[sl].y += kd;
+      }
+    }
+    for (int t = 0; t < s->R; ++t) {
+      const float* D = d->tri_poses + 4 * (size_t)t;
+      const float inv_area = (D[0] * D[3] - D[1] * D[2]) * 2.f;
+      if (!(inv_area > 0.f)) continue;
+      const float area = 1.f / inv_area;
+      const float c = d->tri_materials[5 * (size_t)t] * area + d->tri_materials[5 * (size_t)t + 1] * area;
+      for (int a = 0; a < 3; ++a) row_tri[d->tri_indices[3 * (size_t)t + a]] += c;
+    }
+    std::vector<int32_t> sptr(P + 1), rptr(P + 1);
+    for (int i = 0; i <= P; ++i) {
+      sptr[i] = (int32_t)s->pt.sinc_ptr[i];
+      rptr[i] = (int32_t)s->pt.rinc_ptr[i];
+    }
+    if ((rc = s->upload(&s->d_sinc_ptr, sptr.data(), sptr.size())) ||
+        (rc = s->upload(&s->d_sinc, s->pt.sinc.data(), s->pt.sinc.size())) ||
+        (rc = s->upload(&s->d_sp_idx, d->spring_indices, 2 * (size_t)s->S)) ||
+        (rc = s->upload(&s->d_sp_rest, d->spring_rest_length, (size_t)s->S)) ||
+        (rc = s->upload(&s->d_sp_ke, d->spring_stiffness, (size_t)s->S)) ||
+        (rc = s->upload(&s->d_sp_kd, d->spring_damping, (size_t)s->S)) ||
+        (rc = s->upload(&s->d_slot_spring, slot_spring.data(), slot_spring.size())) ||
+        (rc = s->upload(&s->d_rinc_ptr, rptr.data(), rptr.size())) ||
+        (rc = s->upload(&s->d_rinc, s->pt.rinc.data(), s->pt.rinc.size())) ||
+        (rc = s->upload(&s->d_tri_idx, d->tri_indices, 3 * (size_t)s->R)) ||
+        (rc = s->upload(&s->d_tri_pose, d->tri_poses, 4 * (size_t)s->R)) ||
+        (rc = s->upload(&s->d_tri_mat, d->tri_materials, 5 * (size_t)s->R)) ||
+        (rc = s->upload(&s->d_row_tri, row_tri.data(), row_tri.size())))
+      return cleanup(rc);
+  }
   // Elements must not couple different instances.
   if (s->ninst > 1) {
     std::vector<int32_t> inst_of(P);
@@ -2244,7 +2383,7 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchAssemble>(s->asm_rb, s->H > 0, ra, s->d_asm_groups, s->asm_ngroups, q, dt * dt, s->b_vals, s->b_minv,
+  return dispatch_g<LaunchAssemble>(s->asm_rb, s->full(), ra, s->d_asm_groups, s->asm_ngroups, q, dt, s->b_vals, s->b_minv,
                                     s->asm_lds, (hipStream_t)stream);
 }
 
@@ -2253,7 +2392,7 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
   if (!s || !q || !qd || !out) return fail(FEM_ERR_ARG, "null argument");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchRhs>(s->elem_g, s->H > 0, ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
+  return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
 
@@ -2271,8 +2410,9 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
     HIP_TRY(hipGetLastError());
   }
   RowArgs ra = s->row_args();
-  const int terms = FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
-  return dispatch_g<LaunchRhs>(s->elem_g, s->H > 0, ra, (const float*)q_in, (const float*)qd_in, act, ext,
+  const int terms = FEM_TERM_SPRING | FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_TRI | FEM_TERM_GROUND | FEM_TERM_GRAVITY |
+                    FEM_TERM_DAMPING;
+  return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }
 
@@ -2387,6 +2527,8 @@ int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, co
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (mat_bar && (!dv || !q_asm)) return fail(FEM_ERR_ARG, "material gradients need dv and the assembly positions");
+  if (s->S > 0 || s->R > 0)
+    return fail(FEM_ERR_UNSUPPORTED, "reverse products of springs and triangle membranes are not provided");
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   RowArgs ra = s->row_args();

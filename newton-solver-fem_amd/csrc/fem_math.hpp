@@ -282,6 +282,66 @@ FEM_DI v3 tet_node_force(const TetGeo& g, v3 v0, v3 v1, v3 v2, v3 v3_, float mu,
   return sel4(a, f0, f1, f2, f3) * -1.f;
 }
 
+// Linear spring between particles p (first) and q (second): the force on the
+// endpoint `end` (0: p receives -fs, 1: q receives +fs), fs = dir (ke c +
+// kd dc/dt), dir = (x_p - x_q) / |x_p - x_q| (eval_springs kernels.py:176-219).
+FEM_DI v3 spring_end_force(v3 xp, v3 xq, v3 vp, v3 vq, float rest, float ke, float kd, int end) {
+  const v3 xpq = xp - xq, vpq = vp - vq;
+  const float len = sqrtf(dot(xpq, xpq));
+  if (len < 1.0e-6f) return {0.f, 0.f, 0.f};
+  const v3 dir = xpq * (1.f / len);
+  const float c = len - rest;
+  const v3 fs = dir * (ke * c + kd * dot(dir, vpq));
+  return end == 0 ? fs * -1.f : fs;
+}
+
+FEM_DI v3 normalized(v3 a) {
+  const float l = sqrtf(dot(a, a));
+  return l > 0.f ? a * (1.f / l) : v3{0.f, 0.f, 0.f};
+}
+
+// Triangle membrane (eval_triangles kernels.py:876-970): the force on corner c
+// (0: +f0, 1: -f1, 2: -f2) from the deviatoric and area-preservation terms,
+// their damping, and aerodynamic drag and lift.  pose = inverse rest edge
+// matrix [[D00, D01], [D10, D11]]; mat = (k_mu, k_lambda, k_damp, k_drag, k_lift)
+// per unit rest area.
+FEM_DI v3 tri_corner_force(v3 x0, v3 x1, v3 x2, v3 v0, v3 v1, v3 v2, const float* __restrict__ pose,
+                           const float* __restrict__ mat, int c) {
+  const float D00 = pose[0], D01 = pose[1], D10 = pose[2], D11 = pose[3];
+  const float inv_rest_area = (D00 * D11 - D01 * D10) * 2.f;
+  const float rest_area = 1.f / inv_rest_area;
+  const float k_mu = mat[0] * rest_area, k_lambda = mat[1] * rest_area, k_damp = mat[2] * rest_area;
+  const float k_drag = mat[3], k_lift = mat[4];
+  const v3 x10 = x1 - x0, x20 = x2 - x0, v10 = v1 - v0, v20 = v2 - v0;
+  const v3 F1 = x10 * D00 + x20 * D10, F2 = x10 * D01 + x20 * D11;
+  const v3 dF1 = v10 * D00 + v20 * D10, dF2 = v10 * D01 + v20 * D11;
+  const float Ic = dot(F1, F1) + dot(F2, F2);
+  const float dev = (Ic - 2.f) / Ic;
+  const v3 P1 = F1 * (k_mu * dev) + dF1 * k_damp, P2 = F2 * (k_mu * dev) + dF2 * k_damp;
+  v3 f1 = P1 * D00 + P2 * D01, f2 = P1 * D10 + P2 * D11;
+  v3 n = cross(x10, x20);
+  const float area = sqrtf(dot(n, n)) * 0.5f;
+  const float cst = area * inv_rest_area - 1.f;
+  n = normalized(n);
+  const v3 dcdq = cross(x20, n) * (inv_rest_area * 0.5f);
+  const v3 dcdr = cross(n, x10) * (inv_rest_area * 0.5f);
+  const float f_area = k_lambda * cst;
+  const float dcdt = dot(dcdq, v1) + dot(dcdr, v2) - dot(dcdq + dcdr, v0);
+  const float f_damp = k_damp * dcdt;
+  f1 = f1 + dcdq * (f_area + f_damp);
+  f2 = f2 + dcdr * (f_area + f_damp);
+  v3 f0 = f1 + f2;
+  const v3 vmid = (v0 + v1 + v2) * 0.3333f;
+  const v3 vdir = normalized(vmid);
+  const v3 f_drag = vmid * (k_drag * area * fabsf(dot(n, vmid)));
+  const float cosang = fminf(fmaxf(dot(n, vdir), -1.f), 1.f);
+  const v3 f_lift = n * (k_lift * area * (1.5707963267948966f - acosf(cosang))) * dot(vmid, vmid);
+  f0 = f0 - f_drag - f_lift;
+  f1 = f1 + f_drag + f_lift;
+  f2 = f2 + f_drag + f_lift;
+  return c == 0 ? f0 : (c == 1 ? f1 * -1.f : f2 * -1.f);
+}
+
 // Volumetric activation force on node a (eval_tet_activations kernels.py:484-508).
 FEM_DI v3 tet_activation_force(v3 x0, v3 x1, v3 x2, v3 x3, float lam, float act, int a) {
   const float scale = lam * act / 6.f;

@@ -2,11 +2,13 @@
 // per-node element incidence lists used by the row-centric HIP kernels.
 //
 // The pattern equals what the reference obtains from bsr_set_from_triplets
-// over the triplet layout [P diagonal | 16 per Tet4 | 64 per Hex8]
-// (solver.py:312-341, kernels.py:141-153, :635-850, :1609-1747): per row the
+// over the triplet layout [P diagonal | 2 per spring | 3 diagonal per
+// triangle | 16 per Tet4 | 64 per Hex8] (solver.py:312-341, 1008-1136,
+// kernels.py:141-153, :294-366, :972-1030, :635-850, :1609-1747): per row the
 // sorted unique column indices of every valid triplet.  Tets whose three
 // material coefficients are all zero write row/column -1 and are excluded
-// (kernels.py:679-692); every Hex8 and every particle diagonal is present.
+// (kernels.py:679-692), as are springs with a -1 endpoint; every Hex8 and every
+// particle diagonal is present (triangles only add diagonal blocks).
 //
 // Incidences list, for each node i, the (element, local corner) pairs with
 // n_a == i, in ascending element order.  For each incidence the relative
@@ -22,10 +24,56 @@ namespace fem {
 
 static bool tet_valid(const float* mat) { return !(mat[0] == 0.f && mat[1] == 0.f && mat[2] == 0.f); }
 
-int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx,
-                  Pattern& pt, std::string& err) {
+static bool spring_valid(const int32_t* idx) { return idx[0] >= 0 && idx[1] >= 0; }
+
+int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx, int S,
+                  const int32_t* spring_idx, int R, const int32_t* tri_idx, Pattern& pt, std::string& err) {
   pt = Pattern();
   pt.P = P;
+  // Spring and triangle incidences.
+  pt.sinc_ptr.assign(P + 1, 0);
+  pt.rinc_ptr.assign(P + 1, 0);
+  for (int s = 0; s < S; ++s) {
+    const int32_t* e = spring_idx + 2 * (size_t)s;
+    if (!spring_valid(e)) continue;
+    if (e[0] >= P || e[1] >= P) {
+      err = "spring_indices contains an out-of-range particle";
+      return 1;
+    }
+    pt.sinc_ptr[e[0] + 1]++;
+    pt.sinc_ptr[e[1] + 1]++;
+  }
+  for (int t = 0; t < R; ++t)
+    for (int a = 0; a < 3; ++a) {
+      int n = tri_idx[3 * (size_t)t + a];
+      if (n < 0 || n >= P) {
+        err = "tri_indices contains an out-of-range particle";
+        return 1;
+      }
+      pt.rinc_ptr[n + 1]++;
+    }
+  for (int i = 0; i < P; ++i) {
+    pt.sinc_ptr[i + 1] += pt.sinc_ptr[i];
+    pt.rinc_ptr[i + 1] += pt.rinc_ptr[i];
+  }
+  if (S >= (1 << 30) || R >= (1 << 29)) {
+    err = "spring or triangle count exceeds the packed incidence limit";
+    return 3;
+  }
+  pt.sinc.resize(pt.sinc_ptr[P]);
+  pt.rinc.resize(pt.rinc_ptr[P]);
+  {
+    std::vector<int64_t> fill(pt.sinc_ptr.begin(), pt.sinc_ptr.end() - 1);
+    for (int s = 0; s < S; ++s) {
+      const int32_t* e = spring_idx + 2 * (size_t)s;
+      if (!spring_valid(e)) continue;
+      pt.sinc[fill[e[0]]++] = (int32_t)((uint32_t)s << 1);
+      pt.sinc[fill[e[1]]++] = (int32_t)(((uint32_t)s << 1) | 1u);
+    }
+    std::vector<int64_t> rfill(pt.rinc_ptr.begin(), pt.rinc_ptr.end() - 1);
+    for (int t = 0; t < R; ++t)
+      for (int a = 0; a < 3; ++a) pt.rinc[rfill[tri_idx[3 * (size_t)t + a]]++] = (int32_t)(((uint32_t)t << 2) | a);
+  }
   // Incidence counts.
   pt.tinc_ptr.assign(P + 1, 0);
   pt.hinc_ptr.assign(P + 1, 0);
@@ -86,6 +134,10 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
     for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
       int e = pt.hinc[k] >> 3;
       for (int b = 0; b < 8; ++b) cand.push_back(hex_idx[8 * (size_t)e + b]);
+    }
+    for (int64_t k = pt.sinc_ptr[i]; k < pt.sinc_ptr[i + 1]; ++k) {
+      const int s = pt.sinc[k] >> 1;
+      cand.push_back(spring_idx[2 * (size_t)s + (1 - (pt.sinc[k] & 1))]);
     }
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());

@@ -22,11 +22,17 @@ struct Pattern {
   // element, then hexes): (is_hex << 15) | (incidence index in row << 3) | corner b.
   std::vector<int32_t> slot_ptr;  // [nnz+1]
   std::vector<uint16_t> contrib;
+  // Springs with both endpoints valid: per node, (spring << 1) | end (end 0:
+  // the node is the spring's first particle).  Triangles: (tri << 2) | corner.
+  std::vector<int64_t> sinc_ptr;  // [P+1]
+  std::vector<int32_t> sinc;
+  std::vector<int64_t> rinc_ptr;  // [P+1]
+  std::vector<int32_t> rinc;
   int max_row_nnz = 0;
 };
 
-int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx,
-                  Pattern& pt, std::string& err);
+int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx, int S,
+                  const int32_t* spring_idx, int R, const int32_t* tri_idx, Pattern& pt, std::string& err);
 
 void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds);
 

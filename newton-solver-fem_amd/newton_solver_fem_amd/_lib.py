@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfem_hip.so"
 
 FEM_OK = 0
 FEM_TERM_TET, FEM_TERM_HEX, FEM_TERM_GROUND, FEM_TERM_GRAVITY, FEM_TERM_DAMPING = 1, 2, 4, 8, 16
+FEM_TERM_SPRING, FEM_TERM_TRI = 32, 64
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -39,6 +40,15 @@ class ModelDesc(ctypes.Structure):
         ("fixed_indices", _ip),
         ("instance_count", ctypes.c_int32),
         ("instance_offsets", _ip),
+        ("spring_count", ctypes.c_int32),
+        ("spring_indices", _ip),
+        ("spring_rest_length", _fp),
+        ("spring_stiffness", _fp),
+        ("spring_damping", _fp),
+        ("tri_count", ctypes.c_int32),
+        ("tri_indices", _ip),
+        ("tri_poses", _fp),
+        ("tri_materials", _fp),
     ]
 
 

@@ -3,10 +3,11 @@
 The reference builds its models with Newton 1.7's ``ModelBuilder`` plus the
 package's ``FEMBuilder`` (``src/newton_solver_fem/_src/solvers/fem/builder.py``).
 Newton is not a dependency here, so this module supplies the subset of that
-surface the FEM substep consumes (``SURVEY`` boundary list): particles with
-mass/inverse mass/radius/flags, Tet4 elements with inverted rest matrices
-(``tet_poses``) and (mu, lambda, k_damp) materials, Hex8 reference quadrature
-data under ``model.fem``, gravity, and ``state()`` / ``control()`` factories.
+surface the FEM substep consumes: particles with mass/inverse mass/radius/
+flags, Tet4 elements with inverted rest matrices (``tet_poses``) and (mu,
+lambda, k_damp) materials, linear springs, triangle membranes with inverted
+2x2 rest matrices (``tri_poses``), Hex8 reference quadrature data under
+``model.fem``, gravity, and ``state()`` / ``control()`` factories.
 
 Host copies of every array are kept on the model (``model.host``) because the
 sparsity pattern and element incidence lists are built on the host once.
@@ -97,8 +98,8 @@ class Model:
         self.gravity = tuple(float(g) for g in gravity)
         self.particle_count = int(len(host["particle_q"]))
         self.tet_count = int(len(host["tet_indices"]))
-        self.spring_count = 0
-        self.tri_count = 0
+        self.spring_count = int(len(host["spring_indices"]))
+        self.tri_count = int(len(host["tri_indices"]))
         dev = self.device
 
         def t(a, dtype):
@@ -114,6 +115,14 @@ class Model:
         self.tet_poses = t(host["tet_poses"], torch.float32).reshape(-1, 3, 3)
         self.tet_materials = t(host["tet_materials"], torch.float32).reshape(-1, 3)
         self.tet_activations = t(host["tet_activations"], torch.float32)
+        self.spring_indices = t(host["spring_indices"], torch.int32).reshape(-1)
+        self.spring_rest_length = t(host["spring_rest_length"], torch.float32)
+        self.spring_stiffness = t(host["spring_stiffness"], torch.float32)
+        self.spring_damping = t(host["spring_damping"], torch.float32)
+        self.tri_indices = t(host["tri_indices"], torch.int32).reshape(-1, 3)
+        self.tri_poses = t(host["tri_poses"], torch.float32).reshape(-1, 2, 2)
+        self.tri_materials = t(host["tri_materials"], torch.float32).reshape(-1, 5)
+        self.tri_areas = t(host["tri_areas"], torch.float32)
         h = host["hex_indices"]
         self.fem = LinearFEMData(
             len(h),
@@ -140,9 +149,15 @@ class Model:
 
 
 class ModelBuilder:
-    """Collects particles and Tet4 elements (Newton ``ModelBuilder`` subset)."""
+    """Collects particles, Tet4 elements, springs and triangles (Newton ``ModelBuilder`` subset)."""
 
     default_particle_radius = 0.1
+    # Newton's ModelBuilder defaults for triangle membranes.
+    default_tri_ke = 100.0
+    default_tri_ka = 100.0
+    default_tri_kd = 10.0
+    default_tri_drag = 0.0
+    default_tri_lift = 0.0
 
     def __init__(self, gravity=-9.81, up_axis: int = 2):
         if np.ndim(gravity) == 0:
@@ -152,6 +167,8 @@ class ModelBuilder:
         self.gravity = tuple(float(x) for x in gravity)
         self._q, self._qd, self._m, self._r, self._flags = [], [], [], [], []
         self._tet_idx, self._tet_pose, self._tet_mat, self._tet_act = [], [], [], []
+        self._spring_idx, self._spring_rest, self._spring_ke, self._spring_kd, self._spring_control = [], [], [], [], []
+        self._tri_idx, self._tri_pose, self._tri_mat, self._tri_area = [], [], [], []
 
     @property
     def particle_count(self) -> int:
@@ -164,6 +181,62 @@ class ModelBuilder:
     @property
     def tet_count(self) -> int:
         return sum(len(a) for a in self._tet_idx)
+
+    @property
+    def spring_count(self) -> int:
+        return len(self._spring_ke)
+
+    @property
+    def spring_stiffness(self):
+        return list(self._spring_ke)
+
+    @property
+    def spring_damping(self):
+        return list(self._spring_kd)
+
+    @property
+    def tri_count(self) -> int:
+        return len(self._tri_area)
+
+    def add_spring(self, i: int, j: int, ke: float, kd: float, control: float = 0.0) -> None:
+        """Linear spring between particles i and j; the rest length is their current distance."""
+        q = np.asarray(self._q, dtype=np.float64)
+        self._spring_idx.append((int(i), int(j)))
+        self._spring_rest.append(float(np.linalg.norm(q[i] - q[j])))
+        self._spring_ke.append(float(ke))
+        self._spring_kd.append(float(kd))
+        self._spring_control.append(float(control))
+
+    def add_triangle(self, i: int, j: int, k: int, tri_ke=None, tri_ka=None, tri_kd=None, tri_drag=None,
+                     tri_lift=None) -> float:
+        """Triangle membrane; returns its rest area (0 and nothing added when degenerate).
+
+        The rest pose is the inverse of the 2x2 matrix of the edges (x_j - x_i,
+        x_k - x_i) in an in-plane basis (e1 along the first edge, e2 = n x e1).
+        """
+        q = np.asarray(self._q, dtype=np.float64)
+        qp, rp = q[j] - q[i], q[k] - q[i]
+        n = np.cross(qp, rp)
+        nl, el = np.linalg.norm(n), np.linalg.norm(qp)
+        if nl <= 0.0 or el <= 0.0:
+            warnings.warn("skipping a degenerate triangle element")
+            return 0.0
+        e1 = qp / el
+        e2 = np.cross(n / nl, e1)
+        e2 /= np.linalg.norm(e2)
+        D = np.array((e1, e2)) @ np.array((qp, rp)).T
+        area = np.linalg.det(D) / 2.0
+        if area <= 0.0:
+            warnings.warn("skipping an inverted or degenerate triangle element")
+            return 0.0
+        self._tri_idx.append((int(i), int(j), int(k)))
+        self._tri_pose.append(np.linalg.inv(D))
+        d = lambda v, dflt: float(dflt if v is None else v)
+        self._tri_mat.append((d(tri_ke, self.default_tri_ke), d(tri_ka, self.default_tri_ka),
+                              d(tri_kd, self.default_tri_kd), d(tri_drag, self.default_tri_drag),
+                              d(tri_lift, self.default_tri_lift)))
+        self._tri_area.append(float(area))
+        return float(area)
 
     def add_particle(self, pos, vel=(0.0, 0.0, 0.0), mass=1.0, radius=None, flags=PARTICLE_FLAG_ACTIVE) -> int:
         self._q.append(np.asarray(pos, dtype=np.float64).reshape(3))
@@ -228,6 +301,14 @@ class ModelBuilder:
             "tet_poses": cat(self._tet_pose, (0, 3, 3), np.float32),
             "tet_materials": cat(self._tet_mat, (0, 3), np.float32),
             "tet_activations": cat(self._tet_act, (0,), np.float32),
+            "spring_indices": np.asarray(self._spring_idx, dtype=np.int32).reshape(-1, 2),
+            "spring_rest_length": np.asarray(self._spring_rest, dtype=np.float32),
+            "spring_stiffness": np.asarray(self._spring_ke, dtype=np.float32),
+            "spring_damping": np.asarray(self._spring_kd, dtype=np.float32),
+            "tri_indices": np.asarray(self._tri_idx, dtype=np.int32).reshape(-1, 3),
+            "tri_poses": np.asarray(self._tri_pose, dtype=np.float32).reshape(-1, 2, 2),
+            "tri_materials": np.asarray(self._tri_mat, dtype=np.float32).reshape(-1, 5),
+            "tri_areas": np.asarray(self._tri_area, dtype=np.float32),
             "hex_indices": np.zeros((0, 8), np.int32),
             "hex_materials": np.zeros((0, 3), np.float32),
             "hex_inv_J0": np.zeros((0, 8, 3, 3), np.float32),
@@ -245,6 +326,52 @@ class FEMBuilder(ModelBuilder):
         super().__init__(*args, **kwargs)
         self.fem_hex_indices = []
         self.fem_hex_materials = []
+
+    def add_soft_mesh(self, pos, rot, scale, vel, vertices, indices, density, k_mu, k_lambda, k_damp, tri_ke=None,
+                      tri_ka=None, tri_kd=None, tri_drag=None, tri_lift=None):
+        """Tet4 mesh with lumped masses, open-face triangles and a unique-edge spring network.
+
+        Particles at ``rot * (scale * v) + pos`` (``rot`` a quaternion x, y, z, w)
+        with velocity ``vel`` and mass ``density * volume / 4`` from every
+        incident tet; every face that belongs to one tet becomes a triangle
+        membrane (Newton's ``add_soft_mesh``).  Each unique edge of a tet with
+        positive rest volume receives one spring of stiffness ``k_mu / 2`` and
+        damping ``k_damp / 2`` (reference builder.py:51-79).
+        """
+        verts = np.asarray(vertices, dtype=np.float64).reshape(-1, 3) * float(scale)
+        x, y, z, w = (float(c) for c in np.asarray(rot, dtype=np.float64).reshape(4))
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        start = self.particle_count
+        pts = verts @ R.T + np.asarray(pos, dtype=np.float64).reshape(3)
+        self.add_particles(pts, np.broadcast_to(np.asarray(vel, dtype=np.float64).reshape(3), pts.shape), 0.0)
+        tets = np.asarray(indices, dtype=np.int64).reshape(-1, 4) + start
+        first_tet = self.tet_count
+        faces = {}
+        for t in tets:
+            volume = self.add_tetrahedron(*t, k_mu=k_mu, k_lambda=k_lambda, k_damp=k_damp)
+            if volume > 0.0:
+                for n in t:
+                    self._m[n] += density * volume / 4.0
+                a, b, c, d = (int(v) for v in t)
+                for face in ((a, c, b), (b, c, d), (a, b, d), (a, d, c)):
+                    key = tuple(sorted(face))
+                    if key in faces:
+                        del faces[key]
+                    else:
+                        faces[key] = face
+        for face in faces.values():
+            self.add_triangle(*face, tri_ke=tri_ke, tri_ka=tri_ka, tri_kd=tri_kd, tri_drag=tri_drag,
+                              tri_lift=tri_lift)
+        idx = np.concatenate(self._tet_idx)[first_tet:] if self._tet_idx else np.zeros((0, 4), np.int32)
+        added = set()
+        for a, b, c, d in idx:
+            for left, right in ((a, b), (a, c), (a, d), (b, c), (b, d), (c, d)):
+                key = (int(min(left, right)), int(max(left, right)))
+                if key not in added:
+                    added.add(key)
+                    self.add_spring(key[0], key[1], float(k_mu) * 0.5, float(k_damp) * 0.5, 0.0)
 
     def add_soft_hex_mesh(self, hex_indices, k_mu=1.0e3, k_lambda=1.0e3, k_damp=0.0):
         indices = np.asarray(hex_indices, dtype=np.int32)

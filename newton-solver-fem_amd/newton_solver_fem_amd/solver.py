@@ -134,8 +134,8 @@ class SolverFEM:
                 "this substep implements solver_type='cg' with preconditioner_type='block_diag'; "
                 f"got solver_type={solver_type!r}, preconditioner_type={preconditioner_type!r}"
             )
-        if model.spring_count or model.tri_count:
-            raise NotImplementedError("springs and triangle membranes are not part of this substep path")
+        if (model.spring_count or model.tri_count) and differentiable:
+            raise NotImplementedError("differentiable steps with springs or triangle membranes are not supported")
         if torque_stiffness > 0.0 and spring_rest_direction is not None:
             raise NotImplementedError("spring bend torque is not part of this substep path")
         if self_contact_ke > 0.0 and model.particle_count > 1 and float(np.max(model.host["particle_radius"], initial=0)) > 0:
@@ -209,6 +209,15 @@ class SolverFEM:
         desc.hex_materials = fptr("hm", host["hex_materials"])
         desc.fixed_count = int(fixed.size)
         desc.fixed_indices = iptr("fx", fixed)
+        desc.spring_count = model.spring_count
+        desc.spring_indices = iptr("si", host["spring_indices"])
+        desc.spring_rest_length = fptr("sl", host["spring_rest_length"])
+        desc.spring_stiffness = fptr("sk", host["spring_stiffness"])
+        desc.spring_damping = fptr("sd", host["spring_damping"])
+        desc.tri_count = model.tri_count
+        desc.tri_indices = iptr("ri", host["tri_indices"])
+        desc.tri_poses = fptr("rp", host["tri_poses"])
+        desc.tri_materials = fptr("rm", host["tri_materials"])
         if instance_offsets is not None:
             offs = np.asarray(instance_offsets, dtype=np.int32)
             desc.instance_count = int(len(offs) - 1)
@@ -506,6 +515,12 @@ class SolverFEM:
             "fem_eval_forces",
         )
         return out
+
+    def eval_spring_forces(self, model, state):
+        return self._eval(_lib.FEM_TERM_SPRING, state)
+
+    def eval_triangle_forces(self, model, control, state):
+        return self._eval(_lib.FEM_TERM_TRI, state)
 
     def eval_tetrahedral_forces(self, model, control, state):
         return self._eval(_lib.FEM_TERM_TET, state, control)

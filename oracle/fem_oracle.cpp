@@ -270,12 +270,12 @@ struct Params {
   int maxiter = 50;
 };
 
-enum { TERM_TET = 1, TERM_HEX = 2, TERM_GROUND = 4, TERM_GRAVITY = 8, TERM_DAMPING = 16 };
+enum { TERM_TET = 1, TERM_HEX = 2, TERM_GROUND = 4, TERM_GRAVITY = 8, TERM_DAMPING = 16, TERM_SPRING = 32, TERM_TRI = 64 };
 
 template <typename Real>
 struct Solver {
   using v3 = V3<Real>;
-  int P = 0, T = 0, H = 0;
+  int P = 0, T = 0, H = 0, S = 0, R = 0;
   std::vector<v3> anchor;  // construction-time positions (FixedParticles.reference)
   std::vector<Real> mass, inv_mass, radius;
   std::vector<int> flags;
@@ -284,9 +284,13 @@ struct Solver {
   std::vector<int> hex_idx;  // H*8
   std::vector<Real> hex_invJ0, hex_detJ0w, hex_mat;  // H*72, H*8, H*3
   std::vector<int> fixed_idx, fixed_mask;
+  std::vector<int> spring_idx;  // S*2
+  std::vector<Real> spring_rest, spring_ke, spring_kd;
+  std::vector<int> tri_idx;  // R*3
+  std::vector<Real> tri_pose, tri_mat;  // R*4 (mat22 row-major), R*5 (mu, lambda, damp, drag, lift)
   Params prm;
 
-  // Triplets [P diag | 16T tets | 64H hexes] (solver.py:312-321).
+  // Triplets [P diag | 2S springs | 3R tris | 16T tets | 64H hexes] (solver.py:312-321, 1008-1136).
   std::vector<int> trow, tcol;
   std::vector<Real> tval;  // 9 per triplet
   // BSR.
@@ -372,7 +376,7 @@ struct Solver {
     for (int e = 0; e < T; ++e) {
       int n[4] = {tet_idx[4 * e], tet_idx[4 * e + 1], tet_idx[4 * e + 2], tet_idx[4 * e + 3]};
       Real k_mu = tet_mat[3 * e], k_lambda = tet_mat[3 * e + 1];
-      size_t base = (size_t)P + 16 * (size_t)e;
+      size_t base = tet_off() + 16 * (size_t)e;
       if (k_mu == Real(0) && k_lambda == Real(0) && tet_mat[3 * e + 2] == Real(0)) {
         for (int q = 0; q < 16; ++q) {
           trow[base + q] = -1;
@@ -501,7 +505,7 @@ struct Solver {
   void hex_blocks(const v3* x, Real dt) {
     for (int e = 0; e < H; ++e) {
       Real k_mu = hex_mat[3 * e], k_lambda = hex_mat[3 * e + 1];
-      size_t base = (size_t)P + 16 * (size_t)T + 64 * (size_t)e;
+      size_t base = tet_off() + 16 * (size_t)T + 64 * (size_t)e;
       Real K[64][9];
       for (int q = 0; q < 64; ++q)
         for (int r = 0; r < 9; ++r) K[q][r] = 0;
@@ -587,16 +591,115 @@ struct Solver {
     }
   }
 
+  // ---- springs (kernels.py:176-219) and triangle membranes (:876-970) -----
+  // eval_springs: f_i -= fs, f_j += fs, fs = dir (ke c + kd dc/dt).
+  void eval_springs(const v3* x, const v3* v, std::vector<v3>& f) const {
+    for (int s = 0; s < S; ++s) {
+      int i = spring_idx[2 * s], j = spring_idx[2 * s + 1];
+      if (i == -1 || j == -1) continue;
+      v3 xij = x[i] - x[j], vij = v[i] - v[j];
+      Real len = std::sqrt(dot(xij, xij));
+      if (len < Real(1e-6)) continue;
+      v3 dir = xij * (Real(1) / len);
+      Real c = len - spring_rest[s];
+      Real dcdt = dot(dir, vij);
+      v3 fs = dir * (spring_ke[s] * c + spring_kd[s] * dcdt);
+      f[i] = f[i] - fs;
+      f[j] = f[j] + fs;
+    }
+  }
+  static v3 normalized(v3 a) {
+    Real l = std::sqrt(dot(a, a));
+    return l > Real(0) ? a * (Real(1) / l) : v3{0, 0, 0};
+  }
+  // eval_triangles: deviatoric + area-preservation membrane, damping, drag and lift.
+  void eval_tris(const v3* x, const v3* v, std::vector<v3>& f) const {
+    for (int t = 0; t < R; ++t) {
+      Real k_mu = tri_mat[5 * t], k_lambda = tri_mat[5 * t + 1], k_damp = tri_mat[5 * t + 2];
+      Real k_drag = tri_mat[5 * t + 3], k_lift = tri_mat[5 * t + 4];
+      int i = tri_idx[3 * t], j = tri_idx[3 * t + 1], k = tri_idx[3 * t + 2];
+      v3 x10 = x[j] - x[i], x20 = x[k] - x[i];
+      v3 v10 = v[j] - v[i], v20 = v[k] - v[i];
+      const Real* Dm = &tri_pose[4 * t];  // [[Dm00, Dm01], [Dm10, Dm11]]
+      Real inv_rest_area = (Dm[0] * Dm[3] - Dm[1] * Dm[2]) * Real(2);
+      Real rest_area = Real(1) / inv_rest_area;
+      k_mu *= rest_area;
+      k_lambda *= rest_area;
+      k_damp *= rest_area;
+      v3 F1 = x10 * Dm[0] + x20 * Dm[2], F2 = x10 * Dm[1] + x20 * Dm[3];
+      v3 dF1 = v10 * Dm[0] + v20 * Dm[2], dF2 = v10 * Dm[1] + v20 * Dm[3];
+      Real Ic = dot(F1, F1) + dot(F2, F2);
+      Real dev = (Ic - Real(2)) / Ic;
+      v3 P1 = F1 * (k_mu * dev) + dF1 * k_damp, P2 = F2 * (k_mu * dev) + dF2 * k_damp;
+      v3 f1 = P1 * Dm[0] + P2 * Dm[1], f2 = P1 * Dm[2] + P2 * Dm[3];
+      v3 n = cross(x10, x20);
+      Real area = std::sqrt(dot(n, n)) * Real(0.5);
+      Real c = area * inv_rest_area - Real(1);
+      n = normalized(n);
+      v3 dcdq = cross(x20, n) * (inv_rest_area * Real(0.5));
+      v3 dcdr = cross(n, x10) * (inv_rest_area * Real(0.5));
+      Real f_area = k_lambda * c;
+      Real dcdt = dot(dcdq, v[j]) + dot(dcdr, v[k]) - dot(dcdq + dcdr, v[i]);
+      Real f_damp = k_damp * dcdt;
+      f1 = f1 + dcdq * (f_area + f_damp);
+      f2 = f2 + dcdr * (f_area + f_damp);
+      v3 f0 = f1 + f2;
+      v3 vmid = (v[i] + v[j] + v[k]) * Real(0.3333);
+      v3 vdir = normalized(vmid);
+      v3 f_drag = vmid * (k_drag * area * std::fabs(dot(n, vmid)));
+      Real cosang = std::min(std::max(dot(n, vdir), Real(-1)), Real(1));
+      v3 f_lift = n * (k_lift * area * (Real(1.5707963267948966) - std::acos(cosang))) * dot(vmid, vmid);
+      f0 = f0 - f_drag - f_lift;
+      f1 = f1 + f_drag + f_lift;
+      f2 = f2 + f_drag + f_lift;
+      f[i] = f[i] + f0;
+      f[j] = f[j] - f1;
+      f[k] = f[k] - f2;
+    }
+  }
+
   // ---- assembly ----------------------------------------------------------
+  size_t tet_off() const { return (size_t)P + 2 * (size_t)S + 3 * (size_t)R; }
+
   void write_triplets(const v3* x, Real dt) {
-    size_t n = (size_t)P + 16 * (size_t)T + 64 * (size_t)H;
+    size_t n = tet_off() + 16 * (size_t)T + 64 * (size_t)H;
     trow.assign(n, 0);
     tcol.assign(n, 0);
     tval.assign(9 * n, Real(0));
-    for (int i = 0; i < P; ++i) {  // build_system_matrix_diagonal_mass_kernel kernels.py:141-153
+    const Real dt2 = dt * dt;
+    for (int i = 0; i < P; ++i) {
+      // build_system_matrix_diagonal_mass_kernel kernels.py:141-153, or with
+      // springs build_system_matrix_diagonal_kernel :294-323 (O(P S) scan)
+      Real diag = mass[i];
+      for (int s = 0; s < S; ++s)
+        if (spring_idx[2 * s] == i || spring_idx[2 * s + 1] == i) diag = diag + dt * spring_kd[s] + dt2 * spring_ke[s];
       trow[i] = i;
       tcol[i] = i;
-      tval[9 * i] = tval[9 * i + 4] = tval[9 * i + 8] = mass[i];
+      tval[9 * i] = tval[9 * i + 4] = tval[9 * i + 8] = diag;
+    }
+    for (int s = 0; s < S; ++s) {  // build_system_matrix_sparse_kernel kernels.py:326-366
+      size_t b = (size_t)P + 2 * (size_t)s;
+      int i = spring_idx[2 * s], j = spring_idx[2 * s + 1];
+      Real neg = -(dt * spring_kd[s] + dt * dt * spring_ke[s]);
+      trow[b] = i;
+      tcol[b] = j;
+      trow[b + 1] = j;
+      tcol[b + 1] = i;
+      for (int q = 0; q < 2; ++q) tval[9 * (b + q)] = tval[9 * (b + q) + 4] = tval[9 * (b + q) + 8] = neg;
+    }
+    for (int t = 0; t < R; ++t) {  // build_system_matrix_tri_kernel kernels.py:972-1030
+      size_t b = (size_t)P + 2 * (size_t)S + 3 * (size_t)t;
+      const Real* Dm = &tri_pose[4 * t];
+      Real inv_rest_area = (Dm[0] * Dm[3] - Dm[1] * Dm[2]) * Real(2);
+      Real val = 0;
+      if (inv_rest_area > Real(0)) {
+        Real rest_area = Real(1) / inv_rest_area;
+        val = dt * dt * (tri_mat[5 * t] * rest_area + tri_mat[5 * t + 1] * rest_area) / Real(3);
+      }
+      for (int q = 0; q < 3; ++q) {
+        trow[b + q] = tcol[b + q] = tri_idx[3 * t + q];
+        tval[9 * (b + q)] = tval[9 * (b + q) + 4] = tval[9 * (b + q) + 8] = val;
+      }
     }
     tet_blocks(x, dt);
     hex_blocks(x, dt);
@@ -798,7 +901,7 @@ struct Solver {
     }
     forces_scaled(q_in, qd_in, ext, act, dt, f_out);
     // subtract_elastic_predictor implicit_predictor.py:9-23
-    if (T || H) {
+    if (T || H || S) {
       for (int i = 0; i < P; ++i) {
         Real c[3] = {0, 0, 0};
         for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
@@ -807,6 +910,15 @@ struct Solver {
           for (int r = 0; r < 3; ++r) c[r] += b[3 * r] * d.x + b[3 * r + 1] * d.y + b[3 * r + 2] * d.z;
         }
         f_out[i] = f_out[i] - v3{c[0], c[1], c[2]};
+      }
+      // restore_spring_damping_predictor implicit_predictor.py:26-41
+      for (int s = 0; s < S; ++s) {
+        int a = spring_idx[2 * s], b = spring_idx[2 * s + 1];
+        if (a >= 0 && b >= 0) {
+          v3 corr = (qd_in[a] - qd_in[b]) * (dt * spring_kd[s]);
+          f_out[a] = f_out[a] + corr;
+          f_out[b] = f_out[b] - corr;
+        }
       }
     }
     for (int i : fixed_idx) f_out[i] = v3{0, 0, 0};  // project_rhs
@@ -837,20 +949,24 @@ struct Solver {
   // _accumulate_forces solver.py:542-589 + external add :993-999:
   // particle_f = sum_term h*term in the reference's term order.
   void forces_scaled(const v3* q, const v3* qd, const v3* ext, const Real* act, Real dt, v3* out) {
-    std::vector<v3> tet(P, v3{0, 0, 0}), hex(P, v3{0, 0, 0}), gr(P, v3{0, 0, 0}), grav(P, v3{0, 0, 0}),
-        damp(P, v3{0, 0, 0});
+    std::vector<v3> spr(P, v3{0, 0, 0}), tet(P, v3{0, 0, 0}), hex(P, v3{0, 0, 0}), tri(P, v3{0, 0, 0}),
+        gr(P, v3{0, 0, 0}), grav(P, v3{0, 0, 0}), damp(P, v3{0, 0, 0});
+    if (S) eval_springs(q, qd, spr);
     if (T) {
       eval_tets(q, qd, tet);
       if (prm.enable_activations && act) eval_tet_activations(q, act, tet);
     }
     if (H) eval_hexes(q, qd, hex);
+    if (R) eval_tris(q, qd, tri);
     eval_ground(q, qd, gr);
     eval_gravity(grav);
     eval_damping(qd, damp);
     for (int i = 0; i < P; ++i) {
       v3 f = {0, 0, 0};
+      f = f + spr[i] * dt;
       f = f + tet[i] * dt;
       f = f + hex[i] * dt;
+      f = f + tri[i] * dt;
       f = f + gr[i] * dt;
       f = f + grav[i] * dt;
       f = f + damp[i] * dt;
@@ -1276,6 +1392,8 @@ struct Solver {
 
   void eval_terms(int mask, const v3* q, const v3* qd, const Real* act, v3* out) {
     std::vector<v3> acc(P, v3{0, 0, 0});
+    if ((mask & TERM_SPRING) && S) eval_springs(q, qd, acc);
+    if ((mask & TERM_TRI) && R) eval_tris(q, qd, acc);
     if ((mask & TERM_TET) && T) {
       eval_tets(q, qd, acc);
       if (prm.enable_activations && act) eval_tet_activations(q, act, acc);
@@ -1406,6 +1524,25 @@ void orc_set_params(void* p, const double* a) {
   prm.tol = a[16];
   prm.maxiter = (int)a[17];
   DISPATCH(h, S.prm = prm);
+}
+
+// Springs (indices S x 2, rest length, stiffness, damping) and triangle
+// membranes (indices R x 3, inverse rest pose R x 2 x 2, materials R x 5);
+// call before orc_construct.
+void orc_set_springs_tris(void* p, int ns, const int* si, const double* rest, const double* ke, const double* kd,
+                          int nr, const int* ti, const double* tp, const double* tm) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, {
+    S.S = ns;
+    S.spring_idx.assign(si, si + 2 * (size_t)ns);
+    S.spring_rest.assign(rest, rest + ns);
+    S.spring_ke.assign(ke, ke + ns);
+    S.spring_kd.assign(kd, kd + ns);
+    S.R = nr;
+    S.tri_idx.assign(ti, ti + 3 * (size_t)nr);
+    S.tri_pose.assign(tp, tp + 4 * (size_t)nr);
+    S.tri_mat.assign(tm, tm + 5 * (size_t)nr);
+  });
 }
 
 void orc_construct(void* p, double dt) {

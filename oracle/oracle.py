@@ -19,7 +19,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liborc_fem.so"
 
-TERM_TET, TERM_HEX, TERM_GROUND, TERM_GRAVITY, TERM_DAMPING = 1, 2, 4, 8, 16
+TERM_TET, TERM_HEX, TERM_GROUND, TERM_GRAVITY, TERM_DAMPING, TERM_SPRING, TERM_TRI = 1, 2, 4, 8, 16, 32, 64
 
 _lib = None
 
@@ -61,6 +61,7 @@ def lib():
         _lib.orc_diff_forward.argtypes = [vp, d, d, vp, vp, ctypes.c_double, ctypes.c_double, d, d, d]
         _lib.orc_diff_backward.argtypes = [vp, d, d, vp, d, d, d, ctypes.c_double, ctypes.c_double, d, d, d, d, d]
         _lib.orc_set_tet_materials.argtypes = [vp, d]
+        _lib.orc_set_springs_tris.argtypes = [vp, ctypes.c_int, i, d, d, d, ctypes.c_int, i, d, d]
     return _lib
 
 
@@ -133,6 +134,14 @@ class OracleSolver:
             D(host["tet_poses"]), D(host["tet_materials"]), self.H, I(host["hex_indices"]), D(host["hex_inv_J0"]),
             D(host["hex_det_J0_w"]), D(host["hex_materials"]), len(fixed), I(fixed),
         )
+        self.S = len(host.get("spring_indices", ()))
+        self.R = len(host.get("tri_indices", ()))
+        if self.S or self.R:
+            L.orc_set_springs_tris(
+                self.h, self.S, I(host["spring_indices"]), D(host["spring_rest_length"]), D(host["spring_stiffness"]),
+                D(host["spring_damping"]), self.R, I(host["tri_indices"]), D(host["tri_poses"]),
+                D(host["tri_materials"]),
+            )
         self.params = dict(min_stretch=min_stretch, max_stretch=max_stretch, ground_plane=ground_plane,
                            ground_ke=ground_ke, ground_kd=ground_kd, ground_kf=ground_kf, ground_mu=ground_mu,
                            linear_damping=linear_damping, gravity=gravity,
