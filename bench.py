@@ -110,6 +110,99 @@ def _graph_two_substeps(solver, model, dt):
     return g, (s0, s1)
 
 
+def showcase_model(dev):
+    """Showcase-sized scene: three Tet4 soft meshes (add_soft_mesh: lumped tets, open-face
+    triangles, unique-edge springs) and one Hex8 block, side by side above the ground.
+
+    Sizes follow the reference's published showcase (docs/validation/showcase.json:
+    10,030 particles, 39,800 Tet4 + 385 Hex8, 54,392 springs); materials, per-particle
+    masses (the density value, as the example sets them), ground and damping follow
+    examples/linear_fem.json.  The USD assets themselves are not used.
+    """
+    import numpy as np
+
+    from newton_solver_fem_amd import meshes
+    from newton_solver_fem_amd.model import FEMBuilder
+
+    b = FEMBuilder(gravity=(0.0, 0.0, -9.81))
+    b.default_particle_radius = 0.008
+    masses = []
+    for (nx, ny, nz), y, mu, dens in (((16, 16, 16), -1.0, 1.0e5, 800.0), ((14, 14, 14), 0.0, 3.0e5, 800.0),
+                                      ((10, 10, 11), 1.0, 8.0e5, 2000.0)):
+        start = b.particle_count
+        b.add_soft_mesh(pos=(0.0, y, 0.4), rot=(0, 0, 0, 1), scale=1.0, vel=(0, 0, 0),
+                        vertices=meshes.lattice_points(nx, ny, nz, 0.02), indices=meshes.tet_lattice_indices(nx, ny, nz),
+                        density=dens, k_mu=mu, k_lambda=mu, k_damp=100.0, tri_ke=0.0, tri_ka=0.0, tri_kd=0.0)
+        masses.append((start, b.particle_count, dens))
+    start = b.particle_count
+    b.add_particles(meshes.lattice_points(5, 7, 11, 0.03, origin=(0.0, 2.0, 0.4)), np.zeros((6 * 8 * 12, 3)), 1000.0)
+    b.add_soft_hex_mesh(meshes.hex_lattice_indices(5, 7, 11) + start, k_mu=5.0e5, k_lambda=5.0e5, k_damp=0.01)
+    masses.append((start, b.particle_count, 1000.0))
+    for lo, hi, m in masses:
+        b._m[lo:hi] = [m] * (hi - lo)
+    return b.finalize(dev)
+
+
+def bench_showcase(dev, frames: int = 100, warmup: int = 30):
+    """Frames of the showcase-sized scene: one tangent refresh + 10 substeps (h = 1/600,
+    block-Jacobi PCG, tol 1e-2, <= 50 iterations) per frame, captured in a HIP graph,
+    against the reference's published 46.758 ms/frame (NVIDIA GB10, showcase.json)."""
+    import torch
+
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    model = showcase_model(dev)
+    sub = 10
+    dt = 1.0 / 60.0 / sub
+    solver = SolverFEM(model, dt=dt, solver_type="cg", preconditioner_type="block_diag", linear_solver_maxiter=50,
+                       ground_plane=(0.0, 0.0, 1.0, 0.0), ground_ke=1.0e4, ground_kd=10.0, ground_kf=500.0,
+                       ground_mu=0.5, linear_damping=2.0, self_contact_ke=0.0)
+    s0, s1 = model.state(), model.state()
+
+    def frame():
+        solver.refresh_tangent(s0.particle_q, dt)
+        a, b = s0, s1
+        for _ in range(sub):
+            solver.step(a, b, None, None, dt)
+            a, b = b, a  # an even substep count ends in s0
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        frame()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        frame()
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        g.replay()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / frames * 1e3
+    if not torch.isfinite(s0.particle_q).all():
+        raise RuntimeError("non-finite showcase state")
+    published = 46.757766123240195
+    return {
+        "workload": "showcase-sized scene: 3 Tet4 soft meshes (add_soft_mesh: tets, open-face triangles, edge springs) "
+                    "+ 1 Hex8 block, ground, linear damping; per frame 1 tangent refresh + 10 substeps (h = 1/600, "
+                    "PCG tol 1e-2, <= 50 iterations), HIP graph; no particle self-contact or surface contact",
+        "particles": model.particle_count,
+        "tets": model.tet_count,
+        "hexes": solver.hex_count,
+        "springs": model.spring_count,
+        "triangles": model.tri_count,
+        "ms_per_frame": ms,
+        "ms_per_substep": ms / sub,
+        "cg_iterations_last": solver.solve_stats()["iterations"][0],
+        "reference_ms_per_frame": published,
+        "reference_hardware": "NVIDIA GB10 (docs/validation/showcase.json)",
+        "speedup_vs_reference": published / ms,
+    }
+
+
 def bench_batched(dev, replays: int):
     """256 uneven drop meshes: one batched solver vs 256 solvers stepped in turn.
 
@@ -278,6 +371,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-looped measurement")
     ap.add_argument("--batched-replays", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--no-showcase", action="store_true", help="skip the showcase-sized scene")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -400,6 +494,8 @@ def main():
         out["batched"] = bench_batched(dev, args.batched_replays)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(dev)
+    if not args.no_showcase:
+        out["showcase"] = bench_showcase(dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, model, args.cpu_substeps)
         out["cpu_baseline"]["host_cpus"] = os.cpu_count()

@@ -200,12 +200,19 @@ class ModelBuilder:
 
     def add_spring(self, i: int, j: int, ke: float, kd: float, control: float = 0.0) -> None:
         """Linear spring between particles i and j; the rest length is their current distance."""
+        self.add_springs([(i, j)], ke, kd, control)
+
+    def add_springs(self, pairs, ke, kd, control=0.0) -> None:
+        """Vectorized ``add_spring`` (rest lengths from the current positions, FP64)."""
+        pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
         q = np.asarray(self._q, dtype=np.float64)
-        self._spring_idx.append((int(i), int(j)))
-        self._spring_rest.append(float(np.linalg.norm(q[i] - q[j])))
-        self._spring_ke.append(float(ke))
-        self._spring_kd.append(float(kd))
-        self._spring_control.append(float(control))
+        rest = np.linalg.norm(q[pairs[:, 0]] - q[pairs[:, 1]], axis=1)
+        n = len(pairs)
+        self._spring_idx.extend(map(tuple, pairs.tolist()))
+        self._spring_rest.extend(rest.tolist())
+        self._spring_ke.extend(np.broadcast_to(np.asarray(ke, dtype=np.float64), (n,)).tolist())
+        self._spring_kd.extend(np.broadcast_to(np.asarray(kd, dtype=np.float64), (n,)).tolist())
+        self._spring_control.extend(np.broadcast_to(np.asarray(control, dtype=np.float64), (n,)).tolist())
 
     def add_triangle(self, i: int, j: int, k: int, tri_ke=None, tri_ka=None, tri_kd=None, tri_drag=None,
                      tri_lift=None) -> float:
@@ -214,29 +221,36 @@ class ModelBuilder:
         The rest pose is the inverse of the 2x2 matrix of the edges (x_j - x_i,
         x_k - x_i) in an in-plane basis (e1 along the first edge, e2 = n x e1).
         """
+        return float(self.add_triangles([(i, j, k)], tri_ke, tri_ka, tri_kd, tri_drag, tri_lift)[0])
+
+    def add_triangles(self, indices, tri_ke=None, tri_ka=None, tri_kd=None, tri_drag=None, tri_lift=None):
+        """Vectorized ``add_triangle``; returns the rest areas (0 for skipped triangles)."""
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1, 3)
         q = np.asarray(self._q, dtype=np.float64)
-        qp, rp = q[j] - q[i], q[k] - q[i]
+        qp, rp = q[idx[:, 1]] - q[idx[:, 0]], q[idx[:, 2]] - q[idx[:, 0]]
         n = np.cross(qp, rp)
-        nl, el = np.linalg.norm(n), np.linalg.norm(qp)
-        if nl <= 0.0 or el <= 0.0:
-            warnings.warn("skipping a degenerate triangle element")
-            return 0.0
-        e1 = qp / el
-        e2 = np.cross(n / nl, e1)
-        e2 /= np.linalg.norm(e2)
-        D = np.array((e1, e2)) @ np.array((qp, rp)).T
-        area = np.linalg.det(D) / 2.0
-        if area <= 0.0:
-            warnings.warn("skipping an inverted or degenerate triangle element")
-            return 0.0
-        self._tri_idx.append((int(i), int(j), int(k)))
-        self._tri_pose.append(np.linalg.inv(D))
+        nl, el = np.linalg.norm(n, axis=1), np.linalg.norm(qp, axis=1)
+        area = np.zeros(len(idx))
+        ok = (nl > 0.0) & (el > 0.0)
+        e1 = qp[ok] / el[ok, None]
+        e2 = np.cross(n[ok] / nl[ok, None], e1)
+        e2 /= np.linalg.norm(e2, axis=1)[:, None]
+        D = np.empty((int(ok.sum()), 2, 2))
+        D[:, 0, 0], D[:, 0, 1] = np.sum(e1 * qp[ok], 1), np.sum(e1 * rp[ok], 1)
+        D[:, 1, 0], D[:, 1, 1] = np.sum(e2 * qp[ok], 1), np.sum(e2 * rp[ok], 1)
+        area[ok] = np.linalg.det(D) / 2.0
+        keep = area > 0.0
+        if not keep.all():
+            warnings.warn(f"skipping {int((~keep).sum())} inverted or degenerate triangle element(s)")
+        Dk = D[keep[ok]]
         d = lambda v, dflt: float(dflt if v is None else v)
-        self._tri_mat.append((d(tri_ke, self.default_tri_ke), d(tri_ka, self.default_tri_ka),
-                              d(tri_kd, self.default_tri_kd), d(tri_drag, self.default_tri_drag),
-                              d(tri_lift, self.default_tri_lift)))
-        self._tri_area.append(float(area))
-        return float(area)
+        mat = (d(tri_ke, self.default_tri_ke), d(tri_ka, self.default_tri_ka), d(tri_kd, self.default_tri_kd),
+               d(tri_drag, self.default_tri_drag), d(tri_lift, self.default_tri_lift))
+        self._tri_idx.extend(map(tuple, idx[keep].tolist()))
+        self._tri_pose.extend(np.linalg.inv(Dk) if len(Dk) else [])
+        self._tri_mat.extend([mat] * int(keep.sum()))
+        self._tri_area.extend(area[keep].tolist())
+        return area
 
     def add_particle(self, pos, vel=(0.0, 0.0, 0.0), mass=1.0, radius=None, flags=PARTICLE_FLAG_ACTIVE) -> int:
         self._q.append(np.asarray(pos, dtype=np.float64).reshape(3))
@@ -348,30 +362,31 @@ class FEMBuilder(ModelBuilder):
         self.add_particles(pts, np.broadcast_to(np.asarray(vel, dtype=np.float64).reshape(3), pts.shape), 0.0)
         tets = np.asarray(indices, dtype=np.int64).reshape(-1, 4) + start
         first_tet = self.tet_count
+        volume = self.add_tetrahedra(tets, [[k_mu, k_lambda, k_damp]])
+        ok = volume > 0.0
+        mass = np.asarray(self._m, dtype=np.float64)
+        np.add.at(mass, tets[ok].ravel(), np.repeat(density * volume[ok] / 4.0, 4))
+        self._m = mass.tolist()
         faces = {}
-        for t in tets:
-            volume = self.add_tetrahedron(*t, k_mu=k_mu, k_lambda=k_lambda, k_damp=k_damp)
-            if volume > 0.0:
-                for n in t:
-                    self._m[n] += density * volume / 4.0
-                a, b, c, d = (int(v) for v in t)
-                for face in ((a, c, b), (b, c, d), (a, b, d), (a, d, c)):
-                    key = tuple(sorted(face))
-                    if key in faces:
-                        del faces[key]
-                    else:
-                        faces[key] = face
-        for face in faces.values():
-            self.add_triangle(*face, tri_ke=tri_ke, tri_ka=tri_ka, tri_kd=tri_kd, tri_drag=tri_drag,
-                              tri_lift=tri_lift)
+        for a, b, c, d in tets[ok].tolist():
+            for face in ((a, c, b), (b, c, d), (a, b, d), (a, d, c)):
+                key = tuple(sorted(face))
+                if key in faces:
+                    del faces[key]
+                else:
+                    faces[key] = face
+        if faces:
+            self.add_triangles(list(faces.values()), tri_ke, tri_ka, tri_kd, tri_drag, tri_lift)
         idx = np.concatenate(self._tet_idx)[first_tet:] if self._tet_idx else np.zeros((0, 4), np.int32)
-        added = set()
-        for a, b, c, d in idx:
+        added, pairs = set(), []
+        for a, b, c, d in idx.tolist():
             for left, right in ((a, b), (a, c), (a, d), (b, c), (b, d), (c, d)):
-                key = (int(min(left, right)), int(max(left, right)))
+                key = (min(left, right), max(left, right))
                 if key not in added:
                     added.add(key)
-                    self.add_spring(key[0], key[1], float(k_mu) * 0.5, float(k_damp) * 0.5, 0.0)
+                    pairs.append(key)
+        if pairs:
+            self.add_springs(pairs, float(k_mu) * 0.5, float(k_damp) * 0.5, 0.0)
 
     def add_soft_hex_mesh(self, hex_indices, k_mu=1.0e3, k_lambda=1.0e3, k_damp=0.0):
         indices = np.asarray(hex_indices, dtype=np.int32)
