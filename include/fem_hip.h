@@ -193,6 +193,36 @@ int fem_eval_forces(fem_system* sys, int32_t terms, const float* q_dev, const fl
 int fem_solve_stats(fem_system* sys, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,
                     int32_t* error_flags);
 
+/* ---- Prepared affine mechanics (PreparedFEMStep, prepared_step.py:64-302) --
+ * The prepared right-hand side without extra load is fem_step_rhs on a copy of
+ * the input state (solver.prepare_step, solver.py:483-500). */
+
+/* apply_operator (prepared_step.py:233-241): y = A x, or (PAP + I - P) x with
+ * fixed particles.  x and y must differ. */
+int fem_apply_operator(fem_system* sys, const float* x_dev, float* y_dev, void* stream);
+
+/* apply_preconditioner (:243-251): y = M^-1 x with the block-Jacobi inverse. */
+int fem_apply_preconditioner(fem_system* sys, const float* x_dev, float* y_dev, void* stream);
+
+/* diagonal_blocks (_diagonal :46-61): out[i] = A_ii (zero when the row has no
+ * diagonal block), the identity on fixed rows; [P][9]. */
+int fem_diagonal_blocks(fem_system* sys, float* out_dev, void* stream);
+
+/* rhs(external_forces) (:197-231): out = base + h ext with fixed rows zero
+ * (ext may be NULL; out may alias base but not ext). */
+int fem_add_load(fem_system* sys, const float* base_dev, const float* ext_dev, float dt, float* out_dev,
+                 void* stream);
+
+/* solve (:268-289): the step's normalised block-Jacobi PCG (fem_set_params
+ * tolerance, budget and exit rule) on b with the warm start and result in
+ * dv_dev (a caller-owned increment, not the bound one); no state update. */
+int fem_solve_increment(fem_system* sys, const float* b_dev, float* dv_dev, void* stream);
+
+/* reconstruct (:291-302): v' = qd_in + dv, q' = q_in + h v' (update_state
+ * kernels.py:115-128), fixed particles restored; outputs must not alias inputs. */
+int fem_reconstruct(fem_system* sys, const float* dv_dev, const float* q_in_dev, const float* qd_in_dev,
+                    float* q_out_dev, float* qd_out_dev, float dt, void* stream);
+
 /* ---- Differentiable substep (differentiation.py:149-394) -----------------
  * The matrix of a recorded step is the current assembly (a fixed input, as in
  * the reference's MatrixSnapshot).  The forward right-hand side of a

@@ -1560,14 +1560,16 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       const int i = R0 + threadIdx.x + NT * k;
       const v3 dvi = v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
       st3(a.dv, i, dvi);
-      v3 vel = ld3(a.qd_in, i) + dvi;
-      v3 qo = ld3(a.q_in, i) + vel * a.h;
-      if (fx[k]) {
-        qo = ld3(a.anchor, i);
-        vel = v3{0.f, 0.f, 0.f};
+      if (a.q_out) {
+        v3 vel = ld3(a.qd_in, i) + dvi;
+        v3 qo = ld3(a.q_in, i) + vel * a.h;
+        if (fx[k]) {
+          qo = ld3(a.anchor, i);
+          vel = v3{0.f, 0.f, 0.f};
+        }
+        st3(a.q_out, i, qo);
+        st3(a.qd_out, i, vel);
       }
-      st3(a.q_out, i, qo);
-      st3(a.qd_out, i, vel);
     }
   }
   if (c == cb && threadIdx.x == 0) {
@@ -1575,6 +1577,86 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     a.stats[4 * inst + 1] = __float_as_int((float)rr);
     a.stats[4 * inst + 2] = __float_as_int((float)atol_sq);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Prepared-step actions (PreparedFEMStep, prepared_step.py:64-302).
+// ---------------------------------------------------------------------------
+// y = Op x with Op = A, or PAP + I - P with fixed rows (fixed_particles.py:29-53);
+// G lanes per row, fixed summation order.
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_apply_op(RowArgs a, const float* __restrict__ vals,
+                                                    const float* __restrict__ x, float* __restrict__ y) {
+  constexpr int NG = BLOCK / G;
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const int i = blockIdx.x * NG + grp;
+  const bool active = i < a.P;
+  const bool fixed_row = active && a.row_fixed[i] != 0;
+  v3 acc = {0.f, 0.f, 0.f};
+  if (active && !fixed_row) {
+    const int k1 = a.offsets[i + 1];
+    for (int k = a.offsets[i] + lane; k < k1; k += G) {
+      const int c = a.cols[k];
+      if (a.row_fixed[c]) continue;
+      const float* b = vals + 9 * (size_t)k;
+      const v3 u = ld3(x, c);
+      acc = acc + v3{b[0] * u.x + b[1] * u.y + b[2] * u.z, b[3] * u.x + b[4] * u.y + b[5] * u.z,
+                     b[6] * u.x + b[7] * u.y + b[8] * u.z};
+    }
+  }
+  acc = {group_sum<G>(acc.x), group_sum<G>(acc.y), group_sum<G>(acc.z)};
+  if (active && lane == 0) st3(y, i, fixed_row ? ld3(x, i) : acc);
+}
+
+// y = M^-1 x (block_jacobi_mv_kernel kernels.py:1852-1868 with alpha 1, beta 0).
+__global__ void k_apply_minv(const float* __restrict__ minv, const float* __restrict__ x, float* __restrict__ y,
+                             int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) st3(y, i, mv(minv + 9 * (size_t)i, ld3(x, i)));
+}
+
+// Unconstrained diagonal blocks, identity on fixed rows (prepared_step.py:46-61).
+__global__ void k_diag_blocks(RowArgs a, const float* __restrict__ vals, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.P) return;
+  float d[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (a.row_fixed[i]) {
+    d[0] = d[4] = d[8] = 1.f;
+  } else if (a.diag_rel[i] >= 0) {
+    const float* b = vals + 9 * (size_t)(a.offsets[i] + a.diag_rel[i]);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) d[c] = b[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) out[9 * (size_t)i + c] = d[c];
+}
+
+// out = base + h ext, fixed rows zero (the load term of the prepared right-hand side).
+__global__ void k_add_load(const float* __restrict__ base, const float* __restrict__ ext, float h,
+                           const unsigned char* __restrict__ row_fixed, float* __restrict__ out, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  v3 f = ld3(base, i);
+  if (ext) f = f + ld3(ext, i) * h;
+  if (row_fixed[i]) f = {0.f, 0.f, 0.f};
+  st3(out, i, f);
+}
+
+// update_state kernels.py:115-128 + FixedParticles.restore(state_out).
+__global__ void k_reconstruct(const float* __restrict__ dv, const float* __restrict__ q_in,
+                              const float* __restrict__ qd_in, const unsigned char* __restrict__ row_fixed,
+                              const float* __restrict__ anchor, float h, float* __restrict__ q_out,
+                              float* __restrict__ qd_out, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  v3 vel = ld3(qd_in, i) + ld3(dv, i);
+  v3 qo = ld3(q_in, i) + vel * h;
+  if (row_fixed[i]) {
+    qo = ld3(anchor, i);
+    vel = {0.f, 0.f, 0.f};
+  }
+  st3(q_out, i, qo);
+  st3(qd_out, i, vel);
 }
 
 __global__ void k_restore(const int* __restrict__ fixed, int nfixed, const float* __restrict__ anchor, float* q,
@@ -1622,6 +1704,16 @@ struct LaunchAssemble {
       attr[hex] = cap;
     }
     hipLaunchKernelGGL(kern, dim3(ngroups), dim3(BLOCK), lds, st, ra, groups, x, h, h * h, vals, minv);
+    HIP_TRY(hipGetLastError());
+    return FEM_OK;
+  }
+};
+
+template <int G>
+struct LaunchApplyOp {
+  static int run(const RowArgs& ra, const float* vals, const float* x, float* y, hipStream_t st) {
+    hipLaunchKernelGGL(k_apply_op<G>, dim3((ra.P + BLOCK / G - 1) / (BLOCK / G)), dim3(BLOCK), 0, st, ra, vals, x,
+                       y);
     HIP_TRY(hipGetLastError());
     return FEM_OK;
   }
@@ -2516,6 +2608,77 @@ int fem_solve(fem_system* s, const float* b, float* x, float tolerance, int32_t 
                      (const int*)s->d_cols, (const float*)s->b_vals, (const unsigned char*)s->d_row_fixed,
                      (const float*)x, b, s->P, s->d_norms);
   hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, st, (const double*)s->d_norms, (int)kind, s->d_dstats);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_apply_operator(fem_system* s, const float* x, float* y, void* stream) {
+  if (!s || !x || !y) return fail(FEM_ERR_ARG, "null argument");
+  if (x == y) return fail(FEM_ERR_ARG, "operator action requires distinct input and output storage");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (s->P == 0) return FEM_OK;
+  RowArgs ra = s->row_args();
+  return dispatch_g<LaunchApplyOp>(s->elem_g, ra, (const float*)s->b_vals, x, y, (hipStream_t)stream);
+}
+
+int fem_apply_preconditioner(fem_system* s, const float* x, float* y, void* stream) {
+  if (!s || !x || !y) return fail(FEM_ERR_ARG, "null argument");
+  if (x == y) return fail(FEM_ERR_ARG, "preconditioner action requires distinct input and output storage");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (s->P == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_apply_minv, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
+                     (const float*)s->b_minv, x, y, s->P);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_diagonal_blocks(fem_system* s, float* out, void* stream) {
+  if (!s || !out) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (s->P == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_diag_blocks, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
+                     s->row_args(), (const float*)s->b_vals, out);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_add_load(fem_system* s, const float* base, const float* ext, float dt, float* out, void* stream) {
+  if (!s || !base || !out) return fail(FEM_ERR_ARG, "null argument");
+  if (ext && ext == out) return fail(FEM_ERR_ARG, "the output must not alias the external-force input");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_add_load, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, base, ext,
+                     dt, (const unsigned char*)s->d_row_fixed, out, s->P);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_solve_increment(fem_system* s, const float* b, float* dv, void* stream) {
+  if (!s || !b || !dv) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (s->P == 0) return FEM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int nz = std::max(s->ninst * RHS_MAX_SLOTS, 1);
+  hipLaunchKernelGGL(k_prepare, dim3((nz + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const int*)nullptr, 0,
+                     (const float*)nullptr, (float*)nullptr, (float*)nullptr, s->d_rhs_max, s->ninst);
+  hipLaunchKernelGGL(k_absmax, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b, s->P,
+                     s->ninst > 1 ? (const int*)s->d_row_inst : (const int*)nullptr, s->d_rhs_max);
+  HIP_TRY(hipGetLastError());
+  CgArgs ca = cg_args(s);
+  ca.rhs = b;
+  ca.dv = dv;
+  return launch_cg(s, ca, st);
+}
+
+int fem_reconstruct(fem_system* s, const float* dv, const float* q_in, const float* qd_in, float* q_out,
+                    float* qd_out, float dt, void* stream) {
+  if (!s || !dv || !q_in || !qd_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
+  if (q_out == q_in || qd_out == qd_in) return fail(FEM_ERR_ARG, "reconstruction must not overwrite its input state");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_reconstruct, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, dv,
+                     q_in, qd_in, (const unsigned char*)s->d_row_fixed, (const float*)s->d_anchor, dt, q_out, qd_out,
+                     s->P);
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }

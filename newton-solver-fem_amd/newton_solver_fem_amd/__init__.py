@@ -27,6 +27,7 @@ __all__ = [
     "LinearFEMData",
     "Model",
     "ModelBuilder",
+    "PreparedFEMStep",
     "State",
     "SolverFEM",
     "SolverLinearFEM",
@@ -42,4 +43,8 @@ def __getattr__(name):
         from .solver import SolverFEM
 
         return SolverFEM
+    if name == "PreparedFEMStep":
+        from .prepared_step import PreparedFEMStep
+
+        return PreparedFEMStep
     raise AttributeError(name)

@@ -93,6 +93,12 @@ SIGNATURES = {
                                  ctypes.c_int32, ctypes.c_void_p]),
     "fem_vjp": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6 + [ctypes.c_float] + [ctypes.c_void_p] * 5),
     "fem_set_tet_materials": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fem_apply_operator": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "fem_apply_preconditioner": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "fem_diagonal_blocks": (ctypes.c_int, [ctypes.c_void_p] * 3),
+    "fem_add_load": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
+    "fem_solve_increment": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "fem_reconstruct": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_diff_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int32]),
 }

@@ -272,10 +272,13 @@ class SolverFEM:
         p.enable_tet_activations = 1 if self.enable_tet_activations else 0
         p.tolerance = self._tolerance
         p.maxiter = self._maxiter
-        p.early_exit = 1 if self.linear_solver_use_cuda_graph else 0
+        p.early_exit = 1 if self._early_exit() else 0
         self._params = p
         self._gravity_pushed = tuple(self.model.gravity)
         _lib.check(self._lib.fem_set_params(self._sys, ctypes.byref(p)), "fem_set_params")
+
+    def _params_ref(self):
+        return ctypes.byref(self._params)
 
     def __del__(self):
         sys = getattr(self, "_sys", None)
@@ -442,8 +445,15 @@ class SolverFEM:
         self.refresh_tangent(particle_q, dt)
         self.dv.zero_()
 
+    def _early_exit(self) -> bool:
+        # Device-side residual exit: the reference stops early with the CUDA
+        # graph's conditional loop (linear_solver_use_cuda_graph) or with host
+        # checks every linear_solver_check_every iterations (solver.py:1203-1217);
+        # otherwise it runs the full budget with residual-masked updates.
+        return self.linear_solver_use_cuda_graph or self.linear_solver_check_every > 0
+
     def _sync_params(self):
-        if tuple(self.model.gravity) != self._gravity_pushed:
+        if tuple(self.model.gravity) != self._gravity_pushed or (self._params.early_exit != 0) != self._early_exit():
             self._push_params()
 
     def step(self, state_in: State, state_out: State, control: Control | None, contacts, dt: float, *,
@@ -485,9 +495,31 @@ class SolverFEM:
             "fem_step",
         )
         if self.linear_solver_check_every > 0:
-            stats = self.solve_stats()
-            self.last_cg_iters = int(stats["iterations"][0])
+            # Host-check path (solver.py:1333-1335): the reference tests the
+            # residual every check_every iterations, so it reports the first
+            # multiple of check_every at or after convergence (capped at maxiter).
+            it = int(self.solve_stats()["iterations"][0])
+            n = int(self.linear_solver_check_every)
+            self.last_cg_iters = min(self.linear_solver_maxiter, -(-it // n) * n)
         return state_out
+
+    def prepare_step(self, state_in: State, control: Control | None, contacts, dt: float, *,
+                     external_forces: torch.Tensor | None = None):
+        """Prepare reusable affine mechanics without integrating physical state
+        (reference solver.py:483-500).  Returns the solver's persistent
+        :class:`PreparedFEMStep`; the caller owns the tangent refresh and must
+        prepare again for every new accepted state, control or matrix revision.
+        """
+        from .prepared_step import PreparedFEMStep
+
+        PreparedFEMStep.check_supported(self)
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("Prepare FEM mechanics before CUDA graph capture")
+        if contacts is not None and hasattr(contacts, "soft_contact_count"):
+            raise NotImplementedError("soft-rigid contacts are not part of this substep path")
+        if not hasattr(self, "_prepared_step"):
+            self._prepared_step = PreparedFEMStep(self)
+        return self._prepared_step.prepare(state_in, control, contacts, dt, external_forces)
 
     def solve_stats(self) -> dict:
         """Per-instance statistics of the last solve (synchronizes)."""

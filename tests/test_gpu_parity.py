@@ -269,6 +269,32 @@ def test_true_residual_within_tolerance():
         assert res <= tol * 1.01, (tol, res, s.last_cg_iters)
 
 
+def test_iteration_count_semantics():
+    """linear_solver_check_every > 0 (solver.py:1203-1217, 1333-1335): early exit, and
+    last_cg_iters is the first multiple of check_every at or after convergence; with
+    check_every = 0 and no CUDA-graph exit the full budget runs (residual-masked),
+    giving the same increment."""
+    print("case iteration counts", flush=True)
+    model = cube("tet", device=DEV, gravity=(0, 0, -9.81), material=(500.0, 1000.0, 0.1))
+    rng = np.random.default_rng(14)
+    ext = torch.tensor(rng.normal(0, 0.03, (model.particle_count, 3)), dtype=torch.float32, device=DEV)
+    out = {}
+    for every, graph in ((1, False), (4, False), (0, True), (0, False)):
+        s = solver_for(model, dt=0.002, linear_solver_maxiter=64, linear_solver_tolerance=1e-5,
+                       linear_solver_use_cuda_graph=graph)
+        s.linear_solver_check_every = every
+        st0, st1 = model.state(), model.state()
+        s.step(st0, st1, None, None, 0.002, external_forces=ext)
+        out[(every, graph)] = (s.last_cg_iters, s.solve_stats()["iterations"][0], np_(s.dv))
+    exact = out[(1, False)][0]
+    assert 0 < exact < 64 and out[(1, False)][1] == exact
+    assert out[(4, False)][0] == -(-exact // 4) * 4
+    assert out[(0, True)][1] == exact and out[(0, True)][0] is None
+    assert out[(0, False)][1] == exact  # converged updates masked: counts the live iterations
+    for key in out:
+        np.testing.assert_array_equal(out[key][2], out[(1, False)][2])
+
+
 def test_drop_rollout_positions():
     """Tet4 drop (10^3 cells x 5 tets): 60 frames x 10 substeps vs the FP64 oracle."""
     print("case drop rollout", flush=True)
