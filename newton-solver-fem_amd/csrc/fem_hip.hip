@@ -99,7 +99,7 @@ struct RowArgs {
   const int* tri_idx;    // [R][3]
   const float* tri_pose; // [R][4]
   const float* tri_mat;  // [R][5]
-  const float2* slot_spring;  // per BSR slot (sum ke, sum kd) of its springs (row sums on the diagonal)
+  const float2* slot_spring;  // per BSR slot signed (sum ke, sum kd) of the spring triplets on it
   const float* row_tri;       // per row sum of (k_mu + k_lambda) * rest area of its triangles
   float mn, mx;
   int has_ground;
@@ -391,9 +391,8 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
       // h^2 (k_mu + k_lambda) A / 3 (kernels.py:972-1030).
       float cst = 0.f;
       if (a.slot_spring) {
-        const float2 sk = a.slot_spring[kb + kk];
-        const float c = h * sk.y + h2 * sk.x;
-        cst = diag ? c : -c;
+        const float2 sk = a.slot_spring[kb + kk];  // signed sums
+        cst = h * sk.y + h2 * sk.x;
       }
       if (diag && a.row_tri) cst += h2 * a.row_tri[r0 + lr] / 3.f;
       acc[0] += cst;
@@ -2257,10 +2256,14 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
                          s->R, d->tri_indices, s->pt, err);
   if (rc) return cleanup(fail(rc, err));
   if (s->S > 0 || s->R > 0) {
-    // Constant blocks: per slot the (ke, kd) sums of its springs (the
-    // diagonal slot holds the row's sums), per row the triangles'
-    // (k_mu + k_lambda) * rest area; degenerate triangles contribute nothing
-    // (build_system_matrix_tri_kernel kernels.py:994-1008).
+    // Constant blocks: per slot the signed (ke, kd) sums of the spring
+    // triplets landing on it -- +1 per spring on its endpoints' diagonals
+    // (build_system_matrix_diagonal_kernel kernels.py:294-323 counts a spring
+    // once per endpoint row), -1 per (i, j) and (j, i) block
+    // (build_system_matrix_sparse_kernel :326-366), so a spring whose two
+    // endpoints coincide nets -1 on its diagonal as in the reference -- and
+    // per row the triangles' (k_mu + k_lambda) * rest area; degenerate
+    // triangles contribute nothing (build_system_matrix_tri_kernel :994-1008).
     std::vector<float2> slot_spring(s->S > 0 ? s->pt.columns.size() : 0, make_float2(0.f, 0.f));
     std::vector<float> row_tri(s->R > 0 ? P : 0, 0.f);
     auto slot_of = [&](int r, int c) {
@@ -2268,12 +2271,23 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       return (size_t)(s->pt.offsets[r] + (std::lower_bound(row, row + (s->pt.offsets[r + 1] - s->pt.offsets[r]), c) - row));
     };
     for (int k = 0; k < s->S; ++k) {
+      // A spring with a -1 endpoint keeps its diagonal term at the other
+      // endpoint (the diagonal kernel tests each endpoint); its off-diagonal
+      // triplets carry row or column -1 and are dropped.
       const int p = d->spring_indices[2 * k], q = d->spring_indices[2 * k + 1];
-      if (p < 0 || q < 0) continue;
       const float ke = d->spring_stiffness[k], kd = d->spring_damping[k];
-      for (size_t sl : {slot_of(p, q), slot_of(q, p), slot_of(p, p), slot_of(q, q)}) {
-        slot_spring[sl].x += ke;
-        slot_spring[sl].y += kd;
+      if (p >= 0 && q >= 0)
+        for (size_t sl : {slot_of(p, q), slot_of(q, p)}) {
+          slot_spring[sl].x -= ke;
+          slot_spring[sl].y -= kd;
+        }
+      if (p >= 0) {
+        slot_spring[slot_of(p, p)].x += ke;
+        slot_spring[slot_of(p, p)].y += kd;
+      }
+      if (q >= 0 && q != p) {
+        slot_spring[slot_of(q, q)].x += ke;
+        slot_spring[slot_of(q, q)].y += kd;
       }
     }
     for (int t = 0; t < s->R; ++t) {

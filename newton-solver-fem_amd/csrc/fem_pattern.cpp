@@ -35,11 +35,11 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
   pt.rinc_ptr.assign(P + 1, 0);
   for (int s = 0; s < S; ++s) {
     const int32_t* e = spring_idx + 2 * (size_t)s;
-    if (!spring_valid(e)) continue;
-    if (e[0] >= P || e[1] >= P) {
+    if (e[0] < -1 || e[1] < -1 || e[0] >= P || e[1] >= P) {
       err = "spring_indices contains an out-of-range particle";
       return 1;
     }
+    if (!spring_valid(e)) continue;
     pt.sinc_ptr[e[0] + 1]++;
     pt.sinc_ptr[e[1] + 1]++;
   }

@@ -1,0 +1,77 @@
+"""Edge cases on cuda:0 vs the FP64 oracle: a model without elements
+(free fall, diagonal-only pattern), tets with all-zero materials (left out of
+the pattern, kernels.py:679-692), springs with a -1 endpoint (ignored) and a
+spring whose endpoints coincide (its blocks net -(h kd + h^2 ke) on the
+diagonal, as the reference's triplet sum does)."""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import brute_pattern
+from newton_solver_fem_amd.model import FEMBuilder, ModelBuilder
+from test_gpu_parity import DEV, np_, oracle_for, rel, solver_for
+
+pytestmark = pytest.mark.gpu
+
+
+def test_particles_without_elements_fall_freely():
+    print("case free particles", flush=True)
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    rng = np.random.default_rng(1)
+    b.add_particles(rng.uniform(0, 1, (37, 3)) + [0, 0, 1], rng.normal(0, 0.1, (37, 3)),
+                    rng.uniform(0.01, 2.0, 37), radius=0.001)
+    model = b.finalize(DEV)
+    s = solver_for(model, dt=0.002)
+    assert s.A_bsr.nnz == 37
+    st0, st1 = model.state(), model.state()
+    s.refresh_tangent(st0.particle_q)
+    s.step(st0, st1, None, None, 0.002)
+    np.testing.assert_allclose(np_(s.dv), np.tile([0, 0, -9.81 * 0.002], (37, 1)), rtol=1e-6, atol=2e-8)
+
+
+def test_zero_material_tets_and_degenerate_springs_match_oracle():
+    print("case zero materials / degenerate springs", flush=True)
+    b = FEMBuilder(gravity=(0.0, 0.0, -0.5))
+    corners = np.array(((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)))
+    for p in 0.1 * corners:
+        b.add_particle(p, (0, 0, 0), mass=0.001, radius=0.001)
+    b.add_tetrahedron(0, 1, 3, 4, k_mu=500, k_lambda=1000, k_damp=0.02)
+    b.add_tetrahedron(1, 2, 3, 6, k_mu=0.0, k_lambda=0.0, k_damp=0.0)  # left out of the pattern
+    b.add_tetrahedron(1, 4, 5, 6, k_mu=300, k_lambda=800, k_damp=0.0)
+    b.add_spring(2, 7, 40.0, 0.2)
+    b.add_spring(5, 5, 25.0, 0.1)  # coincident endpoints
+    b._spring_idx.append((-1, 3))  # ignored everywhere
+    b._spring_rest.append(0.1)
+    b._spring_ke.append(10.0)
+    b._spring_kd.append(0.0)
+    b._spring_control.append(0.0)
+    model = b.finalize(DEV)
+    kw = dict(dt=0.002, linear_solver_tolerance=1e-9, linear_solver_maxiter=200, fixed_particle_indices=(0,))
+    s = solver_for(model, **kw)
+    ref = oracle_for(model, fixed=(0,), dt=0.002, linear_solver_tolerance=1e-12, linear_solver_maxiter=400)
+    off, cols, _, _ = ref.matrix()
+    np.testing.assert_array_equal(s.A_bsr.offsets.cpu().numpy(), off)
+    np.testing.assert_array_equal(s.A_bsr.columns.cpu().numpy(), cols)
+    bo, bc = brute_pattern(8, tets=model.host["tet_indices"], tet_materials=model.host["tet_materials"])
+    pairs = set(zip(np.repeat(np.arange(8), np.diff(bo)), bc)) | {(2, 7), (7, 2)}
+    assert len(pairs) == len(cols)
+    rng = np.random.default_rng(3)
+    st0, st1 = model.state(), model.state()
+    st0.particle_q.add_(torch.tensor(rng.normal(0, 0.002, (8, 3)), dtype=torch.float32, device=DEV))
+    st0.particle_qd.copy_(torch.tensor(rng.normal(0, 0.05, (8, 3)), dtype=torch.float32))
+    q, v = np_(st0.particle_q), np_(st0.particle_qd)
+    s.refresh_tangent(st0.particle_q)
+    qf = q.copy()
+    qf[0] = model.host["particle_q"][0]
+    ref.refresh_tangent(qf)
+    _, _, vals, minv = ref.matrix()
+    assert rel(np_(s.A_bsr.values), vals) < 2e-6
+    assert rel(np_(s._pc_inv_diag), minv) < 2e-5
+    s.step(st0, st1, None, None, 0.002)
+    vf = v.copy()
+    vf[0] = 0
+    out = ref.step(qf, vf)
+    assert rel(np_(st0.particle_f), out["f"]) < 1e-5
+    np.testing.assert_allclose(np_(st1.particle_qd), out["qd"], rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(np_(st1.particle_q), out["q"], rtol=1e-6, atol=1e-9)
