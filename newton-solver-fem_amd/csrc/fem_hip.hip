@@ -632,6 +632,33 @@ __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __re
         vb = vb + hv;
       }
     }
+    if ((EL & 4) && a.sinc_ptr) {
+      // Reverse of eval_springs for this endpoint: L = sum_s w . fs with
+      // w = h (z_q - z_p), fs = dir (ke (len - rest) + kd dir . v_pq); and of
+      // the spring-damping predictor restore, h kd (z_p - z_q) . (v_p - v_q).
+      const int s1 = a.sinc_ptr[i + 1];
+      for (int t = a.sinc_ptr[i] + lane; t < s1; t += G) {
+        const int code = a.sinc[t];
+        const int sp = code >> 1;
+        const float sg = (code & 1) ? -1.f : 1.f;  // +1 at the first endpoint p, -1 at q
+        const int p = a.sp_idx[2 * sp], q = a.sp_idx[2 * sp + 1];
+        const float kd = a.sp_kd[sp];
+        const v3 zp = ld3(z, p), zq = ld3(z, q);
+        vb = vb + (zp - zq) * (sg * h * kd);
+        const v3 xpq = ld3(x, p) - ld3(x, q), vpq = ld3(v, p) - ld3(v, q);
+        const float len = sqrtf(dot(xpq, xpq));
+        if (len < 1.0e-6f) continue;
+        const v3 d = xpq * (1.f / len);
+        const float ke = a.sp_ke[sp];
+        const float dvel = dot(d, vpq);
+        const float phi = ke * (len - a.sp_rest[sp]) + kd * dvel;
+        const v3 w = (zq - zp) * h;
+        const float dw = dot(d, w);
+        const v3 gx = (w - d * dw) * (phi / len) + (d * ke + (vpq - d * dvel) * (kd / len)) * dw;
+        xb = xb + gx * sg;
+        vb = vb + d * (sg * dw * kd);
+      }
+    }
     const int k1 = a.offsets[i + 1];
     for (int k = a.offsets[i] + lane; k < k1; k += G) {
       const float* b = vals + 9 * (size_t)k;
@@ -1792,6 +1819,7 @@ struct fem_system {
   float2* d_slot_spring = nullptr;
   // Kernels with Hex8, spring or triangle code (EL_ANY) are needed.
   bool full() const { return H > 0 || S > 0 || R > 0; }
+  bool tri_active = false;  // a triangle with a non-zero material (no reverse products for those)
   // solve scratch
   float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_cgflags = nullptr;
@@ -2291,6 +2319,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       }
     }
     for (int t = 0; t < s->R; ++t) {
+      for (int m = 0; m < 5; ++m) s->tri_active = s->tri_active || d->tri_materials[5 * (size_t)t + m] != 0.f;
       const float* D = d->tri_poses + 4 * (size_t)t;
       const float inv_area = (D[0] * D[3] - D[1] * D[2]) * 2.f;
       if (!(inv_area > 0.f)) continue;
@@ -2704,12 +2733,12 @@ int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, co
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (mat_bar && (!dv || !q_asm)) return fail(FEM_ERR_ARG, "material gradients need dv and the assembly positions");
-  if (s->S > 0 || s->R > 0)
-    return fail(FEM_ERR_UNSUPPORTED, "reverse products of springs and triangle membranes are not provided");
+  if (s->tri_active)
+    return fail(FEM_ERR_UNSUPPORTED, "reverse products of triangle membranes are not provided");
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   RowArgs ra = s->row_args();
-  int rc = dispatch_g<LaunchVjpRows>(s->elem_g, s->H > 0, ra, q, qd, act, z, (const float*)s->b_vals, dt, q_bar, qd_bar, st);
+  int rc = dispatch_g<LaunchVjpRows>(s->elem_g, s->full(), ra, q, qd, act, z, (const float*)s->b_vals, dt, q_bar, qd_bar, st);
   if (rc) return rc;
   if (s->T > 0 && (act_bar || mat_bar)) {
     hipLaunchKernelGGL(k_vjp_tets, dim3((s->T + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ra, s->T, q, qd, act, z, dv,

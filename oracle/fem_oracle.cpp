@@ -1235,8 +1235,31 @@ struct Solver {
   }
 
   // VJP of eval_terms(mask) with per-node weights zw.
+  // Reverse of eval_springs: L = sum_i zw_i . f_i = sum_s (zw_q - zw_p) . fs,
+  // fs = dir (ke (len - rest) + kd dir . v_pq), dir = x_pq / len.
+  void vjp_springs(const v3* x, const v3* v, const v3* zw, v3* xb, v3* vb) const {
+    for (int s = 0; s < S; ++s) {
+      int p = spring_idx[2 * s], q = spring_idx[2 * s + 1];
+      if (p == -1 || q == -1) continue;
+      v3 xpq = x[p] - x[q], vpq = v[p] - v[q];
+      Real len = std::sqrt(dot(xpq, xpq));
+      if (len < Real(1e-6)) continue;
+      v3 d = xpq * (Real(1) / len);
+      Real phi = spring_ke[s] * (len - spring_rest[s]) + spring_kd[s] * dot(d, vpq);
+      v3 w = zw[q] - zw[p];
+      Real dw = dot(d, w), dvel = dot(d, vpq);
+      v3 gx = (w - d * dw) * (phi / len) + (d * spring_ke[s] + (vpq - d * dvel) * (spring_kd[s] / len)) * dw;
+      v3 gv = d * (dw * spring_kd[s]);
+      xb[p] = xb[p] + gx;
+      xb[q] = xb[q] - gx;
+      vb[p] = vb[p] + gv;
+      vb[q] = vb[q] - gv;
+    }
+  }
+
   void vjp_terms(int mask, const v3* x, const v3* v, const Real* act, const v3* zw, v3* xb, v3* vb, Real* actb,
                  Real* matb) const {
+    if ((mask & TERM_SPRING) && S) vjp_springs(x, v, zw, xb, vb);
     if ((mask & TERM_TET) && T) vjp_tets(x, v, (prm.enable_activations ? act : nullptr), zw, xb, vb, actb, matb);
     if ((mask & TERM_HEX) && H) vjp_hexes(x, v, zw, xb, vb);
     if (mask & TERM_GROUND) vjp_ground(x, v, zw, xb, vb);
@@ -1303,10 +1326,11 @@ struct Solver {
     const v3* qd = vr.data();
     std::vector<v3> f(P);
     forces_scaled(q, qd, ext, act, dt, f.data());
-    if (T || H) {
+    if (T || H || S) {
       std::vector<v3> c(P);
       predictor(qd, c.data());
       for (int i = 0; i < P; ++i) f[i] = f[i] - c[i];
+      spring_restore(qd, dt, f.data());
     }
     for (int i : fixed_idx) f[i] = v3{0, 0, 0};
     std::vector<v3> x(P, v3{0, 0, 0});
@@ -1325,6 +1349,17 @@ struct Solver {
     for (int i : fixed_idx) {
       q_out[i] = anchor[i];
       qd_out[i] = v3{0, 0, 0};
+    }
+  }
+  // restore_spring_damping_predictor implicit_predictor.py:26-41
+  void spring_restore(const v3* qd, Real dt, v3* f) const {
+    for (int s = 0; s < S; ++s) {
+      int a = spring_idx[2 * s], b = spring_idx[2 * s + 1];
+      if (a >= 0 && b >= 0) {
+        v3 corr = (qd[a] - qd[b]) * (dt * spring_kd[s]);
+        f[a] = f[a] + corr;
+        f[b] = f[b] - corr;
+      }
     }
   }
   void predictor(const v3* qd, v3* c) const {
@@ -1365,8 +1400,17 @@ struct Solver {
       vb_in[i] = u[i];
       extb[i] = z[i] * dt;
     }
-    vjp_terms(TERM_TET | TERM_HEX | TERM_GROUND | TERM_DAMPING, q, qd, act, zw.data(), qb_in, vb_in, actb, matb);
-    if (T || H) {  // predictor: vb_j += -(A z)_j + (sum_k A_jk)^T z_j
+    vjp_terms(TERM_SPRING | TERM_TET | TERM_HEX | TERM_GROUND | TERM_DAMPING, q, qd, act, zw.data(), qb_in, vb_in,
+              actb, matb);
+    for (int s = 0; s < S; ++s) {  // spring-damping restore: dt kd (z_p - z_q) . (v_p - v_q)
+      int a = spring_idx[2 * s], b = spring_idx[2 * s + 1];
+      if (a >= 0 && b >= 0) {
+        v3 g = (z[a] - z[b]) * (dt * spring_kd[s]);
+        vb_in[a] = vb_in[a] + g;
+        vb_in[b] = vb_in[b] - g;
+      }
+    }
+    if (T || H || S) {  // predictor: vb_j += -(A z)_j + (sum_k A_jk)^T z_j
       for (int i = 0; i < P; ++i) {
         Real az[3] = {0, 0, 0}, rs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int k = offsets[i]; k < offsets[i + 1]; ++k) {

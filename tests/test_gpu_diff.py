@@ -232,3 +232,46 @@ def test_bar_fixed_base_mu_activation_gradient_matches_oracle():
     fd = (oracle_loss(a0 + eps * d) - oracle_loss(a0 - eps * d)) / (2 * eps)
     exact = float(np.sum(np_(a.grad) * d))
     assert abs(exact - fd) <= 1e-3 * abs(fd) + 1e-12, (exact, fd)
+
+
+def test_step_gradients_with_springs_match_oracle():
+    """Tet4 cube + damped edge springs + zero-material surface triangles (the
+    reference fixture's triangles carry no stiffness): one differentiable step
+    and its input gradients vs the oracle's reverse products."""
+    print("case diff springs", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    base = cube("tet", material=(500.0, 1000.0, 0.2), gravity=(0.0, 0.0, -9.81))
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    b.add_particles(base.host["particle_q"], np.zeros((base.particle_count, 3)), base.host["particle_mass"],
+                    radius=0.004)
+    b.add_tetrahedra(base.host["tet_indices"], base.host["tet_materials"])
+    tets = base.host["tet_indices"].tolist()
+    edges = sorted({(min(a, c), max(a, c)) for t in tets for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[3]))})
+    b.add_springs(edges, 30.0, 0.4)
+    b.add_triangles([t[:3] for t in tets[:6]], tri_ke=0.0, tri_ka=0.0, tri_kd=0.0)
+    model = b.finalize(DEV)
+    kw = dict(dt=DT, min_stretch=-1.0, linear_damping=0.3, **GROUND)
+    s = SolverFEM(model, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0, differentiable=True,
+                  linear_solver_maxiter=128, **kw)
+    o = oracle.OracleSolver(model.host, bits=64, gravity=model.gravity, **kw)
+    rng, q0, v0 = initial(model, 5)
+    ext = rng.normal(0, 0.02, q0.shape)
+    qo, vo, dv = o.diff_step(q0, v0, ext)
+    gq_out, gv_out = rng.normal(size=q0.shape), rng.normal(size=q0.shape)
+    ref = o.diff_step_backward(q0, v0, dv, gq_out, gv_out)
+    tq = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    tv = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    te = torch.tensor(ext, dtype=torch.float32, device=DEV, requires_grad=True)
+    st_in, st_out = model.state(), model.state()
+    st_in.particle_q, st_in.particle_qd = tq, tv
+    s.step(st_in, st_out, None, None, DT, external_forces=te)
+    assert rel(np_(st_out.particle_q), qo) < 1e-6
+    assert rel(np_(st_out.particle_qd), vo) < 1e-4
+    loss = (st_out.particle_q * torch.tensor(gq_out, dtype=torch.float32, device=DEV)).sum() + (
+        st_out.particle_qd * torch.tensor(gv_out, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    u = gv_out + DT * gq_out
+    assert rel(np_(tq.grad) - gq_out, ref["q"] - gq_out) < 2e-3
+    assert rel(np_(tv.grad) - u, ref["qd"] - u) < 2e-3
+    assert rel(np_(te.grad), ref["ext"]) < 1e-3
