@@ -250,9 +250,10 @@ int fem_solve(fem_system* sys, const float* b_dev, float* x_dev, float tolerance
  *   mat_bar[e] = d/d(mu, lambda, k_damp)_e of  h z^T terms  - z^T (dA/d(mu, lambda))(dv + qd),
  *     with A assembled at q_asm (written if not NULL; needs dv and q_asm).
  * The reference has no material gradients (tet_materials must not require
- * gradients, differentiation.py:244-262); mat_bar extends it.  Springs are
- * differentiated (force and spring-damping predictor restore); models with a
- * triangle membrane of non-zero material return FEM_ERR_UNSUPPORTED. */
+ * gradients, differentiation.py:244-262); mat_bar extends it.  Springs
+ * (force and spring-damping predictor restore) and triangle membranes are
+ * differentiated; models with triangle drag or lift return
+ * FEM_ERR_UNSUPPORTED. */
 int fem_vjp(fem_system* sys, const float* q_dev, const float* qd_dev, const float* act_dev, const float* z_dev,
             const float* dv_dev, const float* q_asm_dev, float dt, float* q_bar_dev, float* qd_bar_dev,
             float* act_bar_dev, float* mat_bar_dev, void* stream);

@@ -659,6 +659,29 @@ __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __re
         vb = vb + d * (sg * dw * kd);
       }
     }
+    if ((EL & 4) && a.rinc_ptr) {
+      // Reverse of the triangle membranes (weights h z of the corner forces).
+      const int r1 = a.rinc_ptr[i + 1];
+      for (int t = a.rinc_ptr[i] + lane; t < r1; t += G) {
+        const int code = a.rinc[t];
+        const int e = code >> 2, c = code & 3;
+        const int n0 = a.tri_idx[3 * e], n1 = a.tri_idx[3 * e + 1], n2 = a.tri_idx[3 * e + 2];
+        v3 gx10, gx20, gv10, gv20;
+        tri_edge_vjp(ld3(x, n0), ld3(x, n1), ld3(x, n2), ld3(v, n0), ld3(v, n1), ld3(v, n2), ld3(z, n0) * h,
+                     ld3(z, n1) * h, ld3(z, n2) * h, a.tri_pose + 4 * (size_t)e, a.tri_mat + 5 * (size_t)e, gx10, gx20,
+                     gv10, gv20);
+        if (c == 0) {
+          xb = xb - gx10 - gx20;
+          vb = vb - gv10 - gv20;
+        } else if (c == 1) {
+          xb = xb + gx10;
+          vb = vb + gv10;
+        } else {
+          xb = xb + gx20;
+          vb = vb + gv20;
+        }
+      }
+    }
     const int k1 = a.offsets[i + 1];
     for (int k = a.offsets[i] + lane; k < k1; k += G) {
       const float* b = vals + 9 * (size_t)k;
@@ -1819,7 +1842,7 @@ struct fem_system {
   float2* d_slot_spring = nullptr;
   // Kernels with Hex8, spring or triangle code (EL_ANY) are needed.
   bool full() const { return H > 0 || S > 0 || R > 0; }
-  bool tri_active = false;  // a triangle with a non-zero material (no reverse products for those)
+  bool tri_aero = false;  // a triangle with drag or lift (no reverse products for those)
   // solve scratch
   float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_cgflags = nullptr;
@@ -2319,7 +2342,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       }
     }
     for (int t = 0; t < s->R; ++t) {
-      for (int m = 0; m < 5; ++m) s->tri_active = s->tri_active || d->tri_materials[5 * (size_t)t + m] != 0.f;
+      s->tri_aero = s->tri_aero || d->tri_materials[5 * (size_t)t + 3] != 0.f || d->tri_materials[5 * (size_t)t + 4] != 0.f;
       const float* D = d->tri_poses + 4 * (size_t)t;
       const float inv_area = (D[0] * D[3] - D[1] * D[2]) * 2.f;
       if (!(inv_area > 0.f)) continue;
@@ -2733,8 +2756,8 @@ int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, co
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (mat_bar && (!dv || !q_asm)) return fail(FEM_ERR_ARG, "material gradients need dv and the assembly positions");
-  if (s->tri_active)
-    return fail(FEM_ERR_UNSUPPORTED, "reverse products of triangle membranes are not provided");
+  if (s->tri_aero)
+    return fail(FEM_ERR_UNSUPPORTED, "reverse products of triangle drag and lift are not provided");
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   RowArgs ra = s->row_args();

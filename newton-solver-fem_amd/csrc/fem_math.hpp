@@ -342,6 +342,50 @@ FEM_DI v3 tri_corner_force(v3 x0, v3 x1, v3 x2, v3 v0, v3 v1, v3 v2, const float
   return c == 0 ? f0 : (c == 1 ? f1 * -1.f : f2 * -1.f);
 }
 
+// Reverse of tri_corner_force without drag and lift: for weights w_c of the
+// corner forces (L = sum_c w_c . F_c), the gradients with respect to the edge
+// vectors x10, x20 and their velocities v10, v20 (node 0 receives minus their
+// sum).  With a1 = w0 - w1, a2 = w0 - w2: L = <P, [a1 a2] D> + g (a1 . dc/dx10
+// + a2 . dc/dx20), P = k_mu (1 - 2/Ic) F + k_damp dF, g = k_lambda c + k_damp dc/dt.
+FEM_DI void tri_edge_vjp(v3 x0, v3 x1, v3 x2, v3 v0, v3 v1, v3 v2, v3 w0, v3 w1, v3 w2,
+                         const float* __restrict__ pose, const float* __restrict__ mat, v3& gx10, v3& gx20, v3& gv10,
+                         v3& gv20) {
+  const float D00 = pose[0], D01 = pose[1], D10 = pose[2], D11 = pose[3];
+  const float inv_area = (D00 * D11 - D01 * D10) * 2.f;
+  const float A0 = 1.f / inv_area;
+  const float kmu = mat[0] * A0, klam = mat[1] * A0, kd = mat[2] * A0;
+  const v3 x10 = x1 - x0, x20 = x2 - x0, v10 = v1 - v0, v20 = v2 - v0;
+  const v3 a1 = w0 - w1, a2 = w0 - w2;
+  const v3 F1 = x10 * D00 + x20 * D10, F2 = x10 * D01 + x20 * D11;
+  const float Ic = dot(F1, F1) + dot(F2, F2);
+  const float dev = (Ic - 2.f) / Ic;
+  const v3 b1 = a1 * D00 + a2 * D10, b2 = a1 * D01 + a2 * D11;
+  const float cI = 4.f * (dot(F1, b1) + dot(F2, b2)) / (Ic * Ic);
+  const v3 G1 = (b1 * dev + F1 * cI) * kmu, G2 = (b2 * dev + F2 * cI) * kmu;
+  gx10 = G1 * D00 + G2 * D01;
+  gx20 = G1 * D10 + G2 * D11;
+  gv10 = (b1 * D00 + b2 * D01) * kd;
+  gv20 = (b1 * D10 + b2 * D11) * kd;
+  const v3 nr = cross(x10, x20);
+  const float ln = sqrtf(dot(nr, nr));
+  if (!(ln > 0.f)) return;
+  const v3 nh = nr * (1.f / ln);
+  const float kap = inv_area * 0.5f;
+  const v3 cq = cross(x20, nh) * kap, cr = cross(nh, x10) * kap;  // dc/dx10, dc/dx20
+  const float c = ln * kap - 1.f;
+  const float cdot = dot(cq, v10) + dot(cr, v20), hA = dot(cq, a1) + dot(cr, a2);
+  const float g = klam * c + kd * cdot;
+  // Gradients of dc/dx10 . y1 + dc/dx20 . y2 with respect to (x10, x20).
+  const v3 mv_ = cross(v10, x20) + cross(x10, v20), ma = cross(a1, x20) + cross(x10, a2);
+  const v3 uv = (mv_ - nh * dot(nh, mv_)) * (1.f / ln), ua = (ma - nh * dot(nh, ma)) * (1.f / ln);
+  const v3 dv1 = (cross(x20, uv) + cross(v20, nh)) * kap, dv2 = (cross(uv, x10) + cross(nh, v10)) * kap;
+  const v3 da1 = (cross(x20, ua) + cross(a2, nh)) * kap, da2 = (cross(ua, x10) + cross(nh, a1)) * kap;
+  gx10 = gx10 + (cq * klam + dv1 * kd) * hA + da1 * g;
+  gx20 = gx20 + (cr * klam + dv2 * kd) * hA + da2 * g;
+  gv10 = gv10 + cq * (kd * hA);
+  gv20 = gv20 + cr * (kd * hA);
+}
+
 // Volumetric activation force on node a (eval_tet_activations kernels.py:484-508).
 FEM_DI v3 tet_activation_force(v3 x0, v3 x1, v3 x2, v3 x3, float lam, float act, int a) {
   const float scale = lam * act / 6.f;

@@ -134,8 +134,8 @@ class SolverFEM:
                 "this substep implements solver_type='cg' with preconditioner_type='block_diag'; "
                 f"got solver_type={solver_type!r}, preconditioner_type={preconditioner_type!r}"
             )
-        if differentiable and model.tri_count and np.any(model.host["tri_materials"] != 0):
-            raise NotImplementedError("differentiable steps with triangle membranes of non-zero material are not supported")
+        if differentiable and model.tri_count and np.any(model.host["tri_materials"][:, 3:5] != 0):
+            raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
         if torque_stiffness > 0.0 and spring_rest_direction is not None:
             raise NotImplementedError("spring bend torque is not part of this substep path")
         if self_contact_ke > 0.0 and model.particle_count > 1 and float(np.max(model.host["particle_radius"], initial=0)) > 0:

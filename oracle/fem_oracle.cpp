@@ -1257,9 +1257,65 @@ struct Solver {
     }
   }
 
+  // Reverse of eval_triangles without drag and lift (callers reject those):
+  // L = sum_n zw_n . F_n with F_0 = f0, F_1 = -f1, F_2 = -f2.  With a1 = zw_0 - zw_1,
+  // a2 = zw_0 - zw_2: L = <P, [a1 a2] D> + g (a1 . dc/dx10 + a2 . dc/dx20),
+  // P = k_mu (1 - 2/Ic) F + k_damp dF, g = k_lambda c + k_damp dc/dt.
+  void vjp_tris(const v3* x, const v3* v, const v3* zw, v3* xb, v3* vb) const {
+    for (int t = 0; t < R; ++t) {
+      const int i = tri_idx[3 * t], j = tri_idx[3 * t + 1], k = tri_idx[3 * t + 2];
+      const Real* D = &tri_pose[4 * t];
+      const Real inv_area = (D[0] * D[3] - D[1] * D[2]) * Real(2);
+      const Real A0 = Real(1) / inv_area;
+      const Real kmu = tri_mat[5 * t] * A0, klam = tri_mat[5 * t + 1] * A0, kd = tri_mat[5 * t + 2] * A0;
+      const v3 x10 = x[j] - x[i], x20 = x[k] - x[i], v10 = v[j] - v[i], v20 = v[k] - v[i];
+      const v3 a1 = zw[i] - zw[j], a2 = zw[i] - zw[k];
+      const v3 F1 = x10 * D[0] + x20 * D[2], F2 = x10 * D[1] + x20 * D[3];
+      const Real Ic = dot(F1, F1) + dot(F2, F2);
+      const Real dev = (Ic - Real(2)) / Ic;
+      const v3 b1 = a1 * D[0] + a2 * D[2], b2 = a1 * D[1] + a2 * D[3];
+      const Real FB = dot(F1, b1) + dot(F2, b2);
+      const Real cI = Real(4) * FB / (Ic * Ic);
+      const v3 G1 = (b1 * dev + F1 * cI) * kmu, G2 = (b2 * dev + F2 * cI) * kmu;
+      v3 gx10 = G1 * D[0] + G2 * D[1], gx20 = G1 * D[2] + G2 * D[3];
+      v3 gv10 = (b1 * D[0] + b2 * D[1]) * kd, gv20 = (b1 * D[2] + b2 * D[3]) * kd;
+      const v3 nr = cross(x10, x20);
+      const Real ln = std::sqrt(dot(nr, nr));
+      if (ln > Real(0)) {
+        const v3 nh = nr * (Real(1) / ln);
+        const Real kap = inv_area * Real(0.5);
+        const v3 cq = cross(x20, nh) * kap, cr = cross(nh, x10) * kap;  // dc/dx10, dc/dx20
+        const Real c = ln * kap - Real(1);
+        const Real cdot = dot(cq, v10) + dot(cr, v20), hA = dot(cq, a1) + dot(cr, a2);
+        const Real g = klam * c + kd * cdot;
+        // gradient of Dc[Y] = dc/dx10 . y1 + dc/dx20 . y2 with respect to (x10, x20)
+        auto dgrad = [&](v3 y1, v3 y2, v3& o1, v3& o2) {
+          const v3 m = cross(y1, x20) + cross(x10, y2);
+          const v3 u = (m - nh * dot(nh, m)) * (Real(1) / ln);
+          o1 = (cross(x20, u) + cross(y2, nh)) * kap;
+          o2 = (cross(u, x10) + cross(nh, y1)) * kap;
+        };
+        v3 dv1, dv2, da1, da2;
+        dgrad(v10, v20, dv1, dv2);
+        dgrad(a1, a2, da1, da2);
+        gx10 = gx10 + (cq * klam + dv1 * kd) * hA + da1 * g;
+        gx20 = gx20 + (cr * klam + dv2 * kd) * hA + da2 * g;
+        gv10 = gv10 + cq * (kd * hA);
+        gv20 = gv20 + cr * (kd * hA);
+      }
+      xb[j] = xb[j] + gx10;
+      xb[k] = xb[k] + gx20;
+      xb[i] = xb[i] - gx10 - gx20;
+      vb[j] = vb[j] + gv10;
+      vb[k] = vb[k] + gv20;
+      vb[i] = vb[i] - gv10 - gv20;
+    }
+  }
+
   void vjp_terms(int mask, const v3* x, const v3* v, const Real* act, const v3* zw, v3* xb, v3* vb, Real* actb,
                  Real* matb) const {
     if ((mask & TERM_SPRING) && S) vjp_springs(x, v, zw, xb, vb);
+    if ((mask & TERM_TRI) && R) vjp_tris(x, v, zw, xb, vb);
     if ((mask & TERM_TET) && T) vjp_tets(x, v, (prm.enable_activations ? act : nullptr), zw, xb, vb, actb, matb);
     if ((mask & TERM_HEX) && H) vjp_hexes(x, v, zw, xb, vb);
     if (mask & TERM_GROUND) vjp_ground(x, v, zw, xb, vb);
@@ -1400,8 +1456,8 @@ struct Solver {
       vb_in[i] = u[i];
       extb[i] = z[i] * dt;
     }
-    vjp_terms(TERM_SPRING | TERM_TET | TERM_HEX | TERM_GROUND | TERM_DAMPING, q, qd, act, zw.data(), qb_in, vb_in,
-              actb, matb);
+    vjp_terms(TERM_SPRING | TERM_TET | TERM_HEX | TERM_TRI | TERM_GROUND | TERM_DAMPING, q, qd, act, zw.data(), qb_in,
+              vb_in, actb, matb);
     for (int s = 0; s < S; ++s) {  // spring-damping restore: dt kd (z_p - z_q) . (v_p - v_q)
       int a = spring_idx[2 * s], b = spring_idx[2 * s + 1];
       if (a >= 0 && b >= 0) {

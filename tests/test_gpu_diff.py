@@ -235,9 +235,8 @@ def test_bar_fixed_base_mu_activation_gradient_matches_oracle():
 
 
 def test_step_gradients_with_springs_match_oracle():
-    """Tet4 cube + damped edge springs + zero-material surface triangles (the
-    reference fixture's triangles carry no stiffness): one differentiable step
-    and its input gradients vs the oracle's reverse products."""
+    """Tet4 cube + damped edge springs + damped triangle membranes: one
+    differentiable step and its input gradients vs the oracle's reverse products."""
     print("case diff springs", flush=True)
     from newton_solver_fem_amd.solver import SolverFEM
 
@@ -249,7 +248,7 @@ def test_step_gradients_with_springs_match_oracle():
     tets = base.host["tet_indices"].tolist()
     edges = sorted({(min(a, c), max(a, c)) for t in tets for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[3]))})
     b.add_springs(edges, 30.0, 0.4)
-    b.add_triangles([t[:3] for t in tets[:6]], tri_ke=0.0, tri_ka=0.0, tri_kd=0.0)
+    b.add_triangles([t[:3] for t in tets[:6]], tri_ke=30.0, tri_ka=50.0, tri_kd=0.2)
     model = b.finalize(DEV)
     kw = dict(dt=DT, min_stretch=-1.0, linear_damping=0.3, **GROUND)
     s = SolverFEM(model, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0, differentiable=True,

@@ -181,3 +181,47 @@ def test_triangle_only_prepared_constant_load():
     solver.step(current, expected, None, None, DT, external_forces=load)
     np.testing.assert_array_equal(np_(prepared.base_rhs), np_(current.particle_f))
     np.testing.assert_allclose(np_(actual.particle_qd), np_(expected.particle_qd), rtol=1e-6, atol=1e-8)
+
+
+def test_prepared_actions_replay_in_a_hip_graph():
+    """rhs + solve + reconstruct recorded once into a HIP graph and replayed
+    (with eager work between replays) equal the eager sequence each time."""
+    print("case prepared graph", flush=True)
+    model, current, make = fixture()
+    solver = make()
+    solver.refresh_tangent(current.particle_q)
+    prepared = solver.prepare_step(current, None, None, DT)
+    ext = torch.tensor(np.linspace(-0.01, 0.02, 24).reshape(8, 3), dtype=torch.float32, device=DEV)
+    rhs = torch.zeros_like(ext)
+    guess = prepared.velocity_increment.clone()
+    result = model.state()
+
+    def body():
+        prepared.rhs(ext, out=rhs)
+        prepared.solve(rhs)
+        prepared.reconstruct(prepared.velocity_increment, result)
+
+    body()
+    torch.cuda.synchronize()
+    eager = (rhs.clone(), prepared.velocity_increment.clone(), result.particle_q.clone(), result.particle_qd.clone())
+    prepared.velocity_increment.copy_(guess)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        body()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    prepared.velocity_increment.copy_(guess)
+    with torch.cuda.graph(g):
+        body()
+    for k in range(3):
+        prepared.velocity_increment.copy_(guess)
+        rhs.zero_()
+        result.particle_q.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        got = (rhs, prepared.velocity_increment, result.particle_q, result.particle_qd)
+        for a, b in zip(got, eager):
+            assert torch.equal(a, b), k
+        solver.step(current, model.state(), None, None, DT)  # eager work between replays
+        torch.cuda.synchronize()

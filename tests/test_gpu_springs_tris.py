@@ -124,3 +124,37 @@ def test_soft_mesh_drop_rollout():
     print(f"soft mesh rollout position rel err {err:.2e}, min z {q[:, 2].min():.4f}", flush=True)
     assert err < 1e-4
     assert q[:, 2].min() < 0.001  # the lowest nodes are in ground contact
+
+
+def test_timestep_change_rules():
+    """Reference test_solver_timestep.py:117-164 on the mixed model: a
+    differentiable solver rejects a changed dt (matrix and dt unchanged), a
+    changed dt clears only the warm start, and invalid dt values are rejected."""
+    print("case timestep rules", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    model = on_device(mixed_model())
+    kw = dict(solver_type="cg", preconditioner_type="block_diag", linear_solver_maxiter=80,
+              linear_solver_tolerance=1e-5, self_contact_ke=0.0)
+    s = SolverFEM(model, dt=0.002, differentiable=True, **kw)
+    before = s.A_bsr.dense()
+    with pytest.raises(ValueError, match="Differentiable"):
+        s.refresh_tangent(model.particle_q, 0.001)
+    assert s.dt == 0.002
+    np.testing.assert_array_equal(s.A_bsr.dense(), before)
+    s = SolverFEM(model, dt=0.002, fixed_particle_indices=(0, 1), **kw)
+    state = model.state()
+    q0, v0 = state.particle_q.clone(), state.particle_qd.clone()
+    guess = torch.full((8, 3), 0.02, dtype=torch.float32, device=DEV)
+    s.velocity_increment.copy_(guess)
+    s.refresh_tangent(state.particle_q)
+    assert torch.equal(s.velocity_increment, guess)
+    s.refresh_tangent(state.particle_q, 0.001)
+    assert torch.equal(s.velocity_increment, torch.zeros_like(guess))
+    assert torch.equal(state.particle_q, q0) and torch.equal(state.particle_qd, v0)
+    before = s.A_bsr.dense()
+    for dt in (0.0, -1.0, float("nan"), float("inf")):
+        with pytest.raises(ValueError, match="finite and positive"):
+            s.refresh_tangent(model.particle_q, dt)
+        assert s.dt == 0.001
+        np.testing.assert_array_equal(s.A_bsr.dense(), before)

@@ -305,3 +305,31 @@ def test_differentiable_rollout_with_springs():
         return _rollout(o, q0, v0, e, None, weights)[0]
 
     _check(float(np.sum(g_ext[1] * d)), _dd(with_ext, exts[1], d), 2e-6, "ext")
+
+
+def test_triangle_vjp_matches_finite_differences():
+    """Reverse product of eval_triangles (membrane, area preservation, damping;
+    no drag or lift) vs central differences."""
+    from newton_solver_fem_amd import ModelBuilder
+
+    rng = np.random.default_rng(35)
+    b = ModelBuilder(gravity=(0.0, 0.0, 0.0))
+    for p in ((0, 0, 0), (0.1, 0, 0), (0.0, 0.12, 0.01), (0.11, 0.1, 0.03), (0.05, -0.08, 0.02)):
+        b.add_particle(p, (0, 0, 0), mass=0.01)
+    b.add_triangle(0, 1, 2, tri_ke=30.0, tri_ka=50.0, tri_kd=0.4)
+    b.add_triangle(1, 3, 2, tri_ke=20.0, tri_ka=70.0, tri_kd=0.2)
+    b.add_triangle(0, 4, 1, tri_ke=25.0, tri_ka=40.0, tri_kd=0.3)
+    model = b.finalize("cpu")
+    o = oracle.OracleSolver(model.host, bits=64, gravity=model.gravity)
+    q = model.host["particle_q"].astype(np.float64) + rng.normal(0, 0.01, (5, 3))
+    v = rng.normal(0, 0.1, q.shape)
+    z = rng.normal(size=q.shape)
+
+    def L(qq, vv):
+        return float(np.sum(z * o.eval_terms(oracle.TERM_TRI, qq, vv)))
+
+    xb, vb, _, _ = o.vjp_terms(oracle.TERM_TRI, q, v, z)
+    for _ in range(3):
+        d = rng.normal(size=q.shape)
+        _check(float(np.sum(xb * d)), _dd(lambda x: L(x, v), q, d), 1e-6, "triangle dq")
+        _check(float(np.sum(vb * d)), _dd(lambda x: L(q, x), v, d), 1e-6, "triangle dv")
