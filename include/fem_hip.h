@@ -116,11 +116,15 @@ const char* fem_last_error(void);
 int fem_pattern_size(const fem_system* sys, int64_t* nnz, int32_t* max_row_blocks);
 
 /* Binds caller-owned device buffers that stay valid for the system's life:
- * offsets [P+1] int32, columns [nnz] int32 (both filled here, synchronously),
- * values [nnz][9], inv_diag [P][9] (block-Jacobi inverse, SolverFEM
- * _pc_inv_diag) and increment [P][3] (SolverFEM.dv warm start). */
+ * offsets [P+1] int32, columns [capacity] int32 (offsets and the first nnz
+ * columns filled here, synchronously), values [capacity][9], inv_diag [P][9]
+ * (block-Jacobi inverse, SolverFEM _pc_inv_diag) and increment [P][3]
+ * (SolverFEM.dv warm start).  block_capacity (0: nnz) is the number of blocks
+ * the columns/values storage holds; a topology refresh without a reserved
+ * pool re-sorts the pattern within it, as A_bsr's nnz_capacity of
+ * P + 2S + 3R + 16T + 64H blocks does in the reference (solver.py:318-325). */
 int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float* values_dev, float* inv_diag_dev,
-             float* increment_dev);
+             float* increment_dev, int64_t block_capacity);
 
 int fem_set_params(fem_system* sys, const fem_params* params);
 
@@ -269,6 +273,64 @@ int fem_set_tet_materials(fem_system* sys, const float* mat_dev, void* stream);
  * {forward ratio, adjoint ratio, forward violation, adjoint violation};
  * reset != 0 clears them. */
 int fem_diff_stats(fem_system* sys, void* stream, double* out4, int32_t reset);
+
+/* ---- Topology updates (solver.py:695-720, :870-935; streaming.py) ---------
+ *
+ * fem_prepare_topology replaces prepare_tetrahedral_updates
+ * (solver.py:695-720, StreamingTetAssembly.__init__ streaming.py:487-597): the
+ * P particles are P / original_node_count copies of the original nodes (pool
+ * node i stands for original node i % original_node_count); the pattern
+ * becomes the reserved one -- row i holds i and every copy of every original
+ * neighbour of i % original_node_count in the current tets -- with room for the
+ * incidences of any T tets over those rows.  Tets only, no fixed particles,
+ * before component scaling.  The pattern changes, so fem_pattern_size /
+ * fem_bind must be called again, then fem_assemble at the current positions.
+ * Captured launches made after that stay valid across fem_update_topology.
+ * Errors: FEM_ERR_ARG for a pool that is not whole copies, tet nodes outside
+ * the pool or a tet with a repeated original node; FEM_ERR_STATE when
+ * already prepared or component scaling is bound. */
+int fem_prepare_topology(fem_system* sys, int32_t original_node_count);
+
+/* Replaces refresh_topology (solver.py:870-935): publishes edited tets
+ * (T x 4 indices, T x 9 rest inverses, T x 3 materials; zero materials for
+ * unused slots), particle masses / inverse masses / flags (P each), all
+ * device arrays of the model's fixed capacities, and assembles A and M^-1 at
+ * q.  With a reserved pool (StreamingTetAssembly.update streaming.py:624-716)
+ * the edit is validated first and rejected without changing anything
+ * (FEM_ERR_ARG with the reference's message: an element source outside
+ * [-1, T), a node outside the pool, an edge outside the reserved adjacency, a
+ * repeated original node); element_sources (T int32, NULL: identity) names
+ * each slot's previous element or -1.  stats (8 int32, may be NULL) receives
+ * the reference's update counters {dirty, removed, reused, active, touched
+ * rows, topology-inserted, topology-removed, full numeric refresh}.  The pool's
+ * pattern, matrix storage and launch shapes do not change.  Without a pool the
+ * exact pattern of the edited model is built again (within the bound block
+ * capacity; offsets and columns are rewritten and fem_pattern_size reports the
+ * new nnz); device arrays the previous pattern used stay allocated until
+ * fem_destroy.  Synchronizes the stream.  Not for batched systems. */
+int fem_update_topology(fem_system* sys, const int32_t* tet_indices_dev, const float* tet_poses_dev,
+                        const float* tet_materials_dev, const float* particle_mass_dev,
+                        const float* particle_inv_mass_dev, const int32_t* particle_flags_dev, const float* q_dev,
+                        const int32_t* element_sources_dev, float dt, int32_t* stats, void* stream);
+
+/* ---- Component scaling (component_scaling.py; solver.py:722-767) ---------
+ *
+ * fem_bind_component_scaling replaces prepare_component_scaling: caller-owned
+ * device storage for the per-component fp64 total masses [capacity], the
+ * per-row factors [P], and the scaled copies of the matrix values (the bound
+ * block capacity x 9, sharing the matrix's offsets and columns), of the
+ * inverse diagonal blocks [P][9] and of the right-hand side [P][3].  From then
+ * on every fem_step_solve solves (S A) dv = S b with preconditioner M^-1 S^-1,
+ * S = diag(factor), whose exact solution is the physical dv
+ * (ComponentScaling.prepare, solver.py:1300-1304).  No fixed particles.
+ * fem_refresh_component_scaling replaces refresh_component_scaling: factors
+ * from the P int32 labels (-1: unused row; a label outside [0, capacity)
+ * keeps factor 1) and the P particle masses: 1 / the label's total mass, 1
+ * where that total is not positive. */
+int fem_bind_component_scaling(fem_system* sys, int32_t capacity, double* totals_dev, float* scale_dev,
+                               float* values_dev, float* inv_diag_dev, float* rhs_dev);
+int fem_refresh_component_scaling(fem_system* sys, const int32_t* labels_dev, const float* particle_mass_dev,
+                                  void* stream);
 
 #ifdef __cplusplus
 }

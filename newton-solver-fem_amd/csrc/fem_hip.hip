@@ -1718,6 +1718,52 @@ __global__ void k_restore(const int* __restrict__ fixed, int nfixed, const float
   }
 }
 
+__global__ void k_copy_floats(const float* __restrict__ src, float* __restrict__ dst, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// ---------------------------------------------------------------------------
+// Component scaling (component_scaling.py:14-49): every equation of a
+// mechanically disconnected component is divided by the component's total
+// mass.  k_component_mass sums the masses per label in fp64, k_component_scale
+// forms the per-row factor (1 for a row whose label is outside [0, capacity)
+// or whose component has no positive mass), k_scale_system writes the scaled
+// copies of the matrix rows, the inverse diagonal blocks and the right-hand
+// side that the solve then uses.
+// ---------------------------------------------------------------------------
+__global__ void k_component_mass(const int* __restrict__ labels, const float* __restrict__ mass, int P, int cap,
+                                 double* __restrict__ total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const int c = labels[i];
+  if (c >= 0 && c < cap) atomicAdd(total + c, (double)mass[i]);
+}
+
+__global__ void k_component_scale(const int* __restrict__ labels, const double* __restrict__ total, int P, int cap,
+                                  float* __restrict__ scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const int c = labels[i];
+  float f = 1.f;
+  if (c >= 0 && c < cap && total[c] > 0.0) f = (float)(1.0 / total[c]);
+  scale[i] = f;
+}
+
+// One wave per row: lanes stride over the row's 9 * nnz values.
+__global__ __launch_bounds__(BLOCK) void k_scale_system(const int* __restrict__ offsets, const float* __restrict__ vals,
+                                                        const float* __restrict__ minv, const float* __restrict__ rhs,
+                                                        const float* __restrict__ scale, int P, float* __restrict__ svals,
+                                                        float* __restrict__ sminv, float* __restrict__ srhs) {
+  const int row = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= P) return;
+  const float f = scale[row];
+  const size_t b0 = 9 * (size_t)offsets[row], b1 = 9 * (size_t)offsets[row + 1];
+  for (size_t k = b0 + lane; k < b1; k += 64) svals[k] = f * vals[k];
+  if (lane < 9) sminv[9 * (size_t)row + lane] = minv[9 * (size_t)row + lane] / f;
+  if (lane < 3) srhs[3 * (size_t)row + lane] = f * rhs[3 * (size_t)row + lane];
+}
+
 // ---------------------------------------------------------------------------
 // Launch dispatch over the template parameters.
 // ---------------------------------------------------------------------------
@@ -1867,8 +1913,25 @@ struct fem_system {
   size_t asm_lds = 0;
   int* d_asm_groups = nullptr;  // [asm_ngroups + 1] first row of each assembly workgroup
   int asm_ngroups = 0;
+  int asm_cap = 0;  // fixed assembly group count of a reserved topology pool (0: none)
   int max_resident = 0;
   std::vector<void*> allocs;
+  bool keep_replaced = false;
+  int64_t b_cap = 0;  // block capacity of the bound offsets/columns/values storage
+  // Host copies of the element data a topology rebuild needs besides the tets.
+  std::vector<int32_t> h_hex_idx, h_sp_idx, h_tri_idx;
+  std::vector<float> h_sp_ke, h_sp_kd, h_tri_pose, h_tri_mat;
+  // Reserved topology pool (streaming.py): original node count, the last
+  // published elements (host) and the positions of the last assembly (device).
+  int topo_count = 0;
+  std::vector<int32_t> old_nodes;
+  std::vector<float> old_poses, old_mats;
+  std::vector<unsigned char> old_active;
+  float* d_lin_q = nullptr;
+  // Component scaling (component_scaling.py), caller-owned storage.
+  float *cs_scale = nullptr, *cs_vals = nullptr, *cs_minv = nullptr, *cs_rhs = nullptr;
+  double* cs_total = nullptr;
+  int cs_cap = 0;
 
   template <typename T_>
   int upload(T_** dst, const T_* src, size_t n) {
@@ -1883,6 +1946,25 @@ struct fem_system {
   template <typename T_>
   int alloc(T_** dst, size_t n) {
     return upload<T_>(dst, nullptr, n);
+  }
+  // A replaced allocation.  While keep_replaced is set it stays allocated
+  // until the system is destroyed (a graph captured earlier may still name
+  // it); otherwise it is freed now.
+  void drop(void* p) {
+    if (!p || keep_replaced) return;
+    (void)hipFree(p);
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+  }
+  // Device copy of n elements of src into a new allocation of at least cap
+  // elements; the previous allocation *dst (if any) is dropped.
+  template <typename T_>
+  int replace(T_** dst, const T_* src, size_t n, size_t cap = 0) {
+    T_* old = *dst;
+    *dst = nullptr;
+    if (int rc = alloc(dst, std::max(n, cap))) return rc;
+    if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T_), hipMemcpyHostToDevice));
+    drop(old);
+    return FEM_OK;
   }
 
   RowArgs row_args() const {
@@ -1941,6 +2023,78 @@ struct fem_system {
     return a;
   }
 
+  // Constant spring blocks: per slot the signed (ke, kd) sums of the spring
+  // triplets landing on it -- +1 per spring on its endpoints' diagonals
+  // (build_system_matrix_diagonal_kernel kernels.py:294-323 counts a spring
+  // once per endpoint row), -1 per (i, j) and (j, i) block
+  // (build_system_matrix_sparse_kernel :326-366), so a spring whose two
+  // endpoints coincide nets -1 on its diagonal as in the reference.  A spring
+  // with a -1 endpoint keeps its diagonal term at the other endpoint (the
+  // diagonal kernel tests each endpoint); its off-diagonal triplets carry row
+  // or column -1 and are dropped.
+  std::vector<float2> spring_slots() const {
+    std::vector<float2> slot_spring(pt.columns.size(), make_float2(0.f, 0.f));
+    auto slot_of = [&](int r, int c) {
+      const int32_t* row = pt.columns.data() + pt.offsets[r];
+      return (size_t)(pt.offsets[r] + (std::lower_bound(row, row + (pt.offsets[r + 1] - pt.offsets[r]), c) - row));
+    };
+    for (int k = 0; k < S; ++k) {
+      const int p = h_sp_idx[2 * k], q = h_sp_idx[2 * k + 1];
+      const float ke = h_sp_ke[k], kd = h_sp_kd[k];
+      if (p >= 0 && q >= 0)
+        for (size_t sl : {slot_of(p, q), slot_of(q, p)}) {
+          slot_spring[sl].x -= ke;
+          slot_spring[sl].y -= kd;
+        }
+      if (p >= 0) {
+        slot_spring[slot_of(p, p)].x += ke;
+        slot_spring[slot_of(p, p)].y += kd;
+      }
+      if (q >= 0 && q != p) {
+        slot_spring[slot_of(q, q)].x += ke;
+        slot_spring[slot_of(q, q)].y += kd;
+      }
+    }
+    return slot_spring;
+  }
+
+  // Device copies of everything derived from the pattern: rows, incidences,
+  // their slots, per-slot contribution lists and the constant spring blocks.
+  // tinc_cap / contrib_cap reserve room for later incidence updates.
+  int upload_pattern(size_t tinc_cap = 0, size_t contrib_cap = 0) {
+    std::vector<int32_t> off32(P + 1), tptr(P + 1), hptr(P + 1), sptr(P + 1), rptr(P + 1);
+    for (int i = 0; i <= P; ++i) {
+      off32[i] = (int32_t)pt.offsets[i];
+      tptr[i] = (int32_t)pt.tinc_ptr[i];
+      hptr[i] = (int32_t)pt.hinc_ptr[i];
+      sptr[i] = (int32_t)pt.sinc_ptr[i];
+      rptr[i] = (int32_t)pt.rinc_ptr[i];
+    }
+    int rc;
+    if ((rc = replace(&d_offsets, off32.data(), off32.size())) ||
+        (rc = replace(&d_cols, pt.columns.data(), pt.columns.size())) ||
+        (rc = replace(&d_diag, pt.diag_rel.data(), pt.diag_rel.size())) ||
+        (rc = replace(&d_tinc_ptr, tptr.data(), tptr.size())) ||
+        (rc = replace(&d_tinc, pt.tinc.data(), pt.tinc.size(), tinc_cap)) ||
+        (rc = replace(&d_tslots, pt.tslots.data(), pt.tslots.size(), tinc_cap)) ||
+        (rc = replace(&d_hinc_ptr, hptr.data(), hptr.size())) ||
+        (rc = replace(&d_hinc, pt.hinc.data(), pt.hinc.size())) ||
+        (rc = replace(&d_hslots, pt.hslots.data(), pt.hslots.size())) ||
+        (rc = replace(&d_slot_ptr, pt.slot_ptr.data(), pt.slot_ptr.size())) ||
+        (rc = replace(&d_contrib, pt.contrib.data(), pt.contrib.size(), contrib_cap)))
+      return rc;
+    if (S > 0 || R > 0) {
+      const std::vector<float2> slot_spring = S > 0 ? spring_slots() : std::vector<float2>();
+      if ((rc = replace(&d_sinc_ptr, sptr.data(), sptr.size())) ||
+          (rc = replace(&d_sinc, pt.sinc.data(), pt.sinc.size())) ||
+          (rc = replace(&d_rinc_ptr, rptr.data(), rptr.size())) ||
+          (rc = replace(&d_rinc, pt.rinc.data(), pt.rinc.size())) ||
+          (rc = replace(&d_slot_spring, slot_spring.data(), slot_spring.size())))
+        return rc;
+    }
+    return FEM_OK;
+  }
+
   // Dynamic LDS bytes of the assembly of rows r0 .. r1 - 1.
   size_t assemble_lds(int r0, int r1) const {
     const size_t nt = (size_t)(pt.tinc_ptr[r1] - pt.tinc_ptr[r0]);
@@ -1953,11 +2107,10 @@ struct fem_system {
   // Row groups of the assembly: consecutive rows packed greedily, at most rb
   // rows, at most BLOCK element incidences (one pass of phase 1) and at most
   // ASM_LDS_BUDGET bytes of staging per group; a row that alone exceeds the
-  // limits forms its own group.  The group count is padded to a multiple of 8
-  // with empty groups (XCD swizzle).  Replaces the previous table on success.
-  int plan_assemble(int rb) {
-    std::vector<int32_t> g{0};
-    size_t worst = 16;
+  // limits forms its own group.  `worst` receives the largest staging.
+  void assemble_groups(int rb, std::vector<int32_t>& g, size_t& worst) const {
+    g.assign(1, 0);
+    worst = 16;
     int r0 = 0;
     while (r0 < P) {
       int r1 = r0 + 1;
@@ -1970,20 +2123,61 @@ struct fem_system {
       g.push_back(r1);
       r0 = r1;
     }
-    int n = (int)g.size() - 1;
+  }
+
+  // Installs the assembly row groups.  The group count is padded to a
+  // multiple of 8 with empty groups (XCD swizzle).  With a reserved topology
+  // pool the launch keeps a fixed group count and staging size (asm_cap,
+  // asm_lds) and the table is rewritten in place, so that a captured launch
+  // stays valid across topology updates; otherwise the table is replaced.
+  int plan_assemble(int rb) {
+    std::vector<int32_t> g;
+    size_t worst = 0;
+    assemble_groups(rb, g, worst);
+    const int n = (int)g.size() - 1;
+    if (asm_cap > 0) {
+      if (n > asm_cap || worst > asm_lds)
+        return fail(FEM_ERR_UNSUPPORTED, "the edited topology exceeds the reserved assembly launch");
+      g.resize(asm_cap + 1, P);
+      HIP_TRY(hipMemcpy(d_asm_groups, g.data(), g.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      asm_rb = rb;
+      return FEM_OK;
+    }
     const int padded = n > 8 ? (n + 7) & ~7 : std::max(n, 1);
     while ((int)g.size() < padded + 1) g.push_back(P);
     int* dg = nullptr;
     if (int rc = upload(&dg, g.data(), g.size())) return rc;
-    if (d_asm_groups) {
-      (void)hipFree(d_asm_groups);
-      allocs.erase(std::remove(allocs.begin(), allocs.end(), (void*)d_asm_groups), allocs.end());
-    }
+    drop(d_asm_groups);
     d_asm_groups = dg;
     asm_ngroups = padded;
     asm_rb = rb;
     asm_lds = worst;
     return FEM_OK;
+  }
+
+  // Fixed assembly launch for a reserved topology pool: a group count that
+  // bounds the greedy packing of any element set within the capacities (each
+  // closed group either holds rb rows or, together with the next row, exceeds
+  // an incidence or staging limit) and the full staging budget.
+  int reserve_assemble(int rb) {
+    const double inc_total = 4.0 * T;
+    const double lds_total = (double)(ST_TET * inc_total + pt.offsets[P] + 2.0 * P) * sizeof(float) +
+                             16.0 * T * sizeof(uint16_t);
+    std::vector<int32_t> g;
+    size_t worst = 0;
+    assemble_groups(rb, g, worst);
+    long cap = (P + rb - 1) / rb + (long)std::ceil(2.0 * inc_total / BLOCK) +
+               (long)std::ceil(2.0 * lds_total / ASM_LDS_BUDGET) + 8;
+    cap = std::max<long>(cap, (long)g.size() - 1);
+    cap = (cap + 7) & ~7L;
+    int* dg = nullptr;
+    if (int rc = alloc(&dg, (size_t)cap + 1)) return rc;
+    drop(d_asm_groups);
+    d_asm_groups = dg;
+    asm_cap = (int)cap;
+    asm_ngroups = (int)cap;
+    asm_lds = std::max(worst, ASM_LDS_BUDGET);
+    return plan_assemble(rb);
   }
 
   // Row partition of the persistent CG: per instance, contiguous chunks,
@@ -2229,12 +2423,8 @@ struct fem_system {
       return rc;
     for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
                     (void*)d_chunk_tiles, (void*)d_tile_rows, (void*)d_tile_cptr, (void*)d_tile_cols,
-                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row, (void*)d_cgflags}) {
-      if (p) {
-        (void)hipFree(p);
-        allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
-      }
-    }
+                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row, (void*)d_cgflags})
+      drop(p);
     d_chunk_rows = n_rows;
     d_chunk_inst = n_inst;
     d_inst_chunks = n_ichunks;
@@ -2306,41 +2496,18 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
   int rc = build_pattern(P, s->T, d->tet_indices, d->tet_materials, s->H, d->hex_indices, s->S, d->spring_indices,
                          s->R, d->tri_indices, s->pt, err);
   if (rc) return cleanup(fail(rc, err));
+  s->h_hex_idx.assign(d->hex_indices, d->hex_indices + 8 * (size_t)s->H);
+  s->h_sp_idx.assign(d->spring_indices, d->spring_indices + 2 * (size_t)s->S);
+  s->h_sp_ke.assign(d->spring_stiffness, d->spring_stiffness + s->S);
+  s->h_sp_kd.assign(d->spring_damping, d->spring_damping + s->S);
+  s->h_tri_idx.assign(d->tri_indices, d->tri_indices + 3 * (size_t)s->R);
+  s->h_tri_pose.assign(d->tri_poses, d->tri_poses + 4 * (size_t)s->R);
+  s->h_tri_mat.assign(d->tri_materials, d->tri_materials + 5 * (size_t)s->R);
   if (s->S > 0 || s->R > 0) {
-    // Constant blocks: per slot the signed (ke, kd) sums of the spring
-    // triplets landing on it -- +1 per spring on its endpoints' diagonals
-    // (build_system_matrix_diagonal_kernel kernels.py:294-323 counts a spring
-    // once per endpoint row), -1 per (i, j) and (j, i) block
-    // (build_system_matrix_sparse_kernel :326-366), so a spring whose two
-    // endpoints coincide nets -1 on its diagonal as in the reference -- and
-    // per row the triangles' (k_mu + k_lambda) * rest area; degenerate
-    // triangles contribute nothing (build_system_matrix_tri_kernel :994-1008).
-    std::vector<float2> slot_spring(s->S > 0 ? s->pt.columns.size() : 0, make_float2(0.f, 0.f));
+    // Per row the triangles' (k_mu + k_lambda) * rest area; degenerate
+    // triangles contribute nothing (build_system_matrix_tri_kernel
+    // kernels.py:994-1008).
     std::vector<float> row_tri(s->R > 0 ? P : 0, 0.f);
-    auto slot_of = [&](int r, int c) {
-      const int32_t* row = s->pt.columns.data() + s->pt.offsets[r];
-      return (size_t)(s->pt.offsets[r] + (std::lower_bound(row, row + (s->pt.offsets[r + 1] - s->pt.offsets[r]), c) - row));
-    };
-    for (int k = 0; k < s->S; ++k) {
-      // A spring with a -1 endpoint keeps its diagonal term at the other
-      // endpoint (the diagonal kernel tests each endpoint); its off-diagonal
-      // triplets carry row or column -1 and are dropped.
-      const int p = d->spring_indices[2 * k], q = d->spring_indices[2 * k + 1];
-      const float ke = d->spring_stiffness[k], kd = d->spring_damping[k];
-      if (p >= 0 && q >= 0)
-        for (size_t sl : {slot_of(p, q), slot_of(q, p)}) {
-          slot_spring[sl].x -= ke;
-          slot_spring[sl].y -= kd;
-        }
-      if (p >= 0) {
-        slot_spring[slot_of(p, p)].x += ke;
-        slot_spring[slot_of(p, p)].y += kd;
-      }
-      if (q >= 0 && q != p) {
-        slot_spring[slot_of(q, q)].x += ke;
-        slot_spring[slot_of(q, q)].y += kd;
-      }
-    }
     for (int t = 0; t < s->R; ++t) {
       s->tri_aero = s->tri_aero || d->tri_materials[5 * (size_t)t + 3] != 0.f || d->tri_materials[5 * (size_t)t + 4] != 0.f;
       const float* D = d->tri_poses + 4 * (size_t)t;
@@ -2350,20 +2517,10 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       const float c = d->tri_materials[5 * (size_t)t] * area + d->tri_materials[5 * (size_t)t + 1] * area;
       for (int a = 0; a < 3; ++a) row_tri[d->tri_indices[3 * (size_t)t + a]] += c;
     }
-    std::vector<int32_t> sptr(P + 1), rptr(P + 1);
-    for (int i = 0; i <= P; ++i) {
-      sptr[i] = (int32_t)s->pt.sinc_ptr[i];
-      rptr[i] = (int32_t)s->pt.rinc_ptr[i];
-    }
-    if ((rc = s->upload(&s->d_sinc_ptr, sptr.data(), sptr.size())) ||
-        (rc = s->upload(&s->d_sinc, s->pt.sinc.data(), s->pt.sinc.size())) ||
-        (rc = s->upload(&s->d_sp_idx, d->spring_indices, 2 * (size_t)s->S)) ||
+    if ((rc = s->upload(&s->d_sp_idx, d->spring_indices, 2 * (size_t)s->S)) ||
         (rc = s->upload(&s->d_sp_rest, d->spring_rest_length, (size_t)s->S)) ||
         (rc = s->upload(&s->d_sp_ke, d->spring_stiffness, (size_t)s->S)) ||
         (rc = s->upload(&s->d_sp_kd, d->spring_damping, (size_t)s->S)) ||
-        (rc = s->upload(&s->d_slot_spring, slot_spring.data(), slot_spring.size())) ||
-        (rc = s->upload(&s->d_rinc_ptr, rptr.data(), rptr.size())) ||
-        (rc = s->upload(&s->d_rinc, s->pt.rinc.data(), s->pt.rinc.size())) ||
         (rc = s->upload(&s->d_tri_idx, d->tri_indices, 3 * (size_t)s->R)) ||
         (rc = s->upload(&s->d_tri_pose, d->tri_poses, 4 * (size_t)s->R)) ||
         (rc = s->upload(&s->d_tri_mat, d->tri_materials, 5 * (size_t)s->R)) ||
@@ -2380,30 +2537,14 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
         if (inst_of[s->pt.columns[k]] != inst_of[i]) return cleanup(fail(FEM_ERR_ARG, "element spans two instances"));
     if ((rc = s->upload(&s->d_row_inst, inst_of.data(), P))) return cleanup(rc);
   }
-  std::vector<int32_t> off32(P + 1), tptr(P + 1), hptr(P + 1);
-  for (int i = 0; i <= P; ++i) {
-    off32[i] = (int32_t)s->pt.offsets[i];
-    tptr[i] = (int32_t)s->pt.tinc_ptr[i];
-    hptr[i] = (int32_t)s->pt.hinc_ptr[i];
-  }
+  if ((rc = s->upload_pattern())) return cleanup(rc);
   std::vector<unsigned char> fixed_mask(std::max(P, 1), 0);
   for (int k = 0; k < s->nfixed; ++k) {
     int i = d->fixed_indices[k];
     if (i < 0 || i >= P) return cleanup(fail(FEM_ERR_ARG, "fixed index out of range"));
     fixed_mask[i] = 1;
   }
-  if ((rc = s->upload(&s->d_offsets, off32.data(), off32.size())) ||
-      (rc = s->upload(&s->d_cols, s->pt.columns.data(), s->pt.columns.size())) ||
-      (rc = s->upload(&s->d_diag, s->pt.diag_rel.data(), s->pt.diag_rel.size())) ||
-      (rc = s->upload(&s->d_tinc_ptr, tptr.data(), tptr.size())) ||
-      (rc = s->upload(&s->d_tinc, s->pt.tinc.data(), s->pt.tinc.size())) ||
-      (rc = s->upload(&s->d_tslots, s->pt.tslots.data(), s->pt.tslots.size())) ||
-      (rc = s->upload(&s->d_hinc_ptr, hptr.data(), hptr.size())) ||
-      (rc = s->upload(&s->d_hinc, s->pt.hinc.data(), s->pt.hinc.size())) ||
-      (rc = s->upload(&s->d_hslots, s->pt.hslots.data(), s->pt.hslots.size())) ||
-      (rc = s->upload(&s->d_slot_ptr, s->pt.slot_ptr.data(), s->pt.slot_ptr.size())) ||
-      (rc = s->upload(&s->d_contrib, s->pt.contrib.data(), s->pt.contrib.size())) ||
-      (rc = s->upload(&s->d_tet_idx, d->tet_indices, 4 * (size_t)s->T)) ||
+  if ((rc = s->upload(&s->d_tet_idx, d->tet_indices, 4 * (size_t)s->T)) ||
       (rc = s->upload(&s->d_tet_pose, d->tet_poses, 9 * (size_t)s->T)) ||
       (rc = s->upload(&s->d_tet_mat, d->tet_materials, 3 * (size_t)s->T)) ||
       (rc = s->upload(&s->d_hex_idx, d->hex_indices, 8 * (size_t)s->H)) ||
@@ -2451,8 +2592,15 @@ int fem_pattern_size(const fem_system* s, int64_t* nnz, int32_t* max_row_blocks)
   return FEM_OK;
 }
 
-int fem_bind(fem_system* s, int32_t* offsets, int32_t* columns, float* values, float* inv_diag, float* increment) {
+int fem_bind(fem_system* s, int32_t* offsets, int32_t* columns, float* values, float* inv_diag, float* increment,
+             int64_t block_capacity) {
   if (!s || !offsets || !columns || !values || !inv_diag || !increment) return fail(FEM_ERR_ARG, "null buffer");
+  const int64_t nnz = s->pt.offsets[s->P];
+  if (block_capacity == 0) block_capacity = nnz;
+  if (block_capacity < nnz) return fail(FEM_ERR_ARG, "block_capacity is smaller than the pattern");
+  if (s->cs_scale && block_capacity != s->b_cap)
+    return fail(FEM_ERR_STATE, "component scaling storage is bound to the previous block capacity");
+  s->b_cap = block_capacity;
   s->b_offsets = offsets;
   s->b_cols = columns;
   s->b_vals = values;
@@ -2541,8 +2689,15 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
-  return dispatch_g<LaunchAssemble>(s->asm_rb, s->full(), ra, s->d_asm_groups, s->asm_ngroups, q, dt, s->b_vals, s->b_minv,
-                                    s->asm_lds, (hipStream_t)stream);
+  int rc = dispatch_g<LaunchAssemble>(s->asm_rb, s->full(), ra, s->d_asm_groups, s->asm_ngroups, q, dt, s->b_vals,
+                                      s->b_minv, s->asm_lds, (hipStream_t)stream);
+  if (rc || !s->topo_count) return rc;
+  // Positions of the current tangent (streaming.py:603, :716), compared with
+  // by the next topology update.
+  hipLaunchKernelGGL(k_copy_floats, dim3((3 * s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, q,
+                     s->d_lin_q, 3 * s->P);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
 }
 
 int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* qd, const float* act, float* out,
@@ -2638,6 +2793,22 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
   if (s->P == 0) return FEM_OK;
   CgArgs ca = cg_args(s);
   ca.rhs = f_in;
+  if (s->cs_scale) {
+    // Scaled copies (ComponentScaling.prepare, solver.py:1300-1304), then the
+    // maximum of the scaled right-hand side (:1306-1312).
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_scale_system, dim3((s->P + BLOCK / 64 - 1) / (BLOCK / 64)), dim3(BLOCK), 0, st,
+                       (const int*)s->d_offsets, (const float*)s->b_vals, (const float*)s->b_minv, f_in,
+                       (const float*)s->cs_scale, s->P, s->cs_vals, s->cs_minv, s->cs_rhs);
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(BLOCK), 0, st, (const int*)nullptr, 0, (const float*)nullptr,
+                       (float*)nullptr, (float*)nullptr, s->d_rhs_max, 1);
+    hipLaunchKernelGGL(k_absmax, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const float*)s->cs_rhs, s->P,
+                       (const int*)nullptr, s->d_rhs_max);
+    HIP_TRY(hipGetLastError());
+    ca.vals = s->cs_vals;
+    ca.minv = s->cs_minv;
+    ca.rhs = s->cs_rhs;
+  }
   ca.q_in = q_in;
   ca.qd_in = qd_in;
   ca.q_out = q_out;
@@ -2813,6 +2984,302 @@ int fem_solve_stats(fem_system* s, void* stream, int32_t* iterations, float* res
     if (atol_sq) std::memcpy(&atol_sq[k], &st[4 * k + 2], 4);
     if (error_flags) error_flags[k] = err;
   }
+  return FEM_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Topology updates (solver.py:695-720, :870-935; streaming.py).
+// ---------------------------------------------------------------------------
+int fem_prepare_topology(fem_system* s, int32_t original_node_count) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (s->topo_count) return fail(FEM_ERR_STATE, "Tetrahedral updates are already prepared");
+  if (s->cs_scale) return fail(FEM_ERR_STATE, "Prepare tetrahedral updates before component scaling");
+  if (s->nfixed) return fail(FEM_ERR_ARG, "Streaming topology is not supported with fixed_particle_indices");
+  if (s->T < 1 || s->H || s->S || s->R) return fail(FEM_ERR_ARG, "Streaming assembly requires tetrahedral elements only");
+  if (s->ninst > 1) return fail(FEM_ERR_UNSUPPORTED, "topology updates of a batched system are not supported");
+  const int T = s->T, P = s->P;
+  HIP_TRY(hipDeviceSynchronize());
+  std::vector<int32_t> nodes(4 * (size_t)T);
+  std::vector<float> poses(9 * (size_t)T), mats(3 * (size_t)T);
+  HIP_TRY(hipMemcpy(nodes.data(), s->d_tet_idx, nodes.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(poses.data(), s->d_tet_pose, poses.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mats.data(), s->d_tet_mat, mats.size() * 4, hipMemcpyDeviceToHost));
+  Pattern np;
+  std::string err;
+  if (int rc = build_reserved_pattern(P, original_node_count, T, nodes.data(), mats.data(), np, err))
+    return fail(rc == 1 ? FEM_ERR_ARG : FEM_ERR_UNSUPPORTED, err);
+  s->pt = std::move(np);
+  int rc;
+  if ((rc = s->upload_pattern(4 * (size_t)T, 16 * (size_t)T))) return rc;
+  if (!s->d_lin_q && (rc = s->alloc(&s->d_lin_q, 3 * (size_t)P))) return rc;
+  s->old_nodes = nodes;
+  s->old_poses = poses;
+  s->old_mats = mats;
+  s->old_active.assign(T, 0);
+  for (int e = 0; e < T; ++e)
+    s->old_active[e] = !(mats[3 * e] == 0.f && mats[3 * e + 1] == 0.f && mats[3 * e + 2] == 0.f);
+  if ((rc = s->reserve_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+  s->topo_count = original_node_count;
+  // The pattern changed: the matrix storage must be bound again.
+  s->b_offsets = s->b_cols = nullptr;
+  s->b_vals = nullptr;
+  s->b_cap = 0;
+  return FEM_OK;
+}
+
+namespace {
+bool tet_enabled(const float* m) { return m[0] != 0.f || m[1] != 0.f || m[2] != 0.f; }
+
+// Elements of a reserved pool against the last published ones
+// (_stage_elements / _remove_elements / _add_elements / _compact_rows,
+// streaming.py:34-245): validation, lineage and the update counters
+// [dirty, removed, reused, active, touched rows, topology-inserted,
+// topology-removed].  Returns the largest error code (1 source out of range,
+// 2 node outside the pool, 3 edge outside the reserved adjacency, 4 repeated
+// original node; streaming.py:654-662), 0 when valid.  Elements are visited
+// in ascending order, so when several children name one source the first
+// claims its retained topology and cached blocks.
+int stage_elements(const fem_system* s, const std::vector<int32_t>& nodes, const std::vector<float>& poses,
+                   const std::vector<float>& mats, const std::vector<int32_t>& src, const std::vector<float>& q_now,
+                   const std::vector<float>& q_lin, int32_t counters[8]) {
+  const int T = s->T, P = s->P, count = s->topo_count;
+  const Pattern& pt = s->pt;
+  auto has_slot = [&](int r, int c) {
+    const int32_t* row = pt.columns.data() + pt.offsets[r];
+    const int32_t* end = pt.columns.data() + pt.offsets[r + 1];
+    const int32_t* at = std::lower_bound(row, end, c);
+    return at != end && *at == c;
+  };
+  std::vector<unsigned char> kept(T, 0), tkept(T, 0), dirty(T, 0);
+  std::vector<int32_t> retained(T, -1);
+  for (int k = 0; k < 8; ++k) counters[k] = 0;
+  int code = 0;
+  for (int e = 0; e < T; ++e) {
+    if (!tet_enabled(&mats[3 * (size_t)e])) continue;
+    const int source = src[e];
+    if (source < -1 || source >= T) {
+      code = std::max(code, 1);
+      continue;
+    }
+    const int32_t* nd = &nodes[4 * (size_t)e];
+    bool bad = false;
+    for (int a = 0; a < 4 && !bad; ++a) {
+      if (nd[a] < 0 || nd[a] >= P) {
+        code = std::max(code, 2);
+        bad = true;
+        break;
+      }
+      for (int b = 0; b < a; ++b)
+        if (nd[a] % count == nd[b] % count) {
+          code = std::max(code, 4);
+          bad = true;
+          break;
+        }
+    }
+    if (bad) continue;
+    bool same = source >= 0 && s->old_active[source];
+    for (int a = 0; a < 4 && same; ++a) same = nd[a] == s->old_nodes[4 * (size_t)source + a];
+    if (same) {
+      if (!tkept[source]) {
+        tkept[source] = 1;
+        retained[e] = source;
+      }
+    } else {
+      for (int a = 0; a < 4 && !bad; ++a)
+        for (int b = 0; b < 4 && !bad; ++b)
+          if (!has_slot(nd[a], nd[b])) bad = true;
+      if (bad) {
+        code = std::max(code, 3);
+        continue;
+      }
+    }
+    if (retained[e] < 0) {
+      ++counters[5];
+      same = false;
+    }
+    if (same) {
+      for (int a = 0; a < 4 && same; ++a)
+        for (int d = 0; d < 3; ++d)
+          if (q_now[3 * (size_t)nd[a] + d] != q_lin[3 * (size_t)nd[a] + d]) same = false;
+      for (int k = 0; k < 3 && same; ++k) same = mats[3 * (size_t)e + k] == s->old_mats[3 * (size_t)source + k];
+      for (int k = 0; k < 9 && same; ++k) same = poses[9 * (size_t)e + k] == s->old_poses[9 * (size_t)source + k];
+    }
+    if (same) {
+      same = !kept[source];
+      kept[source] = 1;
+    }
+    ++counters[3];
+    if (same) {
+      ++counters[2];
+    } else {
+      dirty[e] = 1;
+      ++counters[0];
+    }
+  }
+  if (code) return code;
+  // Old contributions that are not kept are removed; rows lose or gain
+  // connectivity where an old element's topology was not retained or a new
+  // element's topology was inserted.
+  std::vector<unsigned char> touched(P, 0);
+  int removed = 0, topo_removed = 0;
+  for (int e = 0; e < T; ++e) {
+    if (!s->old_active[e] || kept[e]) continue;
+    ++removed;
+    if (!tkept[e]) {
+      ++topo_removed;
+      for (int a = 0; a < 4; ++a) touched[s->old_nodes[4 * (size_t)e + a]] = 1;
+    }
+  }
+  for (int e = 0; e < T; ++e)
+    if (dirty[e] && retained[e] < 0)
+      for (int a = 0; a < 4; ++a) touched[nodes[4 * (size_t)e + a]] = 1;
+  int ntouched = 0;
+  for (int i = 0; i < P; ++i) ntouched += touched[i];
+  // counters so far: [0] dirty, [2] reused, [3] active, [5] inserted
+  const int dirty_n = counters[0], reused = counters[2], active = counters[3], inserted = counters[5];
+  counters[0] = dirty_n;
+  counters[1] = removed;
+  counters[2] = reused;
+  counters[3] = active;
+  counters[4] = ntouched;
+  counters[5] = inserted;
+  counters[6] = topo_removed;
+  counters[7] = dirty_n > active / 2 ? 1 : 0;
+  return 0;
+}
+}  // namespace
+
+int fem_update_topology(fem_system* s, const int32_t* tet_indices, const float* tet_poses, const float* tet_materials,
+                        const float* particle_mass, const float* particle_inv_mass, const int32_t* particle_flags,
+                        const float* q, const int32_t* element_sources, float dt, int32_t* stats, void* stream) {
+  if (!s || !q || !particle_mass || !particle_inv_mass || !particle_flags ||
+      (s->T > 0 && (!tet_indices || !tet_poses || !tet_materials)))
+    return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->ninst > 1) return fail(FEM_ERR_UNSUPPORTED, "topology updates of a batched system are not supported");
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  const int T = s->T, P = s->P;
+  std::vector<int32_t> nodes(4 * (size_t)T), src(T);
+  std::vector<float> poses(9 * (size_t)T), mats(3 * (size_t)T);
+  if (T > 0) {
+    HIP_TRY(hipMemcpy(nodes.data(), tet_indices, nodes.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(poses.data(), tet_poses, poses.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(mats.data(), tet_materials, mats.size() * 4, hipMemcpyDeviceToHost));
+  }
+  Pattern np;
+  std::string err;
+  int32_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (s->topo_count) {
+    if (element_sources) {
+      HIP_TRY(hipMemcpy(src.data(), element_sources, src.size() * 4, hipMemcpyDeviceToHost));
+    } else {
+      for (int e = 0; e < T; ++e) src[e] = e;
+    }
+    std::vector<float> q_now(3 * (size_t)P), q_lin(3 * (size_t)P);
+    HIP_TRY(hipMemcpy(q_now.data(), q, q_now.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(q_lin.data(), s->d_lin_q, q_lin.size() * 4, hipMemcpyDeviceToHost));
+    static const char* msg[5] = {"", "Element source is outside the previous element capacity",
+                                 "Tetrahedral node is outside the reserved particle pool",
+                                 "Cut introduces an edge outside the reserved original-node adjacency",
+                                 "A tetrahedron must have four distinct original basis nodes"};
+    if (int code = stage_elements(s, nodes, poses, mats, src, q_now, q_lin, counters)) return fail(FEM_ERR_ARG, msg[code]);
+    np.P = P;
+    np.offsets = s->pt.offsets;
+    np.columns = s->pt.columns;
+    np.diag_rel = s->pt.diag_rel;
+    np.max_row_nnz = s->pt.max_row_nnz;
+    if (int rc = assign_tet_incidences(T, nodes.data(), mats.data(), np, err)) return fail(FEM_ERR_ARG, err);
+    // Assembly groups of the new incidences within the reserved launch.
+    Pattern old = std::move(s->pt);
+    s->pt = std::move(np);
+    if (int rc = s->plan_assemble(s->asm_rb)) {
+      s->pt = std::move(old);
+      return rc;
+    }
+    // Commit: incidences into their reserved storage, element and particle data.
+    const Pattern& pt = s->pt;
+    std::vector<int32_t> tptr(P + 1);
+    for (int i = 0; i <= P; ++i) tptr[i] = (int32_t)pt.tinc_ptr[i];
+    HIP_TRY(hipMemcpy(s->d_tinc_ptr, tptr.data(), tptr.size() * 4, hipMemcpyHostToDevice));
+    if (!pt.tinc.empty()) {
+      HIP_TRY(hipMemcpy(s->d_tinc, pt.tinc.data(), pt.tinc.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s->d_tslots, pt.tslots.data(), pt.tslots.size() * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(s->d_slot_ptr, pt.slot_ptr.data(), pt.slot_ptr.size() * 4, hipMemcpyHostToDevice));
+    if (!pt.contrib.empty())
+      HIP_TRY(hipMemcpy(s->d_contrib, pt.contrib.data(), pt.contrib.size() * 2, hipMemcpyHostToDevice));
+    s->old_nodes = nodes;
+    s->old_poses = poses;
+    s->old_mats = mats;
+    for (int e = 0; e < T; ++e) s->old_active[e] = tet_enabled(&mats[3 * (size_t)e]);
+  } else {
+    // No reserved pool: the exact pattern of the edited model, sorted again
+    // (solver.py:920-935), within the bound matrix storage.
+    if (int rc = build_pattern(P, T, nodes.data(), mats.data(), s->H, s->h_hex_idx.data(), s->S, s->h_sp_idx.data(), s->R,
+                               s->h_tri_idx.data(), np, err))
+      return fail(rc, err);
+    if (np.offsets[P] > s->b_cap)
+      return fail(FEM_ERR_UNSUPPORTED, "the edited topology needs more blocks than the bound matrix storage holds");
+    // Arrays a graph captured earlier may name stay allocated from here on.
+    s->keep_replaced = true;
+    s->pt = std::move(np);
+    int rc;
+    if ((rc = s->upload_pattern())) return rc;
+    std::vector<int32_t> off32(P + 1);
+    for (int i = 0; i <= P; ++i) off32[i] = (int32_t)s->pt.offsets[i];
+    HIP_TRY(hipMemcpy(s->b_offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+    if (!s->pt.columns.empty())
+      HIP_TRY(hipMemcpy(s->b_cols, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+  }
+  if (T > 0) {
+    HIP_TRY(hipMemcpyAsync(s->d_tet_idx, tet_indices, 16 * (size_t)T, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_tet_pose, tet_poses, 36 * (size_t)T, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s->d_tet_mat, tet_materials, 12 * (size_t)T, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(s->d_mass, particle_mass, 4 * (size_t)P, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_inv_mass, particle_inv_mass, 4 * (size_t)P, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(s->d_flags, particle_flags, 4 * (size_t)P, hipMemcpyDeviceToDevice, st));
+  if (int rc = fem_assemble(s, q, dt, stream)) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  if (stats)
+    for (int k = 0; k < 8; ++k) stats[k] = counters[k];
+  return FEM_OK;
+}
+
+int fem_bind_component_scaling(fem_system* s, int32_t capacity, double* totals, float* scale, float* values,
+                               float* inv_diag, float* rhs) {
+  if (!s || !totals || !scale || !values || !inv_diag || !rhs) return fail(FEM_ERR_ARG, "null buffer");
+  if (s->cs_scale) return fail(FEM_ERR_STATE, "Component scaling is already prepared");
+  if (s->nfixed) return fail(FEM_ERR_ARG, "Component scaling is not supported with fixed_particle_indices");
+  if (capacity < 1) return fail(FEM_ERR_ARG, "component_capacity must be a positive integer");
+  if (s->ninst > 1) return fail(FEM_ERR_UNSUPPORTED, "component scaling of a batched system is not supported");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  s->cs_cap = capacity;
+  s->cs_total = totals;
+  s->cs_scale = scale;
+  s->cs_vals = values;
+  s->cs_minv = inv_diag;
+  s->cs_rhs = rhs;
+  return FEM_OK;
+}
+
+int fem_refresh_component_scaling(fem_system* s, const int32_t* labels, const float* particle_mass, void* stream) {
+  if (!s || !labels || !particle_mass) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->cs_scale) return fail(FEM_ERR_STATE, "Prepare component scaling before refreshing it");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = (std::max(s->P, s->cs_cap) + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL(k_zero_doubles, dim3((s->cs_cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->cs_total, s->cs_cap);
+  if (s->P > 0) {
+    hipLaunchKernelGGL(k_component_mass, dim3(nb), dim3(BLOCK), 0, st, labels, particle_mass, s->P, s->cs_cap,
+                       s->cs_total);
+    hipLaunchKernelGGL(k_component_scale, dim3(nb), dim3(BLOCK), 0, st, labels, (const double*)s->cs_total, s->P,
+                       s->cs_cap, s->cs_scale);
+  }
+  HIP_TRY(hipGetLastError());
   return FEM_OK;
 }
 

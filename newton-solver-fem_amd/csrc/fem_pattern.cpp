@@ -26,54 +26,10 @@ static bool tet_valid(const float* mat) { return !(mat[0] == 0.f && mat[1] == 0.
 
 static bool spring_valid(const int32_t* idx) { return idx[0] >= 0 && idx[1] >= 0; }
 
-int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx, int S,
-                  const int32_t* spring_idx, int R, const int32_t* tri_idx, Pattern& pt, std::string& err) {
-  pt = Pattern();
-  pt.P = P;
-  // Spring and triangle incidences.
-  pt.sinc_ptr.assign(P + 1, 0);
-  pt.rinc_ptr.assign(P + 1, 0);
-  for (int s = 0; s < S; ++s) {
-    const int32_t* e = spring_idx + 2 * (size_t)s;
-    if (e[0] < -1 || e[1] < -1 || e[0] >= P || e[1] >= P) {
-      err = "spring_indices contains an out-of-range particle";
-      return 1;
-    }
-    if (!spring_valid(e)) continue;
-    pt.sinc_ptr[e[0] + 1]++;
-    pt.sinc_ptr[e[1] + 1]++;
-  }
-  for (int t = 0; t < R; ++t)
-    for (int a = 0; a < 3; ++a) {
-      int n = tri_idx[3 * (size_t)t + a];
-      if (n < 0 || n >= P) {
-        err = "tri_indices contains an out-of-range particle";
-        return 1;
-      }
-      pt.rinc_ptr[n + 1]++;
-    }
-  for (int i = 0; i < P; ++i) {
-    pt.sinc_ptr[i + 1] += pt.sinc_ptr[i];
-    pt.rinc_ptr[i + 1] += pt.rinc_ptr[i];
-  }
-  if (S >= (1 << 30) || R >= (1 << 29)) {
-    err = "spring or triangle count exceeds the packed incidence limit";
-    return 3;
-  }
-  pt.sinc.resize(pt.sinc_ptr[P]);
-  pt.rinc.resize(pt.rinc_ptr[P]);
-  {
-    std::vector<int64_t> fill(pt.sinc_ptr.begin(), pt.sinc_ptr.end() - 1);
-    for (int s = 0; s < S; ++s) {
-      const int32_t* e = spring_idx + 2 * (size_t)s;
-      if (!spring_valid(e)) continue;
-      pt.sinc[fill[e[0]]++] = (int32_t)((uint32_t)s << 1);
-      pt.sinc[fill[e[1]]++] = (int32_t)(((uint32_t)s << 1) | 1u);
-    }
-    std::vector<int64_t> rfill(pt.rinc_ptr.begin(), pt.rinc_ptr.end() - 1);
-    for (int t = 0; t < R; ++t)
-      for (int a = 0; a < 3; ++a) pt.rinc[rfill[tri_idx[3 * (size_t)t + a]]++] = (int32_t)(((uint32_t)t << 2) | a);
-  }
+// Tet4 and Hex8 incidence lists of every node (valid tets only), ascending
+// element order.
+static int element_incidences(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx,
+                              Pattern& pt, std::string& err) {
   // Incidence counts.
   pt.tinc_ptr.assign(P + 1, 0);
   pt.hinc_ptr.assign(P + 1, 0);
@@ -117,47 +73,14 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
     err = "element count exceeds the packed incidence limit";
     return 3;
   }
-  // Rows: sorted unique columns.
-  pt.offsets.assign(P + 1, 0);
-  pt.columns.clear();
-  pt.columns.reserve((size_t)pt.tinc_ptr[P] * 3 + (size_t)pt.hinc_ptr[P] * 7 + P);
-  std::vector<int32_t> cand;
-  pt.diag_rel.assign(P, -1);
-  int maxnnz = 0;
-  for (int i = 0; i < P; ++i) {
-    cand.clear();
-    cand.push_back(i);
-    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
-      int e = pt.tinc[k] >> 2;
-      for (int b = 0; b < 4; ++b) cand.push_back(tet_idx[4 * (size_t)e + b]);
-    }
-    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
-      int e = pt.hinc[k] >> 3;
-      for (int b = 0; b < 8; ++b) cand.push_back(hex_idx[8 * (size_t)e + b]);
-    }
-    for (int64_t k = pt.sinc_ptr[i]; k < pt.sinc_ptr[i + 1]; ++k) {
-      const int s = pt.sinc[k] >> 1;
-      cand.push_back(spring_idx[2 * (size_t)s + (1 - (pt.sinc[k] & 1))]);
-    }
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    int nnz = (int)cand.size();
-    maxnnz = std::max(maxnnz, nnz);
-    if (nnz > 255) {
-      char buf[128];
-      std::snprintf(buf, sizeof buf, "row %d has %d blocks; the assembly kernels support at most 255", i, nnz);
-      err = buf;
-      return 3;
-    }
-    pt.diag_rel[i] = (int32_t)(std::lower_bound(cand.begin(), cand.end(), i) - cand.begin());
-    pt.columns.insert(pt.columns.end(), cand.begin(), cand.end());
-    pt.offsets[i + 1] = (int64_t)pt.columns.size();
-    if (pt.offsets[i + 1] >= ((int64_t)1 << 31)) {
-      err = "more than 2^31 blocks";
-      return 3;
-    }
-  }
-  pt.max_row_nnz = maxnnz;
+  return 0;
+}
+
+// Relative slots of every incidence's blocks and the per-slot contribution
+// lists, against the rows already in pt (offsets/columns).  Returns 2 when an
+// element block has no slot in those rows.
+static int element_slots(const int32_t* tet_idx, const int32_t* hex_idx, Pattern& pt, std::string& err) {
+  const int P = pt.P;
   // Relative slots of every incidence's blocks.
   pt.tslots.resize(pt.tinc.size());
   for (int i = 0; i < P; ++i) {
@@ -168,8 +91,12 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
       uint32_t packed = 0;
       for (int b = 0; b < 4; ++b) {
         int c = tet_idx[4 * (size_t)e + b];
-        uint32_t rel = (uint32_t)(std::lower_bound(row, row + nnz, c) - row);
-        packed |= rel << (8 * b);
+        const int32_t* at = std::lower_bound(row, row + nnz, c);
+        if (at == row + nnz || *at != c) {
+          err = "an element block lies outside the sparse pattern";
+          return 2;
+        }
+        packed |= (uint32_t)(at - row) << (8 * b);
       }
       pt.tslots[k] = packed;
     }
@@ -230,6 +157,178 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
     }
   }
   return 0;
+}
+
+int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx, int S,
+                  const int32_t* spring_idx, int R, const int32_t* tri_idx, Pattern& pt, std::string& err) {
+  pt = Pattern();
+  pt.P = P;
+  // Spring and triangle incidences.
+  pt.sinc_ptr.assign(P + 1, 0);
+  pt.rinc_ptr.assign(P + 1, 0);
+  for (int s = 0; s < S; ++s) {
+    const int32_t* e = spring_idx + 2 * (size_t)s;
+    if (e[0] < -1 || e[1] < -1 || e[0] >= P || e[1] >= P) {
+      err = "spring_indices contains an out-of-range particle";
+      return 1;
+    }
+    if (!spring_valid(e)) continue;
+    pt.sinc_ptr[e[0] + 1]++;
+    pt.sinc_ptr[e[1] + 1]++;
+  }
+  for (int t = 0; t < R; ++t)
+    for (int a = 0; a < 3; ++a) {
+      int n = tri_idx[3 * (size_t)t + a];
+      if (n < 0 || n >= P) {
+        err = "tri_indices contains an out-of-range particle";
+        return 1;
+      }
+      pt.rinc_ptr[n + 1]++;
+    }
+  for (int i = 0; i < P; ++i) {
+    pt.sinc_ptr[i + 1] += pt.sinc_ptr[i];
+    pt.rinc_ptr[i + 1] += pt.rinc_ptr[i];
+  }
+  if (S >= (1 << 30) || R >= (1 << 29)) {
+    err = "spring or triangle count exceeds the packed incidence limit";
+    return 3;
+  }
+  pt.sinc.resize(pt.sinc_ptr[P]);
+  pt.rinc.resize(pt.rinc_ptr[P]);
+  {
+    std::vector<int64_t> fill(pt.sinc_ptr.begin(), pt.sinc_ptr.end() - 1);
+    for (int s = 0; s < S; ++s) {
+      const int32_t* e = spring_idx + 2 * (size_t)s;
+      if (!spring_valid(e)) continue;
+      pt.sinc[fill[e[0]]++] = (int32_t)((uint32_t)s << 1);
+      pt.sinc[fill[e[1]]++] = (int32_t)(((uint32_t)s << 1) | 1u);
+    }
+    std::vector<int64_t> rfill(pt.rinc_ptr.begin(), pt.rinc_ptr.end() - 1);
+    for (int t = 0; t < R; ++t)
+      for (int a = 0; a < 3; ++a) pt.rinc[rfill[tri_idx[3 * (size_t)t + a]]++] = (int32_t)(((uint32_t)t << 2) | a);
+  }
+  if (int rc = element_incidences(P, T, tet_idx, tet_mat, H, hex_idx, pt, err)) return rc;
+  // Rows: sorted unique columns.
+  pt.offsets.assign(P + 1, 0);
+  pt.columns.clear();
+  pt.columns.reserve((size_t)pt.tinc_ptr[P] * 3 + (size_t)pt.hinc_ptr[P] * 7 + P);
+  std::vector<int32_t> cand;
+  pt.diag_rel.assign(P, -1);
+  int maxnnz = 0;
+  for (int i = 0; i < P; ++i) {
+    cand.clear();
+    cand.push_back(i);
+    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
+      int e = pt.tinc[k] >> 2;
+      for (int b = 0; b < 4; ++b) cand.push_back(tet_idx[4 * (size_t)e + b]);
+    }
+    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
+      int e = pt.hinc[k] >> 3;
+      for (int b = 0; b < 8; ++b) cand.push_back(hex_idx[8 * (size_t)e + b]);
+    }
+    for (int64_t k = pt.sinc_ptr[i]; k < pt.sinc_ptr[i + 1]; ++k) {
+      const int s = pt.sinc[k] >> 1;
+      cand.push_back(spring_idx[2 * (size_t)s + (1 - (pt.sinc[k] & 1))]);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    int nnz = (int)cand.size();
+    maxnnz = std::max(maxnnz, nnz);
+    if (nnz > 255) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "row %d has %d blocks; the assembly kernels support at most 255", i, nnz);
+      err = buf;
+      return 3;
+    }
+    pt.diag_rel[i] = (int32_t)(std::lower_bound(cand.begin(), cand.end(), i) - cand.begin());
+    pt.columns.insert(pt.columns.end(), cand.begin(), cand.end());
+    pt.offsets[i + 1] = (int64_t)pt.columns.size();
+    if (pt.offsets[i + 1] >= ((int64_t)1 << 31)) {
+      err = "more than 2^31 blocks";
+      return 3;
+    }
+  }
+  pt.max_row_nnz = maxnnz;
+  return element_slots(tet_idx, hex_idx, pt, err);
+}
+
+// Reserved rows of a duplicated-node pool (streaming.py:487-516): the pool
+// holds P / count copies of `count` original nodes, pool node i standing for
+// original node i % count.  Row i holds i and every copy of every original
+// neighbour of i % count, the neighbours taken from the valid tets' original
+// connectivity; a later cut may connect any copies of those pairs without
+// changing the rows.  The current tets' incidences are assigned as well.
+int build_reserved_pattern(int P, int count, int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt,
+                           std::string& err) {
+  pt = Pattern();
+  pt.P = P;
+  if (count < 1 || P % count) {
+    err = "The particle pool must contain whole copies of the original node set";
+    return 1;
+  }
+  const int copies = P / count;
+  std::vector<std::vector<int32_t>> adj(count);
+  for (int e = 0; e < T; ++e) {
+    if (!tet_valid(tet_mat + 3 * (size_t)e)) continue;
+    int orig[4];
+    for (int a = 0; a < 4; ++a) {
+      const int n = tet_idx[4 * (size_t)e + a];
+      if (n < 0 || n >= P) {
+        err = "Tetrahedral node indices are outside the reserved particle pool";
+        return 1;
+      }
+      orig[a] = n % count;
+      for (int b = 0; b < a; ++b)
+        if (orig[b] == orig[a]) {
+          err = "Each tetrahedron must have four distinct original basis nodes";
+          return 1;
+        }
+    }
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        if (a != b) adj[orig[a]].push_back(orig[b]);
+  }
+  for (auto& v : adj) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  pt.offsets.assign(P + 1, 0);
+  pt.diag_rel.assign(P, -1);
+  std::vector<int32_t> cand;
+  int maxnnz = 0;
+  for (int i = 0; i < P; ++i) {
+    cand.assign(1, i);
+    for (int c = 0; c < copies; ++c)
+      for (int32_t nb : adj[i % count]) cand.push_back(nb + c * count);
+    std::sort(cand.begin(), cand.end());
+    const int nnz = (int)cand.size();
+    maxnnz = std::max(maxnnz, nnz);
+    if (nnz > 255) {
+      char buf[128];
+      std::snprintf(buf, sizeof buf, "row %d has %d blocks; the assembly kernels support at most 255", i, nnz);
+      err = buf;
+      return 3;
+    }
+    pt.diag_rel[i] = (int32_t)(std::lower_bound(cand.begin(), cand.end(), i) - cand.begin());
+    pt.columns.insert(pt.columns.end(), cand.begin(), cand.end());
+    pt.offsets[i + 1] = (int64_t)pt.columns.size();
+    if (pt.offsets[i + 1] >= ((int64_t)1 << 31)) {
+      err = "more than 2^31 blocks";
+      return 3;
+    }
+  }
+  pt.max_row_nnz = maxnnz;
+  return assign_tet_incidences(T, tet_idx, tet_mat, pt, err);
+}
+
+int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt, std::string& err) {
+  pt.sinc_ptr.assign(pt.P + 1, 0);
+  pt.sinc.clear();
+  pt.rinc_ptr.assign(pt.P + 1, 0);
+  pt.rinc.clear();
+  pt.hslots.clear();
+  if (int rc = element_incidences(pt.P, T, tet_idx, tet_mat, 0, nullptr, pt, err)) return rc;
+  return element_slots(tet_idx, nullptr, pt, err);
 }
 
 // Balanced contiguous row ranges: `nchunks` pieces of [r0, r1) with roughly

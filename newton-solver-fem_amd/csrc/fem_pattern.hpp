@@ -34,6 +34,16 @@ struct Pattern {
 int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, int H, const int32_t* hex_idx, int S,
                   const int32_t* spring_idx, int R, const int32_t* tri_idx, Pattern& pt, std::string& err);
 
+// Rows reserved for a pool of P / count copies of `count` original nodes
+// (every copy of every original neighbour), with the valid tets' incidences.
+int build_reserved_pattern(int P, int count, int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt,
+                           std::string& err);
+
+// Replaces the Tet4 incidences, slots and contributions of pt (tets only) for
+// new elements, keeping its rows; returns 2 when a valid tet has a block
+// outside them.
+int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt, std::string& err);
+
 void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds);
 
 }  // namespace fem

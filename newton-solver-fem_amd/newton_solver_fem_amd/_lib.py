@@ -77,7 +77,7 @@ SIGNATURES = {
     "fem_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fem_last_error": (ctypes.c_char_p, []),
     "fem_pattern_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip]),
-    "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_int64]),
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "fem_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fem_get_launch": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip]),
@@ -101,6 +101,10 @@ SIGNATURES = {
     "fem_reconstruct": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_diff_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int32]),
+    "fem_prepare_topology": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "fem_update_topology": (ctypes.c_int, [ctypes.c_void_p] * 9 + [ctypes.c_float, _ip, ctypes.c_void_p]),
+    "fem_bind_component_scaling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 5),
+    "fem_refresh_component_scaling": (ctypes.c_int, [ctypes.c_void_p] * 4),
 }
 
 _lib = None

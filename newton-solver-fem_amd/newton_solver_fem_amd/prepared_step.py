@@ -53,6 +53,8 @@ class PreparedFEMStep:
     def check_supported(solver):
         if solver.differentiable:
             raise RuntimeError("Prepared FEM steps are non-differentiable and cannot run inside a tape")
+        if solver._streaming or solver._component_scaling is not None:
+            raise ValueError("Prepared FEM steps require assembled, unscaled particle coordinates")
 
     def _check_current(self):
         self.check_supported(self.solver)

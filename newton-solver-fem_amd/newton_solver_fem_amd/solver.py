@@ -234,18 +234,18 @@ class SolverFEM:
         maxrow = ctypes.c_int32()
         _lib.check(self._lib.fem_pattern_size(self._sys, ctypes.byref(nnz), ctypes.byref(maxrow)), "fem_pattern_size")
         dev = self.device
-        offsets = torch.zeros(P + 1, dtype=torch.int32, device=dev)
-        columns = torch.zeros(max(int(nnz.value), 1), dtype=torch.int32, device=dev)
-        values = torch.zeros(max(int(nnz.value), 1), 3, 3, dtype=torch.float32, device=dev)
         self._pc_inv_diag = torch.zeros(max(P, 1), 3, 3, dtype=torch.float32, device=dev)
         self.dv = torch.zeros(max(P, 1), 3, dtype=torch.float32, device=dev)[:P]
-        _lib.check(
-            self._lib.fem_bind(self._sys, offsets.data_ptr(), columns.data_ptr(), values.data_ptr(),
-                               self._pc_inv_diag.data_ptr(), self.dv.data_ptr()),
-            "fem_bind",
-        )
-        n = int(nnz.value)
-        self.A_bsr = BSRMatrix(offsets, columns[:n], values[:n], P, P)
+        # Matrix storage sized like the reference's A_bsr nnz_capacity, the
+        # triplet count P + 2S + 3R + 16T + 64H (solver.py:318-325), so that a
+        # topology refresh re-sorts the pattern in place.
+        capacity = P + 2 * model.spring_count + 3 * model.tri_count + 16 * model.tet_count + 64 * self.hex_count
+        self._bind_matrix(max(capacity, int(nnz.value)))
+        self._streaming = False
+        self._component_scaling = None
+        self._stepped = False
+        self.last_update = {}
+        self._topology_storage = self._topology_signature()
         self._push_params()
         # Construction-time rest tangent (solver.py:324-341).
         _lib.check(self._lib.fem_assemble(self._sys, model.particle_q.data_ptr(), self.dt, _stream(dev)), "fem_assemble")
@@ -258,6 +258,165 @@ class SolverFEM:
         self._initial_volume = self._compute_volume(model.particle_q)
 
     # ------------------------------------------------------------------
+    def _bind_matrix(self, capacity: int):
+        """Allocate offsets / columns / values for `capacity` blocks and bind
+        them with the inverse diagonal and warm start."""
+        P = self.model.particle_count
+        dev = self.device
+        cap = max(int(capacity), 1)
+        self._A_storage = (torch.zeros(P + 1, dtype=torch.int32, device=dev),
+                           torch.zeros(cap, dtype=torch.int32, device=dev),
+                           torch.zeros(cap, 3, 3, dtype=torch.float32, device=dev))
+        offsets, columns, values = self._A_storage
+        _lib.check(
+            self._lib.fem_bind(self._sys, offsets.data_ptr(), columns.data_ptr(), values.data_ptr(),
+                               self._pc_inv_diag.data_ptr(), self.dv.data_ptr(), int(capacity)),
+            "fem_bind",
+        )
+        self._view_matrix()
+
+    def _pattern_nnz(self) -> int:
+        nnz = ctypes.c_int64()
+        maxrow = ctypes.c_int32()
+        _lib.check(self._lib.fem_pattern_size(self._sys, ctypes.byref(nnz), ctypes.byref(maxrow)), "fem_pattern_size")
+        return int(nnz.value)
+
+    def _view_matrix(self):
+        offsets, columns, values = self._A_storage
+        n = self._pattern_nnz()
+        P = self.model.particle_count
+        self.A_bsr = BSRMatrix(offsets, columns[:n], values[:n], P, P)
+
+    def _topology_signature(self):
+        """Counts and storage of the model arrays a topology refresh reads
+        (reference solver.py:859-868)."""
+        m = self.model
+        counts = (m.particle_count, m.tet_count, self.hex_count, m.spring_count, m.tri_count)
+        arrays = (m.particle_mass, m.particle_inv_mass, m.particle_flags, m.tet_indices, m.tet_poses, m.tet_materials)
+        return counts, tuple((a.data_ptr(), tuple(a.shape), a.dtype) for a in arrays)
+
+    def prepare_tetrahedral_updates(self, original_node_count: int) -> None:
+        """Reserve persistent sparse connections for duplicated Tet4 basis nodes
+        (reference solver.py:695-720, StreamingTetAssembly streaming.py:487-597).
+
+        Pool node ``i`` represents original node ``i % original_node_count``;
+        every row holds every copy of its original neighbours, so a later cut
+        may connect any copies of originally adjacent nodes.  Call before
+        capturing or stepping.  Tetrahedra only, no fixed particles.
+        """
+        if self.fixed_particle_indices.size:
+            raise ValueError("Streaming topology is not supported with fixed_particle_indices")
+        if self.differentiable:
+            raise NotImplementedError("Differentiation requires fixed topology without streaming/component scaling")
+        if torch.cuda.is_current_stream_capturing() or self._stepped:
+            raise RuntimeError("Prepare tetrahedral updates before capturing or stepping the solver")
+        if self._streaming:
+            raise RuntimeError("Tetrahedral updates are already prepared")
+        if self._component_scaling is not None:
+            raise RuntimeError("Prepare tetrahedral updates before component scaling")
+        if self.preconditioner_type != SUPPORTED_PRECONDITIONER:
+            raise ValueError("Streaming assembly requires the block_diag preconditioner")
+        torch.cuda.synchronize(self.device)
+        _lib.check(self._lib.fem_prepare_topology(self._sys, int(original_node_count)), "prepare_tetrahedral_updates")
+        self._streaming = True
+        self.original_node_count = int(original_node_count)
+        self._bind_matrix(self._pattern_nnz())
+        st = _stream(self.device)
+        _lib.check(self._lib.fem_assemble(self._sys, self.model.particle_q.data_ptr(), self.dt, st), "fem_assemble")
+        self._revision += 1
+
+    def refresh_topology(self, particle_q, dt=None, *, element_sources=None):
+        """Publish changed tetrahedral connectivity, rest poses, materials and
+        masses between graph launches (reference solver.py:870-935).
+
+        Model capacities and array storage must stay unchanged; unused tet
+        slots carry zero materials.  After :meth:`prepare_tetrahedral_updates`
+        the edit is validated against the reserved adjacency and rejected
+        before anything changes, the matrix storage and launch shapes stay
+        valid for an existing graph, and :attr:`last_update` reports the
+        reference's update counters; ``element_sources`` names each slot's
+        previous element (-1: new).  Otherwise the exact pattern is built
+        again within the reserved matrix storage.  Clears the warm start.
+        """
+        if self.differentiable:
+            raise NotImplementedError("Differentiation requires fixed topology without streaming/component scaling")
+        dt = self.dt if dt is None else float(dt)
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("refresh_topology must run outside CUDA graph capture")
+        if dt != self.dt:
+            raise ValueError("refresh_topology requires the solver's currently assembled timestep")
+        if self._topology_signature() != self._topology_storage:
+            raise ValueError("Topology updates must preserve model capacities and array storage")
+        P = self.model.particle_count
+        if (not isinstance(particle_q, torch.Tensor) or tuple(particle_q.shape) != (P, 3)
+                or particle_q.dtype != torch.float32 or particle_q.device != self.device):
+            raise ValueError("particle_q must match the model's particle shape and device")
+        sources = None
+        if self._streaming and element_sources is not None:
+            if (not isinstance(element_sources, torch.Tensor) or element_sources.dtype != torch.int32
+                    or element_sources.dim() != 1 or element_sources.numel() < self.model.tet_count):
+                raise ValueError("Element sources must be a capacity-sized int32 array")
+            if element_sources.device != self.device:
+                raise ValueError("Element sources must reside on the solver device")
+            sources = element_sources.contiguous()
+        m = self.model
+        st = _stream(self.device)
+        _lib.check(self._lib.fem_restore_fixed(self._sys, particle_q.data_ptr(), None, st), "fem_restore_fixed")
+        stats = (ctypes.c_int32 * 8)()
+        _lib.check(
+            self._lib.fem_update_topology(
+                self._sys, m.tet_indices.data_ptr(), m.tet_poses.data_ptr(), m.tet_materials.data_ptr(),
+                m.particle_mass.data_ptr(), m.particle_inv_mass.data_ptr(), m.particle_flags.data_ptr(),
+                particle_q.contiguous().data_ptr(), _ptr(sources), dt, stats, st),
+            "refresh_topology",
+        )
+        if self._streaming:
+            keys = ("dirty_elements", "removed_elements", "reused_elements", "active_elements", "touched_rows",
+                    "topology_inserted_elements", "topology_removed_elements")
+            self.last_update = {k: int(stats[i]) for i, k in enumerate(keys)}
+            self.last_update["full_numeric_refresh"] = bool(stats[7])
+        else:
+            self._view_matrix()
+        self.tet_materials = m.tet_materials.detach().clone()
+        self.dv.zero_()
+        self._revision += 1
+
+    def prepare_component_scaling(self, node_components: torch.Tensor, component_capacity: int) -> None:
+        """Balance disconnected pieces with widely differing masses (reference
+        solver.py:722-761, component_scaling.py): every equation of a
+        component is divided by the component's total mass, which leaves the
+        exact solution and the physical masses unchanged.  ``node_components``
+        holds persistent int32 labels (-1 for unused rows); after a topology
+        edit update them in place and call :meth:`refresh_component_scaling`.
+        """
+        if self.fixed_particle_indices.size:
+            raise ValueError("Component scaling is not supported with fixed_particle_indices")
+        if self.differentiable:
+            raise NotImplementedError("Differentiation requires fixed topology without streaming/component scaling")
+        if torch.cuda.is_current_stream_capturing() or self._stepped:
+            raise RuntimeError("Prepare component scaling before capturing or stepping the solver")
+        if self._component_scaling is not None:
+            raise RuntimeError("Component scaling is already prepared")
+        if self.preconditioner_type != SUPPORTED_PRECONDITIONER:
+            raise ValueError("Component scaling requires block_diag preconditioning")
+        if not isinstance(component_capacity, int) or isinstance(component_capacity, bool) or component_capacity < 1:
+            raise ValueError("component_capacity must be a positive integer")
+        P = self.model.particle_count
+        if (not isinstance(node_components, torch.Tensor) or node_components.dtype != torch.int32
+                or tuple(node_components.shape) != (P,) or node_components.device != self.device
+                or not node_components.is_contiguous()):
+            raise ValueError("node_components must be a particle-sized int32 array on the model device")
+        from .component_scaling import ComponentScaling
+
+        self._component_scaling = ComponentScaling(self, node_components, component_capacity)
+
+    def refresh_component_scaling(self) -> None:
+        """Recompute the component masses and factors on the device after a
+        topology edit (reference solver.py:763-767)."""
+        if self._component_scaling is None:
+            raise RuntimeError("Prepare component scaling before refreshing it")
+        self._component_scaling.refresh()
+
     def _push_params(self):
         p = _lib.Params()
         p.min_stretch = self.min_stretch
@@ -427,6 +586,8 @@ class SolverFEM:
         if dt != self.dt:
             if self.differentiable:
                 raise ValueError("Differentiable tangent refresh requires the construction timestep")
+            if self._streaming:
+                raise ValueError("Streaming tangent refresh requires the construction timestep")
             self.dv.zero_()
         st = _stream(self.device)
         _lib.check(self._lib.fem_restore_fixed(self._sys, particle_q.data_ptr(), None, st), "fem_restore_fixed")
@@ -439,11 +600,6 @@ class SolverFEM:
     def rebuild_element_blocks(self, particle_q, dt, *, update_preconditioner=True):
         """Reassemble the deformation-dependent tangent (reference solver.py:506-508)."""
         self.refresh_tangent(particle_q, dt)
-
-    def refresh_topology(self, particle_q, dt=None, *, element_sources=None):
-        """Same connectivity: equivalent to a tangent refresh and a cleared warm start."""
-        self.refresh_tangent(particle_q, dt)
-        self.dv.zero_()
 
     def _early_exit(self) -> bool:
         # Device-side residual exit: the reference stops early with the CUDA
@@ -487,6 +643,7 @@ class SolverFEM:
             if act.dtype != torch.float32 or act.numel() != self.model.tet_count or act.device != self.device:
                 raise ValueError("control.tet_activations must be a float32 (tet_count,) tensor on the model device")
             act = act.contiguous()
+        self._stepped = True
         st = _stream(self.device)
         _lib.check(
             self._lib.fem_step(self._sys, state_in.particle_q.data_ptr(), state_in.particle_qd.data_ptr(),

@@ -79,3 +79,85 @@ def brute_pattern(P, tets=(), hexes=(), tet_materials=None):
     offsets = np.zeros(P + 1, dtype=np.int64)
     np.add.at(offsets, rows + 1, 1)
     return np.cumsum(offsets).astype(np.int32), cols.astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Duplicated-node pool of the reference's tests/test_reference_streaming.py:
+# three copies of five corners (15 particles) and six tet slots; a record is
+# (family, connectivity, volume fraction).
+POOL_ORIGINAL = 5
+POOL_CAPACITY = 6
+POOL_CORNERS = np.asarray(((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (0.2, 0.3, -1)), dtype=np.float32)
+POOL_INITIAL = ((0, (0, 1, 2, 3), 1.0), (1, (0, 2, 1, 4), 1.0))
+POOL_SPLIT = ((1, (0, 2, 1, 4), 1.0), (0, (5, 6, 7, 8), 0.6), (0, (0, 1, 2, 3), 0.4))
+
+
+def pool_arrays(records):
+    """Tet slots, rest inverses, materials and lumped masses of a record set
+    (the reference test's _edit: a negative rest determinant swaps corners 1
+    and 2; unused slots keep zero materials and identity poses; particles
+    without mass get mass 1, inverse mass 0 and no active flag)."""
+    P = 3 * POOL_ORIGINAL
+    indices = np.zeros((POOL_CAPACITY, 4), dtype=np.int32)
+    poses = np.tile(np.eye(3), (POOL_CAPACITY, 1, 1))
+    materials = np.zeros((POOL_CAPACITY, 3))
+    fractions = np.zeros(POOL_CAPACITY)
+    masses = np.zeros(P)
+    rest = np.tile(POOL_CORNERS.astype(np.float64), (3, 1))
+    for slot, (family, connectivity, fraction) in enumerate(records):
+        nodes = list(connectivity)
+        edges = (rest[nodes[1:]] - rest[nodes[0]]).T
+        if np.linalg.det(edges) < 0:
+            nodes[1], nodes[2] = nodes[2], nodes[1]
+            edges = (rest[nodes[1:]] - rest[nodes[0]]).T
+        indices[slot] = nodes
+        poses[slot] = np.linalg.inv(edges)
+        materials[slot] = fraction * np.asarray((1000.0 + 300.0 * family, 1500.0 + 200.0 * family, 0.0))
+        fractions[slot] = fraction
+        masses[nodes] += fraction * 200.0 * np.linalg.det(edges) / 24.0
+    return {
+        "tet_indices": indices,
+        "tet_poses": poses.astype(np.float32),
+        "tet_materials": materials.astype(np.float32),
+        "tet_weights": fractions.astype(np.float32),
+        "particle_mass": np.where(masses > 0, masses, 1.0).astype(np.float32),
+        "particle_inv_mass": np.divide(1.0, masses, out=np.zeros_like(masses), where=masses > 0).astype(np.float32),
+        "particle_flags": (masses > 0).astype(np.int32),
+    }
+
+
+def pool_model(records=POOL_INITIAL, device="cuda"):
+    """The 15-particle pool with `records` in its six tet slots."""
+    from newton_solver_fem_amd.model import Model
+
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    for point in np.tile(POOL_CORNERS, (3, 1)):
+        b.add_particle(point, (0.0, 0.0, 0.0), mass=1.0, radius=0.0)
+    host = b._host_arrays()
+    arr = pool_arrays(records)
+    for k in ("tet_indices", "tet_poses", "tet_materials", "particle_mass", "particle_inv_mass", "particle_flags"):
+        host[k] = arr[k]
+    host["tet_activations"] = np.zeros(POOL_CAPACITY, np.float32)
+    return Model(device, host, b.gravity)
+
+
+def pool_edit(model, records):
+    """Publish `records` into the model's existing arrays in place."""
+    import torch
+
+    arr = pool_arrays(records)
+    for k in ("tet_indices", "tet_poses", "tet_materials", "particle_mass", "particle_inv_mass", "particle_flags"):
+        getattr(model, k).copy_(torch.as_tensor(arr[k]).reshape(getattr(model, k).shape))
+        model.host[k] = arr[k]
+    return arr
+
+
+def pool_deformed(model):
+    """The reference test's _deformed positions (shear and squash)."""
+    import torch
+
+    q = model.particle_q.cpu().numpy().astype(np.float32).copy()
+    q[:, 0] += 0.17 * q[:, 2]
+    q[:, 1] -= 0.08 * q[:, 0]
+    q[:, 2] *= 0.91
+    return torch.as_tensor(q, device=model.particle_q.device)
