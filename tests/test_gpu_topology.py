@@ -525,3 +525,29 @@ def test_topology_refresh_retains_scaled_connectivity_storage(streaming):
     f3 = np.repeat(cs.scale.cpu().numpy().astype(np.float64), 3)
     np.testing.assert_allclose(cs.matrix.dense(), solver.A_bsr.dense() * f3[:, None], rtol=1e-7, atol=0)
     assert np.all(np.isfinite(np_(following.particle_q)))
+
+
+@pytest.mark.parametrize("method", ["refresh_tangent", "refresh_topology", "rebuild_element_blocks"])
+def test_reassembly_uses_the_same_fixed_positions_as_force_evaluation(method):
+    """Reference tests/test_fixed_particles.py:122-140: every reassembly entry
+    restores the anchors of fixed particles in the positions it is handed."""
+    print(f"case fixed reassembly {method}", flush=True)
+    from helpers import tet_pair
+
+    model = tet_pair(DEV)
+    kw = dict(dt=0.002, fixed_particle_indices=(0, 1), linear_solver_maxiter=64)
+    expected, actual = solver_for(model, **kw), solver_for(model, **kw)
+    reference, changed = model.state(), model.state()
+    positions = model.particle_q.cpu().numpy().copy()
+    positions[2] += (0.002, 0.001, -0.001)
+    reference.particle_q.copy_(torch.tensor(positions))
+    positions[:2] += (0.03, -0.02, 0.04)
+    changed.particle_q.copy_(torch.tensor(positions))
+    getattr(expected, method)(reference.particle_q, 0.002)
+    getattr(actual, method)(changed.particle_q, 0.002)
+    np.testing.assert_array_equal(changed.particle_q.cpu().numpy(), reference.particle_q.cpu().numpy())
+    np.testing.assert_array_equal(actual.A_bsr.dense(), expected.A_bsr.dense())
+    expected_out, actual_out = model.state(), model.state()
+    expected.step(reference, expected_out, None, None, 0.002)
+    actual.step(changed, actual_out, None, None, 0.002)
+    np.testing.assert_allclose(np_(actual_out.particle_qd), np_(expected_out.particle_qd), atol=1e-7)
