@@ -128,6 +128,13 @@ int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float*
 
 int fem_set_params(fem_system* sys, const fem_params* params);
 
+/* Gravity read from caller-owned device memory (float[3]) by every later
+ * kernel instead of fem_params.gravity, so that a recorded graph follows
+ * changes made between replays: the reference's Model.gravity is a device
+ * array read by eval_gravity_from_array and the ground friction cap
+ * (kernels.py:1132-1195, :1273-1286).  NULL returns to fem_params.gravity. */
+int fem_bind_gravity(fem_system* sys, const float* gravity_dev);
+
 /* Launch shapes (0 keeps the current value): lanes per block row of the
  * force/RHS kernel and the maximum rows per workgroup of the assembly (powers
  * of two in [4, 64]; consecutive rows are packed up to that count while their

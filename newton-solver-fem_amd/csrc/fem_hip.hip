@@ -108,6 +108,9 @@ struct RowArgs {
   float lin_damp;
   float grav[3];
   float gmag;
+  // Device gravity vector read at run time (Model.gravity is a device array
+  // in the reference, so a captured graph sees set_gravity); NULL: grav/gmag.
+  const float* gravity;
   int use_act;
 };
 
@@ -532,14 +535,18 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
         if (vs > 0.f) vt = vt * (1.f / vs);
         const float inv_m = a.inv_mass[i];
         const float m = inv_m > 0.f ? 1.f / fmaxf(inv_m, 1e-9f) : 0.f;
-        const float fn_eff = fminf(fabsf(fn), m * a.gmag);
+        const float gmag = a.gravity ? sqrtf(a.gravity[0] * a.gravity[0] + a.gravity[1] * a.gravity[1] +
+                                             a.gravity[2] * a.gravity[2])
+                                     : a.gmag;
+        const float fn_eff = fminf(fabsf(fn), m * gmag);
         const float ftan = fminf(vs * a.kf, a.gmu * fn_eff);
         fg = fg - n * fn - vt * ftan;
       }
     }
     if ((terms & FEM_TERM_GRAVITY) && on) {
       const float m = a.mass[i];
-      fgr = {a.grav[0] * m, a.grav[1] * m, a.grav[2] * m};
+      if (a.gravity) fgr = {a.gravity[0] * m, a.gravity[1] * m, a.gravity[2] * m};
+      else fgr = {a.grav[0] * m, a.grav[1] * m, a.grav[2] * m};
     }
     if ((terms & FEM_TERM_DAMPING) && on && a.lin_damp > 0.f) fdm = vi * (-a.lin_damp * a.mass[i]);
     v3 f;
@@ -702,7 +709,10 @@ __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __re
     if ((a.flags[i] & 1) != 0) {
       if (a.has_ground) {
         v3 gx, gv;
-        ground_vjp(ld3(x, i), ld3(v, i), zi * h, a.radius[i], a.inv_mass[i], a.gn, a.ke, a.kd, a.kf, a.gmu, a.gmag,
+        const float gmag = a.gravity ? sqrtf(a.gravity[0] * a.gravity[0] + a.gravity[1] * a.gravity[1] +
+                                             a.gravity[2] * a.gravity[2])
+                                     : a.gmag;
+        ground_vjp(ld3(x, i), ld3(v, i), zi * h, a.radius[i], a.inv_mass[i], a.gn, a.ke, a.kd, a.kf, a.gmu, gmag,
                    gx, gv);
         xb = xb + gx;
         vb = vb + gv;
@@ -1886,6 +1896,7 @@ struct fem_system {
   float *d_sp_rest = nullptr, *d_sp_ke = nullptr, *d_sp_kd = nullptr, *d_tri_pose = nullptr, *d_tri_mat = nullptr,
         *d_row_tri = nullptr;
   float2* d_slot_spring = nullptr;
+  const float* d_gravity = nullptr;  // caller-owned device gravity (fem_bind_gravity)
   // Kernels with Hex8, spring or triangle code (EL_ANY) are needed.
   bool full() const { return H > 0 || S > 0 || R > 0; }
   bool tri_aero = false;  // a triangle with drag or lift (no reverse products for those)
@@ -2019,6 +2030,7 @@ struct fem_system {
     for (int k = 0; k < 3; ++k) a.grav[k] = prm.gravity[k];
     a.gmag = sqrtf(prm.gravity[0] * prm.gravity[0] + prm.gravity[1] * prm.gravity[1] +
                    prm.gravity[2] * prm.gravity[2]);
+    a.gravity = d_gravity;
     a.use_act = prm.enable_tet_activations;
     return a;
   }
@@ -2611,6 +2623,12 @@ int fem_bind(fem_system* s, int32_t* offsets, int32_t* columns, float* values, f
   HIP_TRY(hipMemcpy(offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
   if (!s->pt.columns.empty())
     HIP_TRY(hipMemcpy(columns, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
+  return FEM_OK;
+}
+
+int fem_bind_gravity(fem_system* s, const float* gravity) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  s->d_gravity = gravity;
   return FEM_OK;
 }
 

@@ -79,6 +79,7 @@ SIGNATURES = {
     "fem_pattern_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip]),
     "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_int64]),
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
+    "fem_bind_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fem_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fem_get_launch": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip]),
     "fem_set_cg_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

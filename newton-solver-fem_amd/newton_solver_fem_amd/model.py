@@ -95,7 +95,9 @@ class Model:
     def __init__(self, device, host: dict, gravity):
         self.device = torch.device(device)
         self.host = host
-        self.gravity = tuple(float(g) for g in gravity)
+        # Device copy of gravity that solvers read at run time.
+        self.gravity_array = torch.zeros(3, dtype=torch.float32, device=self.device)
+        self.gravity = gravity
         self.particle_count = int(len(host["particle_q"]))
         self.tet_count = int(len(host["tet_indices"]))
         self.spring_count = int(len(host["spring_indices"]))
@@ -132,8 +134,20 @@ class Model:
             t(host["hex_det_J0_w"], torch.float32).reshape(-1, 8),
         )
 
+    @property
+    def gravity(self) -> tuple:
+        return self._gravity
+
+    @gravity.setter
+    def gravity(self, value):
+        """Setting gravity updates ``gravity_array`` in place: solvers read it on
+        the device, so graphs recorded earlier use the new value (the
+        reference's Model.gravity is a device array)."""
+        self._gravity = tuple(float(g) for g in value)
+        self.gravity_array.copy_(torch.tensor(self._gravity, dtype=torch.float32))
+
     def set_gravity(self, gravity):
-        self.gravity = tuple(float(g) for g in gravity)
+        self.gravity = gravity
 
     def state(self, requires_grad: bool = False) -> State:
         q = self.particle_q.clone()

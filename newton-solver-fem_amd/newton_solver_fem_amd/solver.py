@@ -246,6 +246,7 @@ class SolverFEM:
         self._stepped = False
         self.last_update = {}
         self._topology_storage = self._topology_signature()
+        _lib.check(self._lib.fem_bind_gravity(self._sys, model.gravity_array.data_ptr()), "fem_bind_gravity")
         self._push_params()
         # Construction-time rest tangent (solver.py:324-341).
         _lib.check(self._lib.fem_assemble(self._sys, model.particle_q.data_ptr(), self.dt, _stream(dev)), "fem_assemble")

@@ -128,3 +128,52 @@ def test_captured_elastic_update_matches_fixed_budget_with_live_tangent():
         np.testing.assert_array_equal(following.particle_qd.cpu().numpy()[:2], np.zeros((2, 3)))
         assert stats["iterations"][0] < actual.linear_solver_maxiter
         assert stats["residual_sq"][0] <= stats["atol_sq"][0]
+
+
+def particle_solver(mass, dt):
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    b.add_particle((0.0, 0.0, 1.0), (0.0, 0.0, 0.0), mass=mass, radius=0.0)
+    m = b.finalize(DEV)
+    return m, solver_for(m, dt=dt, linear_solver_maxiter=8)
+
+
+@pytest.mark.parametrize("mass", [1e-25, 1e-18, 1e-14, 1.0])
+def test_tiny_physical_mass_falls_with_same_gravity(mass):
+    """Reference tests/test_reference_block_preconditioner.py:71-91: the
+    right-hand-side rescaling keeps free fall exact down to 1e-25 kg, and a
+    zero right-hand side clears the warm start."""
+    print(f"case tiny mass {mass}", flush=True)
+    dt = 1 / 240
+    m, s = particle_solver(mass, dt)
+    current, following = m.state(), m.state()
+    pointers = (s.dv.data_ptr(), s._pc_inv_diag.data_ptr())
+    for step in range(1, 4):
+        s.step(current, following, m.control(), None, dt)
+        np.testing.assert_allclose(following.particle_qd.cpu().numpy()[0], (0, 0, -step * 9.81 * dt), atol=1e-7)
+        np.testing.assert_allclose(s.dv.cpu().numpy()[0], (0, 0, -9.81 * dt), atol=1e-7)
+        np.testing.assert_allclose(current.particle_f.cpu().numpy()[0], (0, 0, -mass * 9.81 * dt), rtol=3e-7, atol=0)
+        assert pointers == (s.dv.data_ptr(), s._pc_inv_diag.data_ptr())
+        current, following = following, current
+    m.set_gravity((0, 0, 0))
+    s.step(current, following, m.control(), None, dt)
+    np.testing.assert_allclose(following.particle_qd.cpu().numpy(), current.particle_qd.cpu().numpy(), atol=1e-7)
+    np.testing.assert_array_equal(s.dv.cpu().numpy(), np.zeros((1, 3)))
+
+
+def test_captured_tiny_rhs_replays_with_live_gravity_and_stable_buffers():
+    """Reference tests/test_reference_block_preconditioner.py:93-108: gravity
+    changed between replays of a recorded step is read live by the graph."""
+    print("case capture live gravity", flush=True)
+    dt = 1 / 240
+    m, s = particle_solver(1e-25, dt)
+    current, following = m.state(), m.state()
+    s.step(current, following, m.control(), None, dt)
+    pointers = (s.dv.data_ptr(), s._pc_inv_diag.data_ptr(), m.gravity_array.data_ptr())
+    g = capture(lambda: s.step(current, following, None, None, dt))
+    for gravity in (-9.81, 0.0, -4.0):
+        m.set_gravity((0, 0, gravity))
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(following.particle_qd.cpu().numpy()[0], (0, 0, gravity * dt), atol=1e-7)
+        np.testing.assert_allclose(s.dv.cpu().numpy()[0], (0, 0, gravity * dt), atol=1e-7)
+        assert pointers == (s.dv.data_ptr(), s._pc_inv_diag.data_ptr(), m.gravity_array.data_ptr())
