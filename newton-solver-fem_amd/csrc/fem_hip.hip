@@ -1935,6 +1935,7 @@ struct fem_system {
   // Reserved topology pool (streaming.py): original node count, the last
   // published elements (host) and the positions of the last assembly (device).
   int topo_count = 0;
+  Pattern spare;  // incidence storage of the previous edit
   std::vector<int32_t> old_nodes;
   std::vector<float> old_poses, old_mats;
   std::vector<unsigned char> old_active;
@@ -3204,17 +3205,15 @@ int fem_update_topology(fem_system* s, const int32_t* tet_indices, const float* 
                                  "Cut introduces an edge outside the reserved original-node adjacency",
                                  "A tetrahedron must have four distinct original basis nodes"};
     if (int code = stage_elements(s, nodes, poses, mats, src, q_now, q_lin, counters)) return fail(FEM_ERR_ARG, msg[code]);
-    np.P = P;
-    np.offsets = s->pt.offsets;
-    np.columns = s->pt.columns;
-    np.diag_rel = s->pt.diag_rel;
-    np.max_row_nnz = s->pt.max_row_nnz;
-    if (int rc = assign_tet_incidences(T, nodes.data(), mats.data(), np, err)) return fail(FEM_ERR_ARG, err);
+    // New incidences into the spare pattern (its storage is reused from edit
+    // to edit), then exchanged with the current ones; the previous ones
+    // become the spare.
+    if (int rc = assign_tet_incidences(T, nodes.data(), mats.data(), s->pt, s->spare, err))
+      return fail(FEM_ERR_ARG, err);
+    swap_incidences(s->pt, s->spare);
     // Assembly groups of the new incidences within the reserved launch.
-    Pattern old = std::move(s->pt);
-    s->pt = std::move(np);
     if (int rc = s->plan_assemble(s->asm_rb)) {
-      s->pt = std::move(old);
+      swap_incidences(s->pt, s->spare);
       return rc;
     }
     // Commit: incidences into their reserved storage, element and particle data.

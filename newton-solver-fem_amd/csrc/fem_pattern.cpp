@@ -19,6 +19,9 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <atomic>
+#include <thread>
 
 namespace fem {
 
@@ -76,86 +79,145 @@ static int element_incidences(int P, int T, const int32_t* tet_idx, const float*
   return 0;
 }
 
-// Relative slots of every incidence's blocks and the per-slot contribution
-// lists, against the rows already in pt (offsets/columns).  Returns 2 when an
-// element block has no slot in those rows.
-static int element_slots(const int32_t* tet_idx, const int32_t* hex_idx, Pattern& pt, std::string& err) {
-  const int P = pt.P;
-  // Relative slots of every incidence's blocks.
-  pt.tslots.resize(pt.tinc.size());
-  for (int i = 0; i < P; ++i) {
-    const int32_t* row = pt.columns.data() + pt.offsets[i];
-    int nnz = (int)(pt.offsets[i + 1] - pt.offsets[i]);
+// Rows [r0, r1) of element_slots: relative slot bytes of the rows'
+// incidences, the per-slot counts and their prefix inside each row (the
+// slot_ptr entries of a row's slots are private to that row), and the row's
+// total.  Returns 2 when an element block has no slot in the row.
+static int slots_rows(int r0, int r1, const int32_t* tet_idx, const int32_t* hex_idx, const Pattern& rows,
+                      Pattern& pt, int64_t* row_total) {
+  for (int i = r0; i < r1; ++i) {
+    const int32_t* row = rows.columns.data() + rows.offsets[i];
+    const int nnz = (int)(rows.offsets[i + 1] - rows.offsets[i]);
+    int32_t* cnt = pt.slot_ptr.data() + rows.offsets[i] + 1;  // cnt[s] counts slot s of row i
+    std::fill(cnt, cnt + nnz, 0);
     for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
-      int e = pt.tinc[k] >> 2;
+      const int e = pt.tinc[k] >> 2;
       uint32_t packed = 0;
       for (int b = 0; b < 4; ++b) {
-        int c = tet_idx[4 * (size_t)e + b];
+        const int c = tet_idx[4 * (size_t)e + b];
         const int32_t* at = std::lower_bound(row, row + nnz, c);
-        if (at == row + nnz || *at != c) {
-          err = "an element block lies outside the sparse pattern";
-          return 2;
-        }
+        if (at == row + nnz || *at != c) return 2;
         packed |= (uint32_t)(at - row) << (8 * b);
+        ++cnt[at - row];
       }
       pt.tslots[k] = packed;
     }
-  }
-  pt.hslots.resize(2 * pt.hinc.size());
-  for (int i = 0; i < P; ++i) {
-    const int32_t* row = pt.columns.data() + pt.offsets[i];
-    int nnz = (int)(pt.offsets[i + 1] - pt.offsets[i]);
     for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
-      int e = pt.hinc[k] >> 3;
+      const int e = pt.hinc[k] >> 3;
       uint32_t lo = 0, hi = 0;
       for (int b = 0; b < 8; ++b) {
-        int c = hex_idx[8 * (size_t)e + b];
-        uint32_t rel = (uint32_t)(std::lower_bound(row, row + nnz, c) - row);
+        const int c = hex_idx[8 * (size_t)e + b];
+        const int32_t* at = std::lower_bound(row, row + nnz, c);
+        if (at == row + nnz || *at != c) return 2;
+        const uint32_t rel = (uint32_t)(at - row);
         if (b < 4) lo |= rel << (8 * b);
         else hi |= rel << (8 * (b - 4));
+        ++cnt[rel];
       }
       pt.hslots[2 * k] = lo;
       pt.hslots[2 * k + 1] = hi;
     }
+    int64_t run = 0;
+    for (int s = 0; s < nnz; ++s) {
+      run += cnt[s];
+      cnt[s] = (int32_t)run;
+    }
+    row_total[i] = run;
   }
-  // Slot -> contributing element blocks.
-  const size_t nnz = (size_t)pt.offsets[P];
-  pt.slot_ptr.assign(nnz + 1, 0);
-  for (int i = 0; i < P; ++i) {
+  return 0;
+}
+
+// Rows [r0, r1): the row base added to the row's slot_ptr entries, then each
+// slot's contributions in incidence order (tets, then hexes).
+static void contrib_rows(int r0, int r1, const Pattern& rows, Pattern& pt, const int64_t* row_base) {
+  int32_t cur[256];
+  for (int i = r0; i < r1; ++i) {
+    const int64_t o = rows.offsets[i];
+    const int nnz = (int)(rows.offsets[i + 1] - o);
+    int32_t* sp = pt.slot_ptr.data() + o;  // sp[s] = start of slot s once the base is added
+    const int32_t base = (int32_t)row_base[i];
+    for (int s = 1; s <= nnz; ++s) sp[s] += base;
+    for (int s = 0; s < nnz; ++s) cur[s] = s == 0 ? base : sp[s];
+    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
+      const uint32_t t = (uint32_t)(k - pt.tinc_ptr[i]);
+      for (int b = 0; b < 4; ++b)
+        pt.contrib[cur[(pt.tslots[k] >> (8 * b)) & 255u]++] = (uint16_t)((t << 3) | (uint32_t)b);
+    }
+    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
+      const uint32_t t = (uint32_t)(k - pt.hinc_ptr[i]);
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t w = pt.hslots[2 * k + (b >> 2)];
+        pt.contrib[cur[(w >> (8 * (b & 3))) & 255u]++] = (uint16_t)(0x8000u | (t << 3) | (uint32_t)b);
+      }
+    }
+  }
+}
+
+// Runs fn(r0, r1) over `parts` contiguous row ranges of about equal work
+// (incidences and blocks), on that many threads.
+template <typename F>
+static void for_row_ranges(const Pattern& rows, const Pattern& pt, int parts, F fn) {
+  const int P = pt.P;
+  if (parts <= 1 || P < 2048) {
+    fn(0, P);
+    return;
+  }
+  auto work = [&](int r) { return (double)(4 * pt.tinc_ptr[r] + 8 * pt.hinc_ptr[r] + rows.offsets[r]) + r; };
+  const double total = work(P);
+  std::vector<int> cut(parts + 1, P);
+  cut[0] = 0;
+  for (int j = 1; j < parts; ++j) {
+    int lo = cut[j - 1], hi = P;
+    const double target = total * j / parts;
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if (work(mid) < target) lo = mid + 1;
+      else hi = mid;
+    }
+    cut[j] = lo;
+  }
+  std::vector<std::thread> threads;
+  for (int j = 1; j < parts; ++j) threads.emplace_back(fn, cut[j], cut[j + 1]);
+  fn(cut[0], cut[1]);
+  for (auto& t : threads) t.join();
+}
+
+// Relative slots of every incidence's blocks and the per-slot contribution
+// lists of pt's incidences, against the rows of `rows` (offsets/columns; may be
+// pt itself), in fixed order whatever the thread count.  Returns 2 when an
+// element block has no slot in those rows.
+static int element_slots(const int32_t* tet_idx, const int32_t* hex_idx, const Pattern& rows, Pattern& pt,
+                         std::string& err) {
+  const int P = pt.P;
+  for (int i = 0; i < P; ++i)
     if (pt.tinc_ptr[i + 1] - pt.tinc_ptr[i] > 4096 || pt.hinc_ptr[i + 1] - pt.hinc_ptr[i] > 4096) {
       err = "a node belongs to more than 4096 elements of one kind";
       return 3;
     }
-    for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k)
-      for (int b = 0; b < 4; ++b) pt.slot_ptr[pt.offsets[i] + ((pt.tslots[k] >> (8 * b)) & 255u) + 1]++;
-    for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k)
-      for (int b = 0; b < 8; ++b) {
-        uint32_t w = pt.hslots[2 * k + (b >> 2)];
-        pt.slot_ptr[pt.offsets[i] + ((w >> (8 * (b & 3))) & 255u) + 1]++;
-      }
+  const size_t nnz = (size_t)rows.offsets[P];
+  pt.tslots.resize(pt.tinc.size());
+  pt.hslots.resize(2 * pt.hinc.size());
+  pt.slot_ptr.resize(nnz + 1);  // every entry but the first is written per row
+  std::vector<int64_t> row_total(P + 1, 0);
+  const int parts = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  std::atomic<int> failed{0};
+  for_row_ranges(rows, pt, parts, [&](int r0, int r1) {
+    if (slots_rows(r0, r1, tet_idx, hex_idx, rows, pt, row_total.data())) failed.store(1);
+  });
+  if (failed.load()) {
+    err = "an element block lies outside the sparse pattern";
+    return 2;
   }
-  for (size_t k = 0; k < nnz; ++k) pt.slot_ptr[k + 1] += pt.slot_ptr[k];
-  pt.contrib.resize(pt.slot_ptr[nnz]);
-  {
-    std::vector<int32_t> fill(pt.slot_ptr.begin(), pt.slot_ptr.end() - 1);
-    for (int i = 0; i < P; ++i) {
-      for (int64_t k = pt.tinc_ptr[i]; k < pt.tinc_ptr[i + 1]; ++k) {
-        const uint32_t t = (uint32_t)(k - pt.tinc_ptr[i]);
-        for (int b = 0; b < 4; ++b) {
-          const int64_t slot = pt.offsets[i] + ((pt.tslots[k] >> (8 * b)) & 255u);
-          pt.contrib[fill[slot]++] = (uint16_t)((t << 3) | (uint32_t)b);
-        }
-      }
-      for (int64_t k = pt.hinc_ptr[i]; k < pt.hinc_ptr[i + 1]; ++k) {
-        const uint32_t t = (uint32_t)(k - pt.hinc_ptr[i]);
-        for (int b = 0; b < 8; ++b) {
-          uint32_t w = pt.hslots[2 * k + (b >> 2)];
-          const int64_t slot = pt.offsets[i] + ((w >> (8 * (b & 3))) & 255u);
-          pt.contrib[fill[slot]++] = (uint16_t)(0x8000u | (t << 3) | (uint32_t)b);
-        }
-      }
-    }
+  // Exclusive scan of the row totals: the first contribution of each row.
+  int64_t run = 0;
+  for (int i = 0; i < P; ++i) {
+    const int64_t t = row_total[i];
+    row_total[i] = run;
+    run += t;
   }
+  pt.contrib.resize((size_t)run);
+  for_row_ranges(rows, pt, parts, [&](int r0, int r1) { contrib_rows(r0, r1, rows, pt, row_total.data()); });
+  pt.slot_ptr[0] = 0;
   return 0;
 }
 
@@ -249,7 +311,7 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
     }
   }
   pt.max_row_nnz = maxnnz;
-  return element_slots(tet_idx, hex_idx, pt, err);
+  return element_slots(tet_idx, hex_idx, pt, pt, err);
 }
 
 // Reserved rows of a duplicated-node pool (streaming.py:487-516): the pool
@@ -318,17 +380,34 @@ int build_reserved_pattern(int P, int count, int T, const int32_t* tet_idx, cons
     }
   }
   pt.max_row_nnz = maxnnz;
-  return assign_tet_incidences(T, tet_idx, tet_mat, pt, err);
+  return assign_tet_incidences(T, tet_idx, tet_mat, pt, pt, err);
 }
 
-int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt, std::string& err) {
-  pt.sinc_ptr.assign(pt.P + 1, 0);
-  pt.sinc.clear();
-  pt.rinc_ptr.assign(pt.P + 1, 0);
-  pt.rinc.clear();
-  pt.hslots.clear();
-  if (int rc = element_incidences(pt.P, T, tet_idx, tet_mat, 0, nullptr, pt, err)) return rc;
-  return element_slots(tet_idx, nullptr, pt, err);
+int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, const Pattern& rows, Pattern& out,
+                          std::string& err) {
+  out.P = rows.P;
+  out.sinc_ptr.assign(out.P + 1, 0);
+  out.sinc.clear();
+  out.rinc_ptr.assign(out.P + 1, 0);
+  out.rinc.clear();
+  out.hslots.clear();
+  if (int rc = element_incidences(out.P, T, tet_idx, tet_mat, 0, nullptr, out, err)) return rc;
+  return element_slots(tet_idx, nullptr, rows, out, err);
+}
+
+void swap_incidences(Pattern& a, Pattern& b) {
+  a.tinc_ptr.swap(b.tinc_ptr);
+  a.tinc.swap(b.tinc);
+  a.tslots.swap(b.tslots);
+  a.hinc_ptr.swap(b.hinc_ptr);
+  a.hinc.swap(b.hinc);
+  a.hslots.swap(b.hslots);
+  a.slot_ptr.swap(b.slot_ptr);
+  a.contrib.swap(b.contrib);
+  a.sinc_ptr.swap(b.sinc_ptr);
+  a.sinc.swap(b.sinc);
+  a.rinc_ptr.swap(b.rinc_ptr);
+  a.rinc.swap(b.rinc);
 }
 
 // Balanced contiguous row ranges: `nchunks` pieces of [r0, r1) with roughly

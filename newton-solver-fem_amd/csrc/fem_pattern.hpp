@@ -39,10 +39,15 @@ int build_pattern(int P, int T, const int32_t* tet_idx, const float* tet_mat, in
 int build_reserved_pattern(int P, int count, int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt,
                            std::string& err);
 
-// Replaces the Tet4 incidences, slots and contributions of pt (tets only) for
-// new elements, keeping its rows; returns 2 when a valid tet has a block
-// outside them.
-int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, Pattern& pt, std::string& err);
+// Tet4 incidences, slots and contributions (tets only) of new elements
+// against the rows of `rows`, written into the incidence fields of `out` (which
+// may be `rows` itself; its storage is reused); returns 2 when a valid tet has a
+// block outside the rows.
+int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, const Pattern& rows, Pattern& out,
+                          std::string& err);
+
+// Exchanges the incidence, slot and contribution fields of two patterns.
+void swap_incidences(Pattern& a, Pattern& b);
 
 void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds);
 
