@@ -1728,6 +1728,18 @@ __global__ void k_restore(const int* __restrict__ fixed, int nfixed, const float
   }
 }
 
+// Compact values: block k of the compact pattern is reserved block map[k];
+// the live count is read from the device (the compact offsets' last entry), so
+// one launch shape serves every topology.
+__global__ __launch_bounds__(BLOCK) void k_gather_blocks(const int* __restrict__ map, const int* __restrict__ count,
+                                                         const float* __restrict__ src, float* __restrict__ dst) {
+  const size_t n = 9 * (size_t)*count;
+  for (size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * BLOCK) {
+    const size_t k = t / 9;
+    dst[t] = src[9 * (size_t)map[k] + (t - 9 * k)];
+  }
+}
+
 __global__ void k_copy_floats(const float* __restrict__ src, float* __restrict__ dst, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = src[t];
@@ -1912,6 +1924,8 @@ struct fem_system {
   int *d_res_thr = nullptr, *d_res_row = nullptr;
   uint64_t* d_res_mask = nullptr;
   int nchunks = 0, cg_k = 1, cg_nt = 256;
+  std::vector<int32_t> h_chunk_rows;  // [nchunks + 1] rows of the current CG plan
+  size_t cap_tile_cols = 0, cap_lidx = 0;  // storage of the current CG plan
   bool cg_res = false;
   int cg_variant_request = 0;  // 0 any, 1 streamed, 2 resident
   std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups
@@ -1940,6 +1954,42 @@ struct fem_system {
   std::vector<float> old_poses, old_mats;
   std::vector<unsigned char> old_active;
   float* d_lin_q = nullptr;
+  // Compact Krylov pattern of a reserved pool (streaming.py:209-384): the live
+  // slots -- diagonals and slots with element contributions -- of the reserved
+  // rows, their reserved slot numbers, and their values, gathered after every
+  // assembly.  The CG runs over it; A itself stays in the reserved pattern.
+  bool compact = false;
+  std::vector<int64_t> c_offsets;
+  std::vector<int32_t> c_columns, c_map;
+  int *d_coffsets = nullptr, *d_ccols = nullptr, *d_cmap = nullptr;
+  float* d_cvals = nullptr;
+  const std::vector<int64_t>& kp_offsets() const { return compact ? c_offsets : pt.offsets; }
+  const std::vector<int32_t>& kp_columns() const { return compact ? c_columns : pt.columns; }
+  // Live slots of the reserved rows: the diagonal and every slot with an
+  // element contribution (_compact_rows streaming.py:209-231).
+  void build_compact() {
+    c_offsets.assign(P + 1, 0);
+    c_columns.clear();
+    c_map.clear();
+    for (int i = 0; i < P; ++i) {
+      for (int64_t k = pt.offsets[i]; k < pt.offsets[i + 1]; ++k)
+        if (pt.slot_ptr[k + 1] > pt.slot_ptr[k] || pt.columns[k] == i) {
+          c_columns.push_back(pt.columns[k]);
+          c_map.push_back((int32_t)k);
+        }
+      c_offsets[i + 1] = (int64_t)c_columns.size();
+    }
+  }
+  int upload_compact() {
+    std::vector<int32_t> off32(P + 1);
+    for (int i = 0; i <= P; ++i) off32[i] = (int32_t)c_offsets[i];
+    HIP_TRY(hipMemcpy(d_coffsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+    if (!c_columns.empty()) {
+      HIP_TRY(hipMemcpy(d_ccols, c_columns.data(), c_columns.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_cmap, c_map.data(), c_map.size() * 4, hipMemcpyHostToDevice));
+    }
+    return FEM_OK;
+  }
   // Component scaling (component_scaling.py), caller-owned storage.
   float *cs_scale = nullptr, *cs_vals = nullptr, *cs_minv = nullptr, *cs_rhs = nullptr;
   double* cs_total = nullptr;
@@ -2215,7 +2265,15 @@ struct fem_system {
     return rc2;
   }
 
-  int plan_cg_nt(const int nt, const bool res) {
+  // keep_rows: re-plan the resident kernel for the current Krylov pattern over
+  // the previous plan's chunk rows, written into the previous plan's storage
+  // (same launch, so a recorded graph stays valid); fails if a chunk no longer
+  // fits on chip or the storage is too small.
+  int plan_cg_nt(const int nt, const bool res, const bool keep_rows = false) {
+    const std::vector<int64_t>& koff = kp_offsets();
+    const std::vector<int32_t>& kcol = kp_columns();
+    if (keep_rows && (!res || !cg_res || nt != cg_nt || h_chunk_rows.empty()))
+      return fail(FEM_ERR_STATE, "a kept-row CG plan needs a previous resident plan");
     int dev = 0, ncu = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2229,7 +2287,7 @@ struct fem_system {
       long rows = inst_off[k + 1] - inst_off[k];
       per[k] = (int)std::max(1L, (rows + max_rows_chunk - 1) / max_rows_chunk);
       need += per[k];
-      cost[k] = (double)(pt.offsets[inst_off[k + 1]] - pt.offsets[inst_off[k]]) + 8.0 * rows;
+      cost[k] = (double)(koff[inst_off[k + 1]] - koff[inst_off[k]]) + 8.0 * rows;
       total += cost[k];
     }
     // Default chunk count: about 512 rows per workgroup for small meshes, at
@@ -2239,7 +2297,7 @@ struct fem_system {
     const long res_blocks = (long)RES_NT * (RES_JR + RES_JL + RES_JS);  // blocks one resident chunk holds
     long natural = (nt == 512 || res) ? std::min<long>(P / 512, ncu) : (long)P / 512;
     if (res)
-      natural = std::min<long>(ncu, std::max<long>(natural, (long)std::ceil((double)pt.offsets[P] / (0.95 * res_blocks))));
+      natural = std::min<long>(ncu, std::max<long>(natural, (long)std::ceil((double)koff[P] / (0.95 * res_blocks))));
     long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, natural);
     target = std::max(target, need);
     if (target > need && total > 0) {
@@ -2258,7 +2316,7 @@ struct fem_system {
     std::vector<int32_t> chunk_rows, chunk_inst, inst_chunks;
     std::vector<int32_t> stamp(res ? P : 0, -1);
     int maxrows = 0;
-    for (int attempt = 0;; ++attempt) {
+    for (int attempt = 0; !keep_rows; ++attempt) {
       long n = count();
       if (cg_wg_request == 0 && n > 8 && (n & 7)) {  // automatic count: pad to a multiple of 8 (XCD swizzle)
         int big = (int)(std::max_element(cost.begin(), cost.end()) - cost.begin());
@@ -2274,7 +2332,7 @@ struct fem_system {
       std::vector<int> over;  // instances with a chunk the kernel cannot hold
       for (int k = 0; k < ninst; ++k) {
         std::vector<int32_t> bnd;
-        split_rows(pt.offsets, inst_off[k], inst_off[k + 1], per[k], bnd);
+        split_rows(koff, inst_off[k], inst_off[k + 1], per[k], bnd);
         bool bad = false;
         for (int j = 0; j < per[k]; ++j) {
           chunk_rows.push_back(bnd[j]);
@@ -2284,12 +2342,12 @@ struct fem_system {
             worst = k;
           }
           if (res && !bad) {  // on-chip capacity: blocks and distinct columns of the chunk
-            const int64_t kb = pt.offsets[bnd[j]], ke = pt.offsets[bnd[j + 1]];
+            const int64_t kb = koff[bnd[j]], ke = koff[bnd[j + 1]];
             int ncols = 0;
             const int id = (int)chunk_rows.size();
             for (int64_t q = kb; q < ke; ++q)
-              if (stamp[pt.columns[q]] != id) {
-                stamp[pt.columns[q]] = id;
+              if (stamp[kcol[q]] != id) {
+                stamp[kcol[q]] = id;
                 ++ncols;
               }
             bad = ke - kb > res_blocks || ncols > RES_ZERO || bnd[j + 1] - bnd[j] > max_rows_chunk;
@@ -2303,6 +2361,25 @@ struct fem_system {
       if (over.empty()) over.push_back(worst);
       for (int k : over) per[k] += res ? std::max(1, per[k] / 64) : 1;
       std::fill(stamp.begin(), stamp.end(), -1);
+    }
+    if (keep_rows) {
+      chunk_rows.assign(h_chunk_rows.begin(), h_chunk_rows.end() - 1);
+      chunk_inst.assign(chunk_rows.size(), 0);
+      inst_chunks = {0, (int32_t)chunk_rows.size()};
+      maxrows = 0;
+      for (size_t c = 0; c < chunk_rows.size(); ++c) {
+        const int r0 = chunk_rows[c], r1 = h_chunk_rows[c + 1];
+        maxrows = std::max(maxrows, r1 - r0);
+        const int64_t kb = koff[r0], ke = koff[r1];
+        int ncols = 0;
+        for (int64_t q = kb; q < ke; ++q)
+          if (stamp[kcol[q]] != (int)c) {
+            stamp[kcol[q]] = (int)c;
+            ++ncols;
+          }
+        if (ke - kb > res_blocks || ncols > RES_ZERO)
+          return fail(FEM_ERR_UNSUPPORTED, "the edited topology no longer fits the resident CG launch");
+      }
     }
     chunk_rows.push_back(P);
     const int n_chunks = (int)chunk_inst.size();
@@ -2319,7 +2396,7 @@ struct fem_system {
       while (r < r1) {
         tile_rows.push_back(r);
         int e = r;
-        while (e < r1 && pt.offsets[e + 1] - pt.offsets[r] <= (int64_t)CG_J * nt) ++e;
+        while (e < r1 && koff[e + 1] - koff[r] <= (int64_t)CG_J * nt) ++e;
         if (e == r) return fail(FEM_ERR_UNSUPPORTED, "row longer than a CG tile");
         r = e;
       }
@@ -2329,18 +2406,31 @@ struct fem_system {
     // every block (the SpMV forms each distinct operand once per tile).
     const int ntiles = (int)tile_rows.size();
     std::vector<int32_t> tile_cptr(1, 0), tile_cols;
-    std::vector<uint16_t> lidx(pt.offsets[P]);
+    std::vector<uint16_t> lidx(koff[P]);
     {
-      std::vector<int32_t> uniq;
+      // Per tile: mark its columns in a bitmap, read the marks back in
+      // ascending order (the tile's distinct columns and their ranks), clear.
+      std::vector<uint64_t> mark(((size_t)P + 63) / 64, 0);
+      std::vector<int32_t> rank(P);
       for (int t = 0; t < ntiles; ++t) {
         const int ra = tile_rows[t], rb = t + 1 < ntiles ? tile_rows[t + 1] : P;
-        const int64_t ka = pt.offsets[ra], kb = pt.offsets[rb];
-        uniq.assign(pt.columns.begin() + ka, pt.columns.begin() + kb);
-        std::sort(uniq.begin(), uniq.end());
-        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-        for (int64_t k = ka; k < kb; ++k)
-          lidx[k] = (uint16_t)(std::lower_bound(uniq.begin(), uniq.end(), pt.columns[k]) - uniq.begin());
-        tile_cols.insert(tile_cols.end(), uniq.begin(), uniq.end());
+        const int64_t ka = koff[ra], kb = koff[rb];
+        size_t lo = mark.size(), hi = 0;
+        for (int64_t k = ka; k < kb; ++k) {
+          const size_t w = (size_t)kcol[k] >> 6;
+          mark[w] |= 1ull << (kcol[k] & 63);
+          lo = std::min(lo, w);
+          hi = std::max(hi, w + 1);
+        }
+        const size_t base = tile_cols.size();
+        for (size_t w = lo; w < hi; ++w)
+          for (uint64_t bits = mark[w]; bits; bits &= bits - 1) {
+            const int32_t c = (int32_t)(64 * w + (size_t)__builtin_ctzll(bits));
+            rank[c] = (int32_t)(tile_cols.size() - base);
+            tile_cols.push_back(c);
+          }
+        for (size_t w = lo; w < hi; ++w) mark[w] = 0;
+        for (int64_t k = ka; k < kb; ++k) lidx[k] = (uint16_t)rank[kcol[k]];
         tile_cptr.push_back((int32_t)tile_cols.size());
       }
     }
@@ -2355,7 +2445,7 @@ struct fem_system {
       res_row.assign((size_t)2 * P, 0);
       for (int c = 0; c < n_chunks; ++c) {
         const int r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
-        const int64_t kb = pt.offsets[r0], nb = pt.offsets[r1] - kb;
+        const int64_t kb = koff[r0], nb = koff[r1] - kb;
         const int ncols = tile_cptr[c + 1] - tile_cptr[c];
         if (ncols > RES_ZERO) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many distinct columns in a chunk");
         int run = 0, row = r0;
@@ -2371,12 +2461,12 @@ struct fem_system {
                             (tile_cols.begin() + tile_cptr[c]));
           uint64_t mask = 0;
           for (int64_t b = b0; b < b1; ++b) {
-            while (pt.offsets[row + 1] <= kb + b) ++row;
-            if (b == b0 || pt.offsets[row] == kb + b) {  // a run of `row` starts here
+            while (koff[row + 1] <= kb + b) ++row;
+            if (b == b0 || koff[row] == kb + b) {  // a run of `row` starts here
               if (res_row[2 * row + 1] == 0) res_row[2 * row] = run;
               ++res_row[2 * row + 1];
             }
-            if (pt.offsets[row + 1] == kb + b + 1 || b + 1 == b1) {
+            if (koff[row + 1] == kb + b + 1 || b + 1 == b1) {
               mask |= 1ull << (b - b0);
               ++run;
             }
@@ -2385,6 +2475,18 @@ struct fem_system {
         }
         if (run > RES_SEG) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many row runs in a chunk");
       }
+    }
+    if (keep_rows) {
+      if (n_chunks != nchunks || tile_cols.size() > cap_tile_cols || lidx.size() > cap_lidx)
+        return fail(FEM_ERR_UNSUPPORTED, "the edited topology no longer fits the reserved CG plan storage");
+      HIP_TRY(hipMemcpy(d_res_thr, res_thr.data(), res_thr.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_res_mask, res_mask.data(), res_mask.size() * 8, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_res_row, res_row.data(), res_row.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_tile_cptr, tile_cptr.data(), tile_cptr.size() * 4, hipMemcpyHostToDevice));
+      if (!tile_cols.empty())
+        HIP_TRY(hipMemcpy(d_tile_cols, tile_cols.data(), tile_cols.size() * 4, hipMemcpyHostToDevice));
+      if (!lidx.empty()) HIP_TRY(hipMemcpy(d_lidx, lidx.data(), lidx.size() * 2, hipMemcpyHostToDevice));
+      return FEM_OK;
     }
     const int k_rows = res ? RES_K : maxrows <= nt ? 1 : maxrows <= 2 * nt ? 2 : maxrows <= 4 * nt ? 4 : 8;
     const void* kp = cg_kernel_ptr(k_rows, nt, res);
@@ -2453,6 +2555,9 @@ struct fem_system {
     d_part = n_part;
     d_cgflags = n_flags;
     nchunks = n_chunks;
+    h_chunk_rows = chunk_rows;
+    cap_tile_cols = tile_cols.size();
+    cap_lidx = lidx.size();
     cg_groups = groups;
     cg_k = k_rows;
     cg_nt = nt;
@@ -2643,6 +2748,7 @@ int fem_set_params(fem_system* s, const fem_params* p) {
 
 int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (s->topo_count) return fail(FEM_ERR_STATE, "launch shapes are fixed once tetrahedral updates are prepared");
   auto ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 32 || g == 64; };
   if (elem_lanes && !ok(elem_lanes)) return fail(FEM_ERR_ARG, "elem_lanes must be a power of two in [4, 64]");
   if (assemble_rows && !ok(assemble_rows)) return fail(FEM_ERR_ARG, "assemble_rows must be a power of two in [4, 64]");
@@ -2679,6 +2785,7 @@ int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* assemble_r
 
 int fem_set_cg_variant(fem_system* s, int32_t variant) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (s->topo_count) return fail(FEM_ERR_STATE, "launch shapes are fixed once tetrahedral updates are prepared");
   if (variant < FEM_CG_AUTO || variant > FEM_CG_RESIDENT) return fail(FEM_ERR_ARG, "unknown CG variant");
   const int old = s->cg_variant_request;
   s->cg_variant_request = variant;
@@ -2715,6 +2822,9 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   // by the next topology update.
   hipLaunchKernelGGL(k_copy_floats, dim3((3 * s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, q,
                      s->d_lin_q, 3 * s->P);
+  if (s->compact)
+    hipLaunchKernelGGL(k_gather_blocks, dim3(1024), dim3(BLOCK), 0, (hipStream_t)stream, (const int*)s->d_cmap,
+                       (const int*)(s->d_coffsets + s->P), (const float*)s->b_vals, s->d_cvals);
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }
@@ -2755,9 +2865,9 @@ CgArgs cg_args(const fem_system* s) {
   CgArgs ca;
   std::memset(&ca, 0, sizeof ca);
   ca.P = s->P;
-  ca.offsets = s->d_offsets;
-  ca.cols = s->d_cols;
-  ca.vals = s->b_vals;
+  ca.offsets = s->compact ? s->d_coffsets : s->d_offsets;
+  ca.cols = s->compact ? s->d_ccols : s->d_cols;
+  ca.vals = s->compact ? s->d_cvals : s->b_vals;
   ca.minv = s->b_minv;
   ca.row_fixed = s->d_row_fixed;
   ca.dv = s->b_dv;
@@ -2823,8 +2933,11 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
                        (float*)nullptr, (float*)nullptr, s->d_rhs_max, 1);
     hipLaunchKernelGGL(k_absmax, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const float*)s->cs_rhs, s->P,
                        (const int*)nullptr, s->d_rhs_max);
+    if (s->compact)
+      hipLaunchKernelGGL(k_gather_blocks, dim3(1024), dim3(BLOCK), 0, st, (const int*)s->d_cmap,
+                         (const int*)(s->d_coffsets + s->P), (const float*)s->cs_vals, s->d_cvals);
     HIP_TRY(hipGetLastError());
-    ca.vals = s->cs_vals;
+    ca.vals = s->compact ? s->d_cvals : s->cs_vals;
     ca.minv = s->cs_minv;
     ca.rhs = s->cs_rhs;
   }
@@ -3039,6 +3152,23 @@ int fem_prepare_topology(fem_system* s, int32_t original_node_count) {
   for (int e = 0; e < T; ++e)
     s->old_active[e] = !(mats[3 * e] == 0.f && mats[3 * e + 1] == 0.f && mats[3 * e + 2] == 0.f);
   if ((rc = s->reserve_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+  // Krylov products over the live slots when the resident kernel holds the
+  // reserved rows (the compact plan then reuses the reserved plan's rows and
+  // storage); otherwise over the reserved pattern, whose dead slots are zero.
+  if (s->cg_res) {
+    const size_t cap = s->pt.columns.size();
+    if ((rc = s->alloc(&s->d_coffsets, (size_t)P + 1)) || (rc = s->alloc(&s->d_ccols, cap)) ||
+        (rc = s->alloc(&s->d_cmap, cap)) || (rc = s->alloc(&s->d_cvals, 9 * cap)))
+      return rc;
+    s->build_compact();
+    s->compact = true;
+    if (s->plan_cg_nt(RES_NT, true, true) != FEM_OK) {
+      s->compact = false;
+      if ((rc = s->plan_cg())) return rc;
+    } else if ((rc = s->upload_compact())) {
+      return rc;
+    }
+  }
   s->topo_count = original_node_count;
   // The pattern changed: the matrix storage must be bound again.
   s->b_offsets = s->b_cols = nullptr;
@@ -3211,11 +3341,25 @@ int fem_update_topology(fem_system* s, const int32_t* tet_indices, const float* 
     if (int rc = assign_tet_incidences(T, nodes.data(), mats.data(), s->pt, s->spare, err))
       return fail(FEM_ERR_ARG, err);
     swap_incidences(s->pt, s->spare);
-    // Assembly groups of the new incidences within the reserved launch.
-    if (int rc = s->plan_assemble(s->asm_rb)) {
+    // Assembly groups and, over live slots, the CG plan of the new
+    // incidences within the reserved launches; on failure the previous ones
+    // are planned again (they fitted before).
+    int rc = s->plan_assemble(s->asm_rb);
+    if (rc == FEM_OK && s->compact) {
+      s->build_compact();
+      rc = s->plan_cg_nt(RES_NT, true, true);
+    }
+    if (rc != FEM_OK) {
+      const std::string why = g_error;
       swap_incidences(s->pt, s->spare);
+      if (s->plan_assemble(s->asm_rb) == FEM_OK && s->compact) {
+        s->build_compact();
+        (void)s->plan_cg_nt(RES_NT, true, true);
+      }
+      g_error = why;
       return rc;
     }
+    if (s->compact && (rc = s->upload_compact())) return rc;
     // Commit: incidences into their reserved storage, element and particle data.
     const Pattern& pt = s->pt;
     std::vector<int32_t> tptr(P + 1);
