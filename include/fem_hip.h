@@ -310,7 +310,10 @@ int fem_prepare_topology(fem_system* sys, int32_t original_node_count);
  * each slot's previous element or -1.  stats (8 int32, may be NULL) receives
  * the reference's update counters {dirty, removed, reused, active, touched
  * rows, topology-inserted, topology-removed, full numeric refresh}.  The pool's
- * pattern, matrix storage and launch shapes do not change.  Without a pool the
+ * pattern, matrix storage and launch shapes do not change (fem_set_launch and
+ * fem_set_cg_variant return FEM_ERR_STATE once a pool is prepared); the CG runs
+ * over the live slots -- diagonals and slots with element contributions --
+ * when the resident kernel holds the reserved rows.  Without a pool the
  * exact pattern of the edited model is built again (within the bound block
  * capacity; offsets and columns are rewritten and fem_pattern_size reports the
  * new nnz); device arrays the previous pattern used stay allocated until
