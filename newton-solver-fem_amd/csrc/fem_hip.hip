@@ -352,8 +352,18 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
     rdiag[threadIdx.x] = a.diag_rel[i];
     rmass[threadIdx.x] = a.mass[i];
   }
-  for (int k = threadIdx.x; k <= ke - kb; k += BLOCK) sptr[k] = a.slot_ptr[kb + k] - cb;
-  for (int j = threadIdx.x; j < nc; j += BLOCK) codes[j] = a.contrib[cb + j];
+  // Phase 2's slot pointers and contribution codes: the first ASM_PF per
+  // thread are loaded into registers now, in flight with phase 1's element
+  // loads, and stored to LDS after phase 1; the rest (large groups) after.
+  constexpr int ASM_PF = 4;
+  int pf_ptr = 0;
+  uint16_t pf_code[ASM_PF];
+  if ((int)threadIdx.x <= ke - kb) pf_ptr = a.slot_ptr[kb + threadIdx.x];
+#pragma unroll
+  for (int u = 0; u < ASM_PF; ++u) {
+    const int j = threadIdx.x + u * BLOCK;
+    pf_code[u] = j < nc ? a.contrib[cb + j] : (uint16_t)0;
+  }
   for (int t = threadIdx.x; t < nt; t += BLOCK) {
     const int code = a.tinc[tb + t];
     const int e = code >> 2, la = code & 3;
@@ -378,6 +388,14 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
 #pragma unroll
       for (int c = 0; c < 9; ++c) o[9 * b + c] = K[b][c];
   }
+  if ((int)threadIdx.x <= ke - kb) sptr[threadIdx.x] = pf_ptr - cb;
+  for (int k = threadIdx.x + BLOCK; k <= ke - kb; k += BLOCK) sptr[k] = a.slot_ptr[kb + k] - cb;
+#pragma unroll
+  for (int u = 0; u < ASM_PF; ++u) {
+    const int j = threadIdx.x + u * BLOCK;
+    if (j < nc) codes[j] = pf_code[u];
+  }
+  for (int j = threadIdx.x + ASM_PF * BLOCK; j < nc; j += BLOCK) codes[j] = a.contrib[cb + j];
   __syncthreads();
   for (int kk = threadIdx.x; kk < ke - kb; kk += BLOCK) {
     int lr = 0;
