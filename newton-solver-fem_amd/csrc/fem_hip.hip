@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1986,17 +1987,28 @@ struct fem_system {
   // Live slots of the reserved rows: the diagonal and every slot with an
   // element contribution (_compact_rows streaming.py:209-231).
   void build_compact() {
+    auto live = [&](int i, int64_t k) { return pt.slot_ptr[k + 1] > pt.slot_ptr[k] || pt.columns[k] == i; };
     c_offsets.assign(P + 1, 0);
-    c_columns.clear();
-    c_map.clear();
-    for (int i = 0; i < P; ++i) {
-      for (int64_t k = pt.offsets[i]; k < pt.offsets[i + 1]; ++k)
-        if (pt.slot_ptr[k + 1] > pt.slot_ptr[k] || pt.columns[k] == i) {
-          c_columns.push_back(pt.columns[k]);
-          c_map.push_back((int32_t)k);
-        }
-      c_offsets[i + 1] = (int64_t)c_columns.size();
-    }
+    parallel_pieces(P, 2048, [&](int r0, int r1) {  // live count per row
+      for (int i = r0; i < r1; ++i) {
+        int64_t n = 0;
+        for (int64_t k = pt.offsets[i]; k < pt.offsets[i + 1]; ++k) n += live(i, k);
+        c_offsets[i + 1] = n;
+      }
+    });
+    for (int i = 0; i < P; ++i) c_offsets[i + 1] += c_offsets[i];
+    c_columns.resize(c_offsets[P]);
+    c_map.resize(c_offsets[P]);
+    parallel_pieces(P, 2048, [&](int r0, int r1) {
+      for (int i = r0; i < r1; ++i) {
+        int64_t o = c_offsets[i];
+        for (int64_t k = pt.offsets[i]; k < pt.offsets[i + 1]; ++k)
+          if (live(i, k)) {
+            c_columns[o] = pt.columns[k];
+            c_map[o++] = (int32_t)k;
+          }
+      }
+    });
   }
   int upload_compact() {
     std::vector<int32_t> off32(P + 1);
@@ -2461,16 +2473,25 @@ struct fem_system {
       res_thr.assign((size_t)n_chunks * nt * 4, 0);
       res_mask.assign((size_t)n_chunks * nt, 0);
       res_row.assign((size_t)2 * P, 0);
-      for (int c = 0; c < n_chunks; ++c) {
+      // Chunks are independent (each writes its own threads' entries and its
+      // own rows'), so they are mapped on host threads.
+      std::atomic<int> why{0};
+      parallel_pieces(n_chunks, 8, [&](int c0, int c1) {
+      for (int c = c0; c < c1; ++c) {
         const int r0 = chunk_rows[c], r1 = chunk_rows[c + 1];
         const int64_t kb = koff[r0], nb = koff[r1] - kb;
         const int ncols = tile_cptr[c + 1] - tile_cptr[c];
-        if (ncols > RES_ZERO) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many distinct columns in a chunk");
+        if (ncols > RES_ZERO) {
+          why.store(1);
+          return;
+        }
         int run = 0, row = r0;
         for (int t = 0; t < nt; ++t) {
           const int64_t b0 = nb * t / nt, b1 = nb * (t + 1) / nt;
-          if (b1 - b0 > RES_JR + RES_JL + RES_JS)
-            return fail(FEM_ERR_UNSUPPORTED, "resident CG: chunk exceeds the on-chip capacity");
+          if (b1 - b0 > RES_JR + RES_JL + RES_JS) {
+            why.store(2);
+            return;
+          }
           int32_t* th = &res_thr[((size_t)c * nt + t) * 4];
           th[0] = (int32_t)b0;
           th[1] = (int32_t)(b1 - b0);
@@ -2491,8 +2512,16 @@ struct fem_system {
           }
           res_mask[(size_t)c * nt + t] = mask;
         }
-        if (run > RES_SEG) return fail(FEM_ERR_UNSUPPORTED, "resident CG: too many row runs in a chunk");
+        if (run > RES_SEG) {
+          why.store(3);
+          return;
+        }
       }
+      });
+      static const char* msg[4] = {"", "resident CG: too many distinct columns in a chunk",
+                                   "resident CG: chunk exceeds the on-chip capacity",
+                                   "resident CG: too many row runs in a chunk"};
+      if (why.load()) return fail(FEM_ERR_UNSUPPORTED, msg[why.load()]);
     }
     if (keep_rows) {
       if (n_chunks != nchunks || tile_cols.size() > cap_tile_cols || lidx.size() > cap_lidx)

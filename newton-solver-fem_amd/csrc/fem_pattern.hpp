@@ -1,8 +1,10 @@
 // Host-side BSR pattern and element incidence construction.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace fem {
@@ -48,6 +50,23 @@ int assign_tet_incidences(int T, const int32_t* tet_idx, const float* tet_mat, c
 
 // Exchanges the incidence, slot and contribution fields of two patterns.
 void swap_incidences(Pattern& a, Pattern& b);
+
+// Runs fn(begin, end) over contiguous pieces of [0, n) of about min_per items
+// or more, on up to 16 host threads (the calling thread takes the first piece).
+template <typename F>
+void parallel_pieces(int n, int min_per, F fn) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int parts = std::max(1, std::min({16, hw, n / std::max(1, min_per)}));
+  if (parts <= 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> threads;
+  for (int j = 1; j < parts; ++j)
+    threads.emplace_back(fn, (int)((int64_t)n * j / parts), (int)((int64_t)n * (j + 1) / parts));
+  fn(0, (int)((int64_t)n / parts));
+  for (auto& t : threads) t.join();
+}
 
 void split_rows(const std::vector<int64_t>& offsets, int r0, int r1, int nchunks, std::vector<int32_t>& bounds);
 
