@@ -225,3 +225,50 @@ def test_prepared_actions_replay_in_a_hip_graph():
             assert torch.equal(a, b), k
         solver.step(current, model.state(), None, None, DT)  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_interrupted_preparation_fails_closed_without_mutating_input():
+    """Reference tests/test_fem_prepared_step.py:200-211: a preparation that
+    fails part-way leaves the workspace unusable (not stale) and the caller's
+    state untouched; preparing again recovers."""
+    print("case prepared interrupted", flush=True)
+    from unittest import mock
+
+    _, current, make = fixture(mixed=False)
+    solver = make()
+    solver.refresh_tangent(current.particle_q)
+    prepared = solver.prepare_step(current, None, None, DT)
+    before = current.particle_q.clone()
+    with mock.patch.object(type(prepared), "_call", side_effect=RuntimeError("injected failure")):
+        with pytest.raises(RuntimeError, match="injected failure"):
+            solver.prepare_step(current, None, None, DT)
+    with pytest.raises(RuntimeError, match="Prepare FEM mechanics"):
+        prepared.rhs()
+    assert torch.equal(current.particle_q, before)
+    solver.prepare_step(current, None, None, DT)
+    assert torch.isfinite(prepared.rhs()).all()
+
+
+def test_unsupported_modes_and_invalid_alias_fail_before_state_change():
+    """Reference tests/test_fem_prepared_step.py:214-231 (the tape case has no
+    counterpart here: differentiable solvers are rejected instead)."""
+    print("case prepared unsupported", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    model, current, make = fixture(mixed=False, fixed=False)
+    diff = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0,
+                     differentiable=True)
+    with pytest.raises(RuntimeError, match="non-differentiable"):
+        diff.prepare_step(current, None, None, DT)
+    solver = make()
+    solver.refresh_tangent(current.particle_q)
+    prepared = solver.prepare_step(current, None, None, DT)
+    vector = torch.zeros_like(prepared.base_rhs)
+    with pytest.raises(ValueError, match="alias"):
+        prepared.rhs(vector, vector)
+    with pytest.raises(ValueError, match="distinct"):
+        prepared.apply_operator(vector, vector)
+    solver = make()
+    solver.prepare_tetrahedral_updates(model.particle_count)
+    with pytest.raises(ValueError, match="assembled"):
+        solver.prepare_step(current, None, None, DT)
