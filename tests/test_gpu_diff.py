@@ -274,3 +274,64 @@ def test_step_gradients_with_springs_match_oracle():
     assert rel(np_(tq.grad) - gq_out, ref["q"] - gq_out) < 2e-3
     assert rel(np_(tv.grad) - u, ref["qd"] - u) < 2e-3
     assert rel(np_(te.grad), ref["ext"]) < 1e-3
+
+
+def _two_step_loss(model, s, q0, v0, act):
+    """Two recorded steps from (q0, v0) with activations `act`; returns the
+    loss and the leaf tensors."""
+    q = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    v = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    a = torch.tensor(act, dtype=torch.float32, device=DEV, requires_grad=True)
+    ctrl = model.control()
+    ctrl.tet_activations = a
+    s0, s1, s2 = model.state(), model.state(), model.state()
+    s0.particle_q, s0.particle_qd = q, v
+    s.step(s0, s1, ctrl, None, DT)
+    s.step(s1, s2, ctrl, None, DT)
+    loss = (s2.particle_q * s2.particle_q).sum() + 0.1 * (s2.particle_qd * s2.particle_qd).sum()
+    return loss, (q, v, a)
+
+
+def test_recorded_backward_survives_unrelated_forward_and_rejects_revision_change():
+    """Reference tests/test_activation_gradients.py:147-166 and :227-233: a
+    recorded rollout owns every primal its backward needs (unrelated forward
+    steps in between leave its gradients unchanged), and reassembling the
+    matrix after recording invalidates the backward."""
+    print("case diff snapshots", flush=True)
+    model, s, _ = make("tet")
+    _, q0, v0 = initial(model, 5)
+    act = np.full(model.tet_count, 0.03, np.float32)
+    s.refresh_tangent(torch.tensor(q0, dtype=torch.float32, device=DEV))
+    loss, leaves = _two_step_loss(model, s, q0, v0, act)
+    with torch.no_grad():  # unrelated steps overwrite every persistent solver buffer
+        a, b = model.state(), model.state()
+        a.particle_q.copy_(torch.tensor(q0 + (0.01, 0.02, 0.0), dtype=torch.float32))
+        a.particle_qd.copy_(torch.tensor(-v0, dtype=torch.float32))
+        ctrl = model.control()
+        ctrl.tet_activations = torch.full_like(ctrl.tet_activations, -0.03)
+        for _ in range(3):
+            s.step(a, b, ctrl, None, DT)
+            a, b = b, a
+    loss.backward()
+    first = [t.grad.clone() for t in leaves]
+    loss2, leaves2 = _two_step_loss(model, s, q0, v0, act)
+    loss2.backward()
+    for g1, t2 in zip(first, leaves2):
+        np.testing.assert_allclose(np_(g1), np_(t2.grad), rtol=2e-6, atol=5e-6)
+    loss3, _ = _two_step_loss(model, s, q0, v0, act)
+    s.refresh_tangent(torch.tensor(q0, dtype=torch.float32, device=DEV))
+    with pytest.raises(RuntimeError, match="changed"):
+        loss3.backward()
+
+
+def test_taped_step_rejects_aliased_states():
+    """Reference tests/test_activation_gradients.py:211-225: a recorded step
+    needs distinct input and output states."""
+    print("case diff alias", flush=True)
+    model, s, _ = make("tet")
+    _, q0, v0 = initial(model, 6)
+    cur = model.state()
+    cur.particle_q = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    cur.particle_qd = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    with pytest.raises(ValueError, match="unique"):
+        s.step(cur, cur, None, None, DT)
