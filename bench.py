@@ -51,31 +51,45 @@ def build_large(device):
 
 
 def cpu_baseline(scene, model, substeps: int):
-    """FP32 CPU restatement of the reference substep (single thread)."""
+    """FP32 CPU restatement of the reference substep, timed on one host thread
+    and on all host threads (up to 16; the CG loops run in parallel, the element
+    loops serially); the faster of the two is the reported baseline."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
 
     import oracle
 
-    # tolerance 1e-12: every substep runs the full 50-iteration budget, as the timed GPU substep does
-    o = oracle.OracleSolver(model.host, bits=32, dt=DT, gravity=model.gravity, fixed=scene.fixed, tolerance=1e-12,
-                            maxiter=CG_ITERS)
-    q = model.host["particle_q"].astype(np.float64)
-    v = np.zeros_like(q)
-    q = o.refresh_tangent(q)  # first refresh builds the slot map (setup, untimed)
-    t0 = time.perf_counter()
-    for _ in range(substeps):
-        q = o.refresh_tangent(q)
-        out = o.step(q, v)
-        q, v = out["q"], out["qd"]
-    el = time.perf_counter() - t0
+    def run(threads):
+        # tolerance 1e-12: every substep runs the full 50-iteration budget, as the timed GPU substep does
+        o = oracle.OracleSolver(model.host, bits=32, dt=DT, gravity=model.gravity, fixed=scene.fixed,
+                                tolerance=1e-12, maxiter=CG_ITERS)
+        o.set_threads(threads)
+        q = model.host["particle_q"].astype(np.float64)
+        v = np.zeros_like(q)
+        q = o.refresh_tangent(q)  # first refresh builds the slot map (setup, untimed)
+        t0 = time.perf_counter()
+        for _ in range(substeps):
+            q = o.refresh_tangent(q)
+            out = o.step(q, v)
+            q, v = out["q"], out["qd"]
+        el = time.perf_counter() - t0
+        return substeps / el, el
+
+    single, el1 = run(1)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    multi, elm = run(threads) if threads > 1 else (single, el1)
+    best_multi = multi > single
     return {
-        "value": substeps / el,
+        "value": max(single, multi),
         "unit": "substeps/s",
-        "cores": 1,
+        "cores": threads if best_multi else 1,
         "kind": "port",
+        "single_thread": single,
+        "multi_thread": multi,
+        "threads": threads,
         "sample": f"{substeps} substeps (tangent refresh + forces/predictor + 50 PCG iterations) of the same "
-                  f"1,008,000-tet workload, fp32 C++ restatement, one thread, {el:.1f} s",
+                  f"1,008,000-tet workload, fp32 C++ restatement: one thread {el1:.1f} s; {threads} threads "
+                  f"(CG loops in parallel, element loops serial) {elm:.1f} s",
     }
 
 

@@ -299,6 +299,7 @@ struct Solver {
   bool have_slots = false;
   std::vector<v3> dv;
   int last_iters = 0;
+  int nthreads = 1;  // host threads of the CG loops (1: serial, the order the tests pin)
   double last_rr = 0, last_atol_sq = 0, last_scale = 1;
 
   // ---- element kernels ---------------------------------------------------
@@ -372,7 +373,8 @@ struct Solver {
   }
 
   // build_system_matrix_tet_kernel, kernels.py:635-850 (block_offset = P).
-  void tet_blocks(const v3* x, Real dt) {
+  void tet_blocks(const v3* x, Real dt) {  // each element writes only its own 16 triplets
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int e = 0; e < T; ++e) {
       int n[4] = {tet_idx[4 * e], tet_idx[4 * e + 1], tet_idx[4 * e + 2], tet_idx[4 * e + 3]};
       Real k_mu = tet_mat[3 * e], k_lambda = tet_mat[3 * e + 1];
@@ -503,6 +505,8 @@ struct Solver {
 
   // build_system_matrix_hex_kernel, kernels.py:1609-1747 (block_offset = P+16T).
   void hex_blocks(const v3* x, Real dt) {
+    // each hex writes only its own 64 triplets
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int e = 0; e < H; ++e) {
       Real k_mu = hex_mat[3 * e], k_lambda = hex_mat[3 * e + 1];
       size_t base = tet_off() + 16 * (size_t)T + 64 * (size_t)e;
@@ -774,6 +778,7 @@ struct Solver {
   // invert_block_diagonal_kernel kernels.py:1836-1849.
   void refresh_preconditioner() {
     minv.assign(9 * (size_t)P, Real(0));
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int i = 0; i < P; ++i) {
       Real d[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
       for (int k = offsets[i]; k < offsets[i + 1]; ++k)
@@ -809,6 +814,7 @@ struct Solver {
 
   // ---- sparse operators --------------------------------------------------
   void bsr_mv(const v3* x, v3* y) const {
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int i = 0; i < P; ++i) {
       Real acc[3] = {0, 0, 0};
       for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
@@ -825,6 +831,7 @@ struct Solver {
       bsr_mv(x, y);
       return;
     }
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int i = 0; i < P; ++i) {
       if (fixed_mask[i]) {
         y[i] = x[i];
@@ -842,15 +849,18 @@ struct Solver {
     }
   }
   void precond(const v3* r, v3* z) const {
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
     for (int i = 0; i < P; ++i) {
       const Real* m = &minv[9 * (size_t)i];
       z[i] = {m[0] * r[i].x + m[1] * r[i].y + m[2] * r[i].z, m[3] * r[i].x + m[4] * r[i].y + m[5] * r[i].z,
               m[6] * r[i].x + m[7] * r[i].y + m[8] * r[i].z};
     }
   }
-  static double vdot(const std::vector<v3>& a, const std::vector<v3>& b) {
+  double vdot(const std::vector<v3>& a, const std::vector<v3>& b) const {
     double s = 0;
-    for (size_t i = 0; i < a.size(); ++i)
+    const long n = (long)a.size();
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static) reduction(+ : s)
+    for (long i = 0; i < n; ++i)
       s += (double)a[i].x * b[i].x + (double)a[i].y * b[i].y + (double)a[i].z * b[i].z;
     return s;
   }
@@ -876,6 +886,7 @@ struct Solver {
       op_mv(p.data(), Ap.data());
       double pAp = vdot(p, Ap);
       Real alpha = pAp != 0.0 ? (Real)(rz / pAp) : Real(0);
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
       for (int i = 0; i < P; ++i) {
         x[i] = x[i] + p[i] * alpha;
         r[i] = r[i] - Ap[i] * alpha;
@@ -885,6 +896,7 @@ struct Solver {
       double rz_new = vdot(r, z);
       Real beta = rr > atol_sq ? (Real)(rz_new / rz) : Real(0);
       rz = rz_new;
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
       for (int i = 0; i < P; ++i) p[i] = z[i] + p[i] * beta;
       iters = it + 1;
     }
@@ -902,6 +914,7 @@ struct Solver {
     forces_scaled(q_in, qd_in, ext, act, dt, f_out);
     // subtract_elastic_predictor implicit_predictor.py:9-23
     if (T || H || S) {
+#pragma omp parallel for num_threads(nthreads) if (nthreads > 1) schedule(static)
       for (int i = 0; i < P; ++i) {
         Real c[3] = {0, 0, 0};
         for (int k = offsets[i]; k < offsets[i + 1]; ++k) {
@@ -1607,6 +1620,13 @@ void orc_destroy(void* p) {
 }
 
 // params: min_s, max_s, has_ground, g0..g3, ke, kd, kf, mu, lin_damp, gx, gy, gz, enable_act, tol, maxiter
+// Host threads of the CG loops of later solves (1, the default, keeps the
+// serial summation order; more threads only for timing).
+void orc_set_threads(void* p, int n) {
+  Handle* h = (Handle*)p;
+  DISPATCH(h, S.nthreads = n < 1 ? 1 : n);
+}
+
 void orc_set_params(void* p, const double* a) {
   Handle* h = (Handle*)p;
   Params prm;

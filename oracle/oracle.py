@@ -42,6 +42,7 @@ def lib():
                                     d, d, d, ctypes.c_int, i]
         _lib.orc_destroy.argtypes = [ctypes.c_void_p]
         _lib.orc_set_params.argtypes = [ctypes.c_void_p, d]
+        _lib.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib.orc_construct.argtypes = [ctypes.c_void_p, ctypes.c_double]
         _lib.orc_refresh.argtypes = [ctypes.c_void_p, d, ctypes.c_double]
         _lib.orc_nnz.argtypes = [ctypes.c_void_p]
@@ -162,6 +163,10 @@ class OracleSolver:
         _, ptr = _d(arr)
         self._params_arr = arr
         lib().orc_set_params(self.h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def set_threads(self, n: int):
+        """Host threads of the CG loops (1: serial order; more only for timing)."""
+        lib().orc_set_threads(self.h, int(n))
 
     def set(self, **kw):
         self.params.update(kw)
