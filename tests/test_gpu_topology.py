@@ -551,3 +551,45 @@ def test_reassembly_uses_the_same_fixed_positions_as_force_evaluation(method):
     expected.step(reference, expected_out, None, None, 0.002)
     actual.step(changed, actual_out, None, None, 0.002)
     np.testing.assert_allclose(np_(actual_out.particle_qd), np_(expected_out.particle_qd), atol=1e-7)
+
+
+def test_scaled_pool_capture_replays_after_edit():
+    """A step recorded with a reserved pool and component scaling (scaled
+    copies gathered into the compact Krylov pattern) replays after an edit,
+    refreshed labels and factors like an eager solver built for the edit."""
+    print("case scaled pool capture", flush=True)
+    model, solver = pool_solver(streaming=True)
+    labels = torch.tensor([0] * 5 + [-1] * 10, dtype=torch.int32, device=DEV)
+    solver.prepare_component_scaling(labels, 6)
+    state, output = model.state(), model.state()
+    state.particle_q.copy_(pool_deformed(model))
+    state.particle_qd.copy_(state.particle_q * torch.tensor((0.1, -0.05, 0.02), device=DEV))
+    solver.refresh_tangent(state.particle_q)
+    solver.step(state, output, None, None, solver.dt)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        solver.refresh_tangent(state.particle_q)
+        solver.step(state, output, None, None, solver.dt)
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        solver.refresh_tangent(state.particle_q)
+        solver.step(state, output, None, None, solver.dt)
+    split_labels = [0] * 5 + [1] * 4 + [-1] * 6
+    pool_edit(model, POOL_SPLIT)
+    labels.copy_(torch.tensor(split_labels, dtype=torch.int32))
+    solver.refresh_component_scaling()
+    solver.refresh_topology(state.particle_q, element_sources=sources((1, 0, 0)))
+    g.replay()
+    torch.cuda.synchronize()
+    ref_model, ref = pool_solver(POOL_SPLIT, streaming=True)
+    ref.prepare_component_scaling(torch.tensor(split_labels, dtype=torch.int32, device=DEV), 6)
+    ref_in, expected = ref_model.state(), ref_model.state()
+    ref_in.particle_q.copy_(state.particle_q)
+    ref_in.particle_qd.copy_(state.particle_qd)
+    ref.refresh_tangent(ref_in.particle_q)
+    ref.step(ref_in, expected, None, None, ref.dt)
+    np.testing.assert_allclose(np_(output.particle_q), np_(expected.particle_q), rtol=2e-5, atol=2e-6)
+    np.testing.assert_allclose(np_(output.particle_qd), np_(expected.particle_qd), rtol=1e-4, atol=2e-5)
+    np.testing.assert_allclose(np_(solver._component_scaling.scale), np_(ref._component_scaling.scale), rtol=1e-7)
