@@ -9,7 +9,10 @@ block-Jacobi inverse), evaluates forces + predictor, and runs exactly 50
 PCG iterations (device residual exit disabled; the updates are
 residual-masked so the result equals the tol-1e-2 solve).
 
-Prints one JSON line (see README of the measurement contract in DESIGN.md).
+Prints one JSON line: the substep rate, the CG kernel's HBM roofline
+(algorithmic bytes over its HIP-event duration, plus the measured HBM traffic
+per launch from the committed rocprofv3 summary), the per-phase times, and the
+batched, adjoint, showcase-sized, cut-scene and CPU-baseline measurements.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -529,7 +532,8 @@ def main():
     ap.add_argument("--no-cut", action="store_true", help="skip the streaming-topology cut scene")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design only "
+                                   "(row slabs with one halo exchange and two scalar all-reduces per CG iteration)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
