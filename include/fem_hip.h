@@ -316,8 +316,9 @@ int fem_prepare_topology(fem_system* sys, int32_t original_node_count);
  * when the resident kernel holds the reserved rows.  Without a pool the
  * exact pattern of the edited model is built again (within the bound block
  * capacity; offsets and columns are rewritten and fem_pattern_size reports the
- * new nnz); device arrays the previous pattern used stay allocated until
- * fem_destroy.  Synchronizes the stream.  Not for batched systems. */
+ * new nnz); when the pattern changes, device arrays the previous pattern used
+ * stay allocated until fem_destroy (an unchanged topology allocates
+ * nothing).  Synchronizes the stream.  Not for batched systems. */
 int fem_update_topology(fem_system* sys, const int32_t* tet_indices_dev, const float* tet_poses_dev,
                         const float* tet_materials_dev, const float* particle_mass_dev,
                         const float* particle_inv_mass_dev, const int32_t* particle_flags_dev, const float* q_dev,

@@ -3431,17 +3431,23 @@ int fem_update_topology(fem_system* s, const int32_t* tet_indices, const float* 
       return fail(rc, err);
     if (np.offsets[P] > s->b_cap)
       return fail(FEM_ERR_UNSUPPORTED, "the edited topology needs more blocks than the bound matrix storage holds");
-    // Arrays a graph captured earlier may name stay allocated from here on.
-    s->keep_replaced = true;
-    s->pt = std::move(np);
-    int rc;
-    if ((rc = s->upload_pattern())) return rc;
-    std::vector<int32_t> off32(P + 1);
-    for (int i = 0; i <= P; ++i) off32[i] = (int32_t)s->pt.offsets[i];
-    HIP_TRY(hipMemcpy(s->b_offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
-    if (!s->pt.columns.empty())
-      HIP_TRY(hipMemcpy(s->b_cols, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+    const Pattern& cur = s->pt;
+    const bool same = np.offsets == cur.offsets && np.columns == cur.columns && np.tinc == cur.tinc &&
+                      np.tinc_ptr == cur.tinc_ptr && np.hinc == cur.hinc && np.hinc_ptr == cur.hinc_ptr &&
+                      np.sinc == cur.sinc && np.sinc_ptr == cur.sinc_ptr;
+    if (!same) {  // an unchanged topology keeps every array (and allocates nothing)
+      // Arrays a graph captured earlier may name stay allocated from here on.
+      s->keep_replaced = true;
+      s->pt = std::move(np);
+      int rc;
+      if ((rc = s->upload_pattern())) return rc;
+      std::vector<int32_t> off32(P + 1);
+      for (int i = 0; i <= P; ++i) off32[i] = (int32_t)s->pt.offsets[i];
+      HIP_TRY(hipMemcpy(s->b_offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
+      if (!s->pt.columns.empty())
+        HIP_TRY(hipMemcpy(s->b_cols, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
+      if ((rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+    }
   }
   if (T > 0) {
     HIP_TRY(hipMemcpyAsync(s->d_tet_idx, tet_indices, 16 * (size_t)T, hipMemcpyDeviceToDevice, st));

@@ -593,3 +593,21 @@ def test_scaled_pool_capture_replays_after_edit():
     np.testing.assert_allclose(np_(output.particle_q), np_(expected.particle_q), rtol=2e-5, atol=2e-6)
     np.testing.assert_allclose(np_(output.particle_qd), np_(expected.particle_qd), rtol=1e-4, atol=2e-5)
     np.testing.assert_allclose(np_(solver._component_scaling.scale), np_(ref._component_scaling.scale), rtol=1e-7)
+
+
+def test_unchanged_topology_refresh_keeps_every_array():
+    """A refresh_topology without a pool and without an edit re-derives the
+    same pattern and keeps the library's arrays and launch (nothing is
+    allocated, so per-frame refreshes do not grow memory)."""
+    print("case unchanged refresh", flush=True)
+    model, solver = pool_solver()
+    positions = pool_deformed(model)
+    solver.refresh_tangent(positions)
+    before = solver.A_bsr.dense()
+    torch.cuda.synchronize()
+    used = torch.cuda.mem_get_info()[0]
+    for _ in range(5):
+        solver.refresh_topology(positions)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(solver.A_bsr.dense(), before)
+    assert torch.cuda.mem_get_info()[0] >= used - (1 << 20)
