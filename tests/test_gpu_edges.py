@@ -75,3 +75,46 @@ def test_zero_material_tets_and_degenerate_springs_match_oracle():
     assert rel(np_(st0.particle_f), out["f"]) < 1e-5
     np.testing.assert_allclose(np_(st1.particle_qd), out["qd"], rtol=1e-4, atol=1e-7)
     np.testing.assert_allclose(np_(st1.particle_q), out["q"], rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("check_every", [0, 1])
+def test_fixed_physics_with_live_loads_and_warm_starts(check_every):
+    """Reference tests/test_solver_cpu_batch.py:16-81 (block_diag): each step
+    with a changed load and warm start equals the reduced dense solve of the
+    free rows, the load is not modified, anchors are exact, and an exactly
+    zero right-hand side takes no iteration."""
+    print(f"case live loads check_every={check_every}", flush=True)
+    from newton_solver_fem_amd import ModelBuilder
+    from test_gpu_parity import solver_for
+
+    b = ModelBuilder(gravity=(0.0, 0.0, 0.0))
+    for point in ((0, 0, 0), (0.125, 0, 0), (0, 0.125, 0), (0, 0, 0.125)):
+        b.add_particle(point, (0.0, 0.0, 0.0), mass=0.001, radius=0.0)
+    b.add_tetrahedron(0, 1, 2, 3, k_mu=5000, k_lambda=10000, k_damp=0)
+    model = b.finalize(DEV)
+    dt = 0.002
+    s = solver_for(model, dt=dt, fixed_particle_indices=(0, 1), linear_solver_maxiter=80,
+                   linear_solver_tolerance=1e-5)
+    s.linear_solver_check_every = check_every
+    matrix = s.A_bsr.dense()
+    current, following = model.state(), model.state()
+    elastic = s.eval_tetrahedral_forces(model, model.control(), current).cpu().numpy().astype(np.float64)
+    forces = torch.zeros(4, 3, dtype=torch.float32, device=DEV)
+    for scale, warm_scale in ((0, 2), (1, -3), (-2, 0.5), (0, 4)):
+        force = scale * np.array(((20, 30, 40), (10, -30, 60), (0.02, 0.04, 0.03), (-0.01, 0.03, 0.02)), np.float32)
+        force -= elastic.astype(np.float32)  # scale 0: an exactly zero physical right-hand side
+        warm = np.full((4, 3), warm_scale * 0.01, dtype=np.float32)
+        warm[:2] = 0
+        forces.copy_(torch.tensor(force))
+        s.velocity_increment.copy_(torch.tensor(warm))
+        expected = np.zeros(12)
+        expected[6:] = np.linalg.solve(matrix[6:, 6:], dt * (elastic + force.astype(np.float64)).ravel()[6:])
+        s.step(current, following, None, None, dt, external_forces=forces)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(s.velocity_increment.cpu().numpy().ravel(), expected, rtol=3e-5, atol=2e-7)
+        np.testing.assert_array_equal(following.particle_q.cpu().numpy()[:2], model.particle_q.cpu().numpy()[:2])
+        np.testing.assert_array_equal(following.particle_qd.cpu().numpy()[:2], np.zeros((2, 3)))
+        np.testing.assert_array_equal(forces.cpu().numpy(), force)
+        assert np.isfinite(following.particle_q.cpu().numpy()).all()
+    if check_every:
+        assert s.last_cg_iters == 0
