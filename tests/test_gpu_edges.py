@@ -118,3 +118,34 @@ def test_fixed_physics_with_live_loads_and_warm_starts(check_every):
         assert np.isfinite(following.particle_q.cpu().numpy()).all()
     if check_every:
         assert s.last_cg_iters == 0
+
+
+def test_tolerance_change_refreshes_the_solve():
+    """Reference tests/test_solver.py:160-183: a tighter tolerance set between
+    steps gives a smaller true residual after more iterations, and setting the
+    first tolerance back reproduces the first solve."""
+    print("case tolerance change", flush=True)
+    from helpers import cube
+
+    model = cube("tet", device=DEV)
+    s = solver_for(model, dt=0.002, linear_solver_maxiter=200)
+    s.linear_solver_check_every = 1
+    ext = torch.tensor(np.random.default_rng(13).normal(0, 0.03, (model.particle_count, 3)), dtype=torch.float32,
+                       device=DEV)
+    residuals, iterations = [], []
+    matrix = s.A_bsr.dense()
+    for tol in (1e-2, 1e-5, 1e-2):
+        s.linear_solver_tolerance = tol
+        s.velocity_increment.zero_()
+        initial, output = model.state(), model.state()
+        s.step(initial, output, None, None, 0.002, external_forces=ext)
+        torch.cuda.synchronize()
+        rhs = initial.particle_f.cpu().numpy().astype(np.float64).ravel()
+        r = matrix @ s.velocity_increment.cpu().numpy().astype(np.float64).ravel() - rhs
+        residuals.append(np.linalg.norm(r) / np.linalg.norm(rhs))
+        iterations.append(s.last_cg_iters)
+    assert residuals[1] < 2e-5
+    assert residuals[1] < 0.01 * residuals[0]
+    assert iterations[1] > iterations[0]
+    assert iterations[0] == iterations[2]
+    np.testing.assert_allclose(residuals[0], residuals[2], rtol=1e-5)
