@@ -335,3 +335,24 @@ def test_taped_step_rejects_aliased_states():
     cur.particle_qd = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
     with pytest.raises(ValueError, match="unique"):
         s.step(cur, cur, None, None, DT)
+
+
+def test_forward_is_independent_of_interleaved_large_adjoint_solves():
+    """Reference tests/test_solver.py:185-201: adjoint solves with huge seeds
+    between two forward rollouts leave the forward result unchanged."""
+    print("case diff interleaved adjoint", flush=True)
+    model, s, _ = make("tet")
+    _, q0, v0 = initial(model, 7)
+    act = np.full(model.tet_count, 0.02, np.float32)
+    s.refresh_tangent(torch.tensor(q0, dtype=torch.float32, device=DEV))
+
+    def forward():
+        with torch.no_grad():
+            loss, _ = _two_step_loss(model, s, q0, v0, act)
+        return float(loss)
+
+    baseline = forward()
+    for seed in (1.0e4, 1.0e8):
+        loss, _ = _two_step_loss(model, s, q0, v0, act)
+        (loss * seed).backward()
+        assert forward() == pytest.approx(baseline, rel=1e-7, abs=0)
