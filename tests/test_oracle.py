@@ -185,6 +185,110 @@ def test_tangent_quadratic_form_is_dev_hessian_plus_volumetric_gauss_newton(kind
     assert np.linalg.eigvalsh(K).min() > -1e-8 * np.abs(K).max()
 
 
+# ---------------------------------------------------------------------------
+# Rayleigh damping and tet activations, each against an independently written
+# potential: damping forces are -dD/dv of a dissipation function, activation
+# forces are -dE/dx of an activation energy.
+def dissipation(host, q, v):
+    """D(v) = sum_tet 1/2 k_d V (|dF/dt|^2 + (dJ/dt)^2) + sum_hex,gp 1/2 k_d w_g |dF/dt|^2.
+
+    Tet4 (reference kernels.py:599, :618): dF/dt = [v_b - v_0] Dm and the
+    volume rate dJ/dt of the unclamped F = Ds Dm, written here as
+    det(F) tr(F^-1 dF/dt) (Jacobi's formula).  Hex8 (kernels.py:1582): only
+    the k_d dF/dt stress at each Gauss point, weighted by det_J0_w."""
+    total = 0.0
+    for ids, pose, mat in zip(host["tet_indices"], host["tet_poses"].astype(np.float64),
+                              host["tet_materials"].astype(np.float64)):
+        grad = np.vstack((-pose.sum(axis=0), pose))
+        vol = 1.0 / (6.0 * np.linalg.det(pose))
+        F, Fd = q[ids].T @ grad, v[ids].T @ grad
+        jdot = np.linalg.det(F) * np.trace(np.linalg.solve(F, Fd))
+        total += 0.5 * mat[2] * vol * (np.sum(Fd * Fd) + jdot**2)
+    hs = [s for s in samples(host)[len(host["tet_indices"]):]]
+    kd_hex = np.repeat(host["hex_materials"][:, 2].astype(np.float64), 8) if len(host["hex_indices"]) else []
+    for (ids, g, w, _, _), kd in zip(hs, kd_hex):
+        Fd = v[ids].T @ g
+        total += 0.5 * kd * w * np.sum(Fd * Fd)
+    return total
+
+
+def activation_energy(host, q, act):
+    """E(x) = sum_tet lambda_e a_e V_e det(Ds Dm): the work of the activation
+    pressure lambda a on the volume change (reference kernels.py:485-508)."""
+    total = 0.0
+    for ids, pose, mat, a in zip(host["tet_indices"], host["tet_poses"].astype(np.float64),
+                                 host["tet_materials"].astype(np.float64), act):
+        grad = np.vstack((-pose.sum(axis=0), pose))
+        vol = 1.0 / (6.0 * np.linalg.det(pose))
+        total += mat[1] * a * vol * np.linalg.det(q[ids].T @ grad)
+    return total
+
+
+@pytest.mark.parametrize("kind", ["tet", "hex"])
+def test_rayleigh_damping_is_negative_dissipation_gradient(kind):
+    """The damping part of the element force (force at (q, v) minus force at
+    (q, 0); the elastic part does not depend on v) equals -dD/dv by central
+    differences of D, which is quadratic in v, so the differences are exact up
+    to rounding.  A flipped sign, a dropped V or a dropped volume-rate term
+    changes the result by O(1)."""
+    model = cube(kind, material=(500.0, 1000.0, 3.0))
+    o = make(model)
+    rng = np.random.default_rng(21)
+    q = model.host["particle_q"].astype(np.float64)
+    q = q + rng.normal(0.0, 0.003, q.shape)
+    v = rng.normal(0.0, 0.2, q.shape)
+    term = oracle.TERM_TET if kind == "tet" else oracle.TERM_HEX
+    f_damp = o.eval_terms(term, q, v) - o.eval_terms(term, q, np.zeros_like(v))
+    assert np.abs(f_damp).max() > 1e-3
+    for _ in range(4):
+        w = rng.normal(size=q.shape)
+        eps = 1e-3
+        fd = (dissipation(model.host, q, v + eps * w) - dissipation(model.host, q, v - eps * w)) / (2 * eps)
+        assert np.sum(f_damp * w) == pytest.approx(-fd, rel=1e-8, abs=1e-12)
+    # Linear in v, and zero for a uniform translation velocity.
+    np.testing.assert_allclose(o.eval_terms(term, q, 2.0 * v) - o.eval_terms(term, q, np.zeros_like(v)),
+                               2.0 * f_damp, rtol=1e-9, atol=1e-12)
+    drift = np.tile((0.3, -0.2, 0.1), (len(q), 1))
+    np.testing.assert_allclose(o.eval_terms(term, q, drift), o.eval_terms(term, q, np.zeros_like(v)), atol=1e-10)
+    if kind == "tet":
+        # The volume-rate term carries weight in this configuration, so a
+        # force without it would fail the differences above.
+        D_no_vol = 0.0
+        for ids, pose, mat in zip(model.host["tet_indices"], model.host["tet_poses"].astype(np.float64),
+                                  model.host["tet_materials"].astype(np.float64)):
+            grad = np.vstack((-pose.sum(axis=0), pose))
+            Fd = v[ids].T @ grad
+            D_no_vol += 0.5 * mat[2] * np.sum(Fd * Fd) / (6.0 * np.linalg.det(pose))
+        assert abs(D_no_vol - dissipation(model.host, q, v)) > 1e-3 * abs(D_no_vol)
+
+
+def test_tet_activation_force_is_negative_activation_energy_gradient():
+    """Activation force = force with activations minus force without, at zero
+    velocity, against -dE/dx of activation_energy (cubic in x: central
+    differences with eps = 1e-6 are exact to ~1e-12 relative)."""
+    model = cube("tet", material=(500.0, 1000.0, 0.0))
+    o = make(model, enable_tet_activations=True)
+    rng = np.random.default_rng(23)
+    q = model.host["particle_q"].astype(np.float64)
+    q = q + rng.normal(0.0, 0.004, q.shape)
+    act = rng.uniform(0.0, 0.3, model.host["tet_indices"].shape[0])
+    zero = np.zeros_like(q)
+    f_act = o.eval_terms(oracle.TERM_TET, q, zero, activations=act) - o.eval_terms(oracle.TERM_TET, q, zero)
+    assert np.abs(f_act).max() > 1e-3
+    for _ in range(4):
+        d = rng.normal(size=q.shape)
+        eps = 1e-6
+        fd = (activation_energy(model.host, q + eps * d, act) - activation_energy(model.host, q - eps * d, act)) / (2 * eps)
+        assert np.sum(f_act * d) == pytest.approx(-fd, rel=1e-7, abs=1e-12)
+    np.testing.assert_allclose(f_act.sum(axis=0), 0.0, atol=1e-12)
+    # Linear in the activations; disabled activations leave the force unchanged.
+    f2 = o.eval_terms(oracle.TERM_TET, q, zero, activations=2.0 * act) - o.eval_terms(oracle.TERM_TET, q, zero)
+    np.testing.assert_allclose(f2, 2.0 * f_act, rtol=1e-10, atol=1e-14)
+    off = make(model, enable_tet_activations=False)
+    np.testing.assert_array_equal(off.eval_terms(oracle.TERM_TET, q, zero, activations=act),
+                                  off.eval_terms(oracle.TERM_TET, q, zero))
+
+
 def test_ground_contact_closed_form():
     """Reference test_solver.py:333-347."""
     model = cube(gravity=(0, 0, -9.81))
