@@ -322,26 +322,29 @@ def test_drop_rollout_positions():
 
 
 def test_hex_cantilever_rollout():
-    """Hex8 64x16x16 cantilever, x=0 fixed: 25 substeps vs the FP64 oracle."""
+    """Hex8 64x16x16 cantilever, x=0 fixed: 100 substeps vs the FP64 oracle (positions 1e-4 relative)."""
     print("case cantilever", flush=True)
     sc = meshes.hex_cantilever_scene()
     model = build_hex_model(sc, device=DEV)
     kw = dict(dt=1 / 600, linear_solver_maxiter=50, linear_solver_tolerance=1e-2)
     s = solver_for(model, fixed_particle_indices=sc.fixed, **kw)
     ref = oracle_for(model, fixed=sc.fixed, **kw)
+    ref.set_threads(16)
     st0, st1 = model.state(), model.state()
     q, v = np_(st0.particle_q), np_(st0.particle_qd)
-    for k in range(25):
+    worst = 0.0
+    for k in range(100):
         s.refresh_tangent(st0.particle_q)
         s.step(st0, st1, None, None, 1 / 600)
         ref.refresh_tangent(q)
         out = ref.step(q, v)
         q, v = out["q"], out["qd"]
         st0, st1 = st1, st0
-    err = rel(np_(st0.particle_q), q)
-    disp = np.abs(np_(st0.particle_q) - model.host["particle_q"]).max()
-    print(f"cantilever position rel err {err:.2e}, tip displacement {disp:.3e}", flush=True)
-    assert err < 1e-4
+        worst = max(worst, rel(np_(st0.particle_q), q))
+        if k % 25 == 24:
+            disp = np.abs(np_(st0.particle_q) - model.host["particle_q"]).max()
+            print(f"  substep {k + 1}: worst position rel err {worst:.2e}, tip displacement {disp:.3e}", flush=True)
+    assert worst < 1e-4
     np.testing.assert_array_equal(np_(st0.particle_q)[sc.fixed], model.host["particle_q"][sc.fixed])
 
 
