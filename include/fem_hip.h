@@ -200,7 +200,13 @@ int fem_eval_forces(fem_system* sys, int32_t terms, const float* q_dev, const fl
 
 /* Last solve's per-instance statistics (synchronizes the stream): CG
  * iterations performed, final squared residual, squared absolute tolerance,
- * and FEM_STATUS_* error flags.  Arrays have instance_count entries. */
+ * and FEM_STATUS_* error flags.  Arrays have instance_count entries.  A solve
+ * whose persistent launch was not fully co-resident (another kernel or
+ * process holding CUs) ends its bounded waits with
+ * FEM_STATUS_BARRIER_TIMEOUT and writes NaN into the increment and the
+ * output state (or the solution of fem_solve) instead of a partial solve, so
+ * an unsynchronized or recorded step cannot return a plausible wrong state;
+ * the warm start stays NaN until it is cleared. */
 int fem_solve_stats(fem_system* sys, void* stream, int32_t* iterations, float* residual_sq, float* atol_sq,
                     int32_t* error_flags);
 
@@ -214,6 +220,21 @@ int fem_apply_operator(fem_system* sys, const float* x_dev, float* y_dev, void* 
 
 /* apply_preconditioner (:243-251): y = M^-1 x with the block-Jacobi inverse. */
 int fem_apply_preconditioner(fem_system* sys, const float* x_dev, float* y_dev, void* stream);
+
+/* Bound of every wait of the persistent CG, in polls of the awaited flag (a
+ * device-scope load and a short sleep each; 0 restores the default 2^22,
+ * which bounds a wait to seconds).
+ * No counterpart in the reference, whose CG needs no cross-workgroup waits;
+ * a small bound forces the barrier-timeout path of a multi-workgroup solve
+ * from the host (failure-handling tests). */
+int fem_set_barrier_spin_limit(fem_system* sys, int64_t spins);
+
+/* The preconditioner operator SolverFEM.M_bsr (solver.py:1175-1201) applied as
+ * LinearOperator.matvec(x, y, z, alpha, beta): z = alpha M^-1 x + beta y with
+ * the block-Jacobi inverse (block_jacobi_mv_kernel kernels.py:1852-1868).  y
+ * may be NULL when beta is 0; z may alias x or y. */
+int fem_block_jacobi_mv(fem_system* sys, const float* x_dev, const float* y_dev, float* z_dev, float alpha,
+                        float beta, void* stream);
 
 /* diagonal_blocks (_diagonal :46-61): out[i] = A_ii (zero when the row has no
  * diagonal block), the identity on fixed rows; [P][9]. */

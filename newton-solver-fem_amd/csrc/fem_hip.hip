@@ -891,6 +891,7 @@ struct CgArgs {
   unsigned* flags;  // [nchunks_total] last phase each chunk has published
   int* stats;
   int* err;
+  long spin_limit;  // sleep-polls a wait may take before it gives up
 };
 
 __device__ __forceinline__ unsigned ld_acq(const unsigned* p) {
@@ -900,11 +901,11 @@ __device__ __forceinline__ unsigned ld_acq(const unsigned* p) {
 // Spin until the monotonic word *p reaches target; bounded so that a launch
 // whose workgroups are not all resident ends with an error flag instead of
 // hanging.
-__device__ __forceinline__ bool spin_until(const unsigned* p, unsigned target, int* err) {
+__device__ __forceinline__ bool spin_until(const unsigned* p, unsigned target, int* err, long limit) {
   long spins = 0;
   while ((int)(ld_acq(p) - target) < 0) {
     __builtin_amdgcn_s_sleep(1);
-    if (++spins > BARRIER_SPIN_LIMIT ||
+    if (++spins > limit ||
         ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
       __hip_atomic_fetch_or(err, FEM_STATUS_BARRIER_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
@@ -989,13 +990,14 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
 // false (every thread) when a wait timed out.
 template <int NT, int N>
 __device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* part, int cb, int ce, unsigned epoch,
-                                         const int (&slot)[N], double (&out)[N], double* red, int* err) {
+                                         const int (&slot)[N], double (&out)[N], double* red, int* err,
+                                         long limit) {
   double acc[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) acc[n] = 0.0;
   int bad = 0;
   for (int c = cb + (int)threadIdx.x; c < ce; c += NT) {
-    if (ce - cb > 1 && !spin_until(flags + c, epoch, err)) {
+    if (ce - cb > 1 && !spin_until(flags + c, epoch, err, limit)) {
       bad = 1;
       break;
     }
@@ -1531,8 +1533,9 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   double tot[3];
   {
     const int sl[3] = {PT_BB, PT_RR, PT_RZ};
-    ok = wait_sum<NT, 3>(a.flags, a.part, cb, ce, epoch, sl, tot, red, a.err);
+    ok = wait_sum<NT, 3>(a.flags, a.part, cb, ce, epoch, sl, tot, red, a.err, a.spin_limit);
   }
+  bool failed = !ok;
   double rr = tot[1], rz = tot[2];
   double atol = fmax(0.0, (double)a.tolerance * sqrt(tot[0]));
   atol = fmax(atol, residual_floor());
@@ -1545,7 +1548,10 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       double g[7];
       {
         const int sl[7] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR};
-        if (!wait_sum<NT, 7>(a.flags, a.part + NPART * nch * cur, cb, ce, epoch, sl, g, red, a.err)) break;
+        if (!wait_sum<NT, 7>(a.flags, a.part + NPART * nch * cur, cb, ce, epoch, sl, g, red, a.err, a.spin_limit)) {
+          failed = true;
+          break;
+        }
       }
       rz = g[5];
       rr = g[6];
@@ -1628,20 +1634,27 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     }
     publish(a.flags + c, ++epoch);
   }
+  // A workgroup whose waits timed out (or that sees another's timeout) has no
+  // valid iterate: it writes NaN instead of a plausible partial solve, so a
+  // caller that does not read the status word still cannot take the result
+  // for a state (the increment stays NaN until the next reset).
+  if (!failed) failed = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
   // restore_linear_increment kernels.py:1808-1811, update_state :115-128,
   // FixedParticles.restore(state_out) solver.py:1350-1351.
+  const float nan = __builtin_nanf("");
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (own[k] && a.x_out) {
-      st3(a.x_out, R0 + threadIdx.x + NT * k, v3{x[k][0], x[k][1], x[k][2]} * (1.f / s));
+      const v3 xo = failed ? v3{nan, nan, nan} : v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
+      st3(a.x_out, R0 + threadIdx.x + NT * k, xo);
     } else if (own[k]) {
       const int i = R0 + threadIdx.x + NT * k;
-      const v3 dvi = v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
+      const v3 dvi = failed ? v3{nan, nan, nan} : v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
       st3(a.dv, i, dvi);
       if (a.q_out) {
         v3 vel = ld3(a.qd_in, i) + dvi;
         v3 qo = ld3(a.q_in, i) + vel * a.h;
-        if (fx[k]) {
+        if (fx[k] && !failed) {
           qo = ld3(a.anchor, i);
           vel = v3{0.f, 0.f, 0.f};
         }
@@ -1691,6 +1704,17 @@ __global__ void k_apply_minv(const float* __restrict__ minv, const float* __rest
                              int P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < P) st3(y, i, mv(minv + 9 * (size_t)i, ld3(x, i)));
+}
+
+// z = alpha M^-1 x + beta y (block_jacobi_mv_kernel kernels.py:1852-1868);
+// y is not read when beta is 0; z may alias x or y (one row per thread).
+__global__ void k_block_jacobi_mv(const float* __restrict__ minv, const float* x, const float* y, float* z,
+                                  float alpha, float beta, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  v3 out = mv(minv + 9 * (size_t)i, ld3(x, i)) * alpha;
+  if (beta != 0.f) out = out + ld3(y, i) * beta;
+  st3(z, i, out);
 }
 
 // Unconstrained diagonal blocks, identity on fixed rows (prepared_step.py:46-61).
@@ -1937,6 +1961,7 @@ struct fem_system {
   double *d_norms = nullptr, *d_dstats = nullptr;  // checked-solve residual norms and running maxima
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
+  long spin_limit = BARRIER_SPIN_LIMIT;  // bound of every persistent-CG wait
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
       *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
   uint16_t* d_lidx = nullptr;
@@ -2939,6 +2964,7 @@ CgArgs cg_args(const fem_system* s) {
   ca.flags = s->d_cgflags;
   ca.stats = s->d_stats;
   ca.err = s->d_err;
+  ca.spin_limit = s->spin_limit;
   ca.nchunks_total = s->nchunks;
   return ca;
 }
@@ -3044,6 +3070,24 @@ int fem_apply_preconditioner(fem_system* s, const float* x, float* y, void* stre
   if (s->P == 0) return FEM_OK;
   hipLaunchKernelGGL(k_apply_minv, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
                      (const float*)s->b_minv, x, y, s->P);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+
+int fem_set_barrier_spin_limit(fem_system* s, int64_t spins) {
+  if (!s) return fail(FEM_ERR_ARG, "null argument");
+  if (spins < 0) return fail(FEM_ERR_ARG, "the spin limit must be nonnegative");
+  s->spin_limit = spins == 0 ? BARRIER_SPIN_LIMIT : (long)spins;
+  return FEM_OK;
+}
+
+int fem_block_jacobi_mv(fem_system* s, const float* x, const float* y, float* z, float alpha, float beta,
+                        void* stream) {
+  if (!s || !x || !z || (beta != 0.f && !y)) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_minv) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (s->P == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_block_jacobi_mv, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
+                     (const float*)s->b_minv, x, y, z, alpha, beta, s->P);
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }

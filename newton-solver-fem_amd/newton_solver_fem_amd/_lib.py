@@ -96,6 +96,8 @@ SIGNATURES = {
     "fem_set_tet_materials": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fem_apply_operator": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "fem_apply_preconditioner": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "fem_set_barrier_spin_limit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "fem_block_jacobi_mv": (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_float, ctypes.c_float, ctypes.c_void_p]),
     "fem_diagonal_blocks": (ctypes.c_int, [ctypes.c_void_p] * 3),
     "fem_add_load": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
     "fem_solve_increment": (ctypes.c_int, [ctypes.c_void_p] * 4),

@@ -99,6 +99,13 @@ class Model:
         self.gravity_array = torch.zeros(3, dtype=torch.float32, device=self.device)
         self.gravity = gravity
         self.particle_count = int(len(host["particle_q"]))
+        # Hash grid of particle-particle contact.  Models built here are
+        # FEM meshes and carry none, so SolverFEM skips that force term, as
+        # the reference does when Model.particle_grid is None
+        # (solver.py:1477-1485, kernels.py:1032-1038).
+        self.particle_grid = None
+        self.particle_ke = 0.0
+        self.particle_kd = 0.0
         self.tet_count = int(len(host["tet_indices"]))
         self.spring_count = int(len(host["spring_indices"]))
         self.tri_count = int(len(host["tri_indices"]))

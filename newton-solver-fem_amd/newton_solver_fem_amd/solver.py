@@ -26,6 +26,31 @@ SUPPORTED_SOLVER = "cg"
 SUPPORTED_PRECONDITIONER = "block_diag"
 
 
+class BlockJacobiOperator:
+    """The block-Jacobi preconditioner ``M_bsr`` of the reference
+    (solver.py:1175-1201: a warp LinearOperator over ``_pc_inv_diag``).
+
+    ``matvec(x, y, z, alpha, beta)`` writes ``z = alpha M^-1 x + beta y``
+    (block_jacobi_mv_kernel kernels.py:1852-1868) on the HIP device.
+    """
+
+    def __init__(self, solver):
+        self._solver = solver
+        n = solver.model.particle_count
+        self.shape = (3 * n, 3 * n)
+        self.dtype = torch.float32
+        self.device = solver.device
+
+    def matvec(self, x, y, z, alpha, beta):
+        s = self._solver
+        n = s.model.particle_count
+        for name, t in (("x", x), ("y", y), ("z", z)):
+            _check_vec3(name, t, n, s.device)
+        _lib.check(s._lib.fem_block_jacobi_mv(s._sys, x.data_ptr(), y.data_ptr(), z.data_ptr(), float(alpha),
+                                              float(beta), _stream(s.device)), "M_bsr.matvec")
+        return z
+
+
 @dataclass
 class BSRMatrix:
     """Block-sparse (3x3) matrix view over device tensors (warp BsrMatrix subset)."""
@@ -138,8 +163,15 @@ class SolverFEM:
             raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
         if torque_stiffness > 0.0 and spring_rest_direction is not None:
             raise NotImplementedError("spring bend torque is not part of this substep path")
-        if self_contact_ke > 0.0 and model.particle_count > 1 and float(np.max(model.host["particle_radius"], initial=0)) > 0:
-            raise NotImplementedError("particle self-contact is not part of this substep path; pass self_contact_ke=0")
+        # Particle-particle contact runs only for models with a hash grid
+        # (reference eval_particle_particle_forces solver.py:1477-1485); FEM
+        # meshes have none and skip the term whatever self_contact_ke is.
+        if (self_contact_ke > 0.0 and model.particle_count > 1 and getattr(model, "particle_grid", None) is not None
+                and float(np.max(model.host["particle_radius"], initial=0)) > 0):
+            raise NotImplementedError("hash-grid particle self-contact is not part of this substep path")
+        # The reference stores the contact coefficients on the model (solver.py:266-267).
+        model.particle_ke = float(self_contact_ke)
+        model.particle_kd = float(self_contact_kd)
         if model.device.type != "cuda":
             raise ValueError("SolverFEM runs on the HIP device: finalize the model with device='cuda'")
         if tet_volume_weights is not None:
@@ -257,6 +289,7 @@ class SolverFEM:
         self._q_asm = model.particle_q.detach().clone() if self.differentiable else None
         self._diff_revision = None
         self._initial_volume = self._compute_volume(model.particle_q)
+        self.M_bsr = BlockJacobiOperator(self)
 
     # ------------------------------------------------------------------
     def _bind_matrix(self, capacity: int):
@@ -349,9 +382,7 @@ class SolverFEM:
         if self._topology_signature() != self._topology_storage:
             raise ValueError("Topology updates must preserve model capacities and array storage")
         P = self.model.particle_count
-        if (not isinstance(particle_q, torch.Tensor) or tuple(particle_q.shape) != (P, 3)
-                or particle_q.dtype != torch.float32 or particle_q.device != self.device):
-            raise ValueError("particle_q must match the model's particle shape and device")
+        _check_vec3("particle_q", particle_q, P, self.device)
         sources = None
         if self._streaming and element_sources is not None:
             if (not isinstance(element_sources, torch.Tensor) or element_sources.dtype != torch.int32
@@ -368,7 +399,7 @@ class SolverFEM:
             self._lib.fem_update_topology(
                 self._sys, m.tet_indices.data_ptr(), m.tet_poses.data_ptr(), m.tet_materials.data_ptr(),
                 m.particle_mass.data_ptr(), m.particle_inv_mass.data_ptr(), m.particle_flags.data_ptr(),
-                particle_q.contiguous().data_ptr(), _ptr(sources), dt, stats, st),
+                particle_q.data_ptr(), _ptr(sources), dt, stats, st),
             "refresh_topology",
         )
         if self._streaming:
@@ -678,6 +709,12 @@ class SolverFEM:
         if not hasattr(self, "_prepared_step"):
             self._prepared_step = PreparedFEMStep(self)
         return self._prepared_step.prepare(state_in, control, contacts, dt, external_forces)
+
+    def set_barrier_spin_limit(self, spins: int = 0) -> None:
+        """Bound the persistent CG's cross-workgroup waits (flag polls; 0 =
+        default).  A wait that runs out marks the solve as timed out and the
+        step writes NaN instead of a partial result (see :meth:`solve_stats`)."""
+        _lib.check(self._lib.fem_set_barrier_spin_limit(self._sys, int(spins)), "set_barrier_spin_limit")
 
     def solve_stats(self) -> dict:
         """Per-instance statistics of the last solve (synchronizes)."""
