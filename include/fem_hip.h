@@ -191,6 +191,22 @@ int fem_step_rhs(fem_system* sys, float* q_in_dev, float* qd_in_dev, float* f_ou
 int fem_step_solve(fem_system* sys, const float* q_in_dev, const float* qd_in_dev, const float* f_in_dev,
                    float* q_out_dev, float* qd_out_dev, float dt, void* stream);
 
+/* fem_step_solve without library-allocated scratch, for callers that cannot
+ * allocate device memory (code under HIP graph capture that must not touch
+ * the allocator, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The persistent CG's partial sums,
+ * publication flags and exchange records live in a static device array of
+ * 32 MiB per GPU (held until the process ends) instead of the system's own
+ * buffers: 96 bytes per row plus 132 bytes per CG chunk, so up to about
+ * 340,000 rows; a larger system returns FEM_ERR_UNSUPPORTED before any
+ * launch.  The call allocates and frees nothing; the flags it needs zeroed
+ * are cleared by a kernel on the stream.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other systems)
+ * race and must be serialized by the caller.  Results equal fem_step_solve's
+ * bit for bit. */
+int fem_step_solve_static(fem_system* sys, const float* q_in_dev, const float* qd_in_dev, const float* f_in_dev,
+                          float* q_out_dev, float* qd_out_dev, float dt, void* stream);
+
 /* Sum of the selected unscaled force terms (FEM_TERM_* mask) at (q, qd):
  * eval_tetrahedra + eval_tet_activations (kernels.py:484-632),
  * eval_hexahedra (:1481-1601), eval_particle_ground_contacts (:1132-1195),

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1024,6 +1025,15 @@ __device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* pa
   return bad == 0;
 }
 
+// Scratch of fem_step_solve_static (callers that cannot allocate): the
+// persistent CG's partial sums, publication flags and exchange records live
+// in this file-scope array instead of library-allocated memory.  Every GPU
+// the library runs on holds its own copy until the process ends.  The host
+// never uses the array's name as an address: its device address is taken
+// once per device with hipGetSymbolAddress (static_scratch_ptr).
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
 constexpr int CG_J = 8;  // blocks per thread per tile (a tile is CG_J * NT blocks)
 // Exchange record of one row: z(3) w(3) p(3) + pad, 48 bytes (three 16-byte
 // aligned loads per gathered column).
@@ -1962,6 +1972,7 @@ struct fem_system {
   double* d_part = nullptr;
   int *d_stats = nullptr, *d_err = nullptr;
   long spin_limit = BARRIER_SPIN_LIMIT;  // bound of every persistent-CG wait
+  unsigned char* static_scratch = nullptr;  // this device's `scratch` (device address)
   int *d_chunk_rows = nullptr, *d_chunk_inst = nullptr, *d_inst_chunks = nullptr, *d_chunk_tiles = nullptr,
       *d_tile_rows = nullptr, *d_tile_cptr = nullptr, *d_tile_cols = nullptr;
   uint16_t* d_lidx = nullptr;
@@ -2350,7 +2361,8 @@ struct fem_system {
     // of the 256-thread kernel); the resident kernel also needs enough chunks
     // for every chunk's blocks to fit on chip.  cg_wg_request overrides.
     const long res_blocks = (long)RES_NT * (RES_JR + RES_JL + RES_JS);  // blocks one resident chunk holds
-    long natural = (nt == 512 || res) ? std::min<long>(P / 512, ncu) : (long)P / 512;
+    // (The 256-thread streamed kernel holds two workgroups per CU.)
+    long natural = (nt == 512 || res) ? std::min<long>(P / 512, ncu) : std::min<long>(P / 512, 2L * ncu);
     if (res)
       natural = std::min<long>(ncu, std::max<long>(natural, (long)std::ceil((double)koff[P] / (0.95 * res_blocks))));
     long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, natural);
@@ -2643,6 +2655,26 @@ struct fem_system {
 };
 
 namespace {
+// Device address of `scratch` on the current device, taken once per device
+// (hipGetSymbolAddress) and kept for the life of the process.
+int static_scratch_ptr(unsigned char** out) {
+  constexpr int MAX_DEVICES = 64;
+  static std::mutex mu;
+  static unsigned char* addr[MAX_DEVICES] = {};
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(FEM_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FEM_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    addr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = addr[dev];
+  return FEM_OK;
+}
+
 int pick_lanes(double avg) {
   int g = 4;
   while (g < 64 && g < avg) g *= 2;
@@ -2766,6 +2798,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
   s->prm.early_exit = 1;
   s->prm.gravity[2] = -9.81f;
   if (P > 0 && (rc = s->plan_cg())) return cleanup(rc);
+  if ((rc = static_scratch_ptr(&s->static_scratch))) return cleanup(rc);
   *out = s;
   return FEM_OK;
 }
@@ -2973,7 +3006,7 @@ CgArgs cg_args(const fem_system* s) {
 int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
   const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt, s->cg_res);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
-  hipLaunchKernelGGL(k_cg_reset, dim3((s->nchunks + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s->d_cgflags,
+  hipLaunchKernelGGL(k_cg_reset, dim3((s->nchunks + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ca.flags,
                      s->nchunks, s->d_err);
   HIP_TRY(hipGetLastError());
   for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
@@ -2985,15 +3018,41 @@ int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
 }
 }  // namespace
 
-extern "C" {
+namespace {
+// Bytes of static scratch a solve of `s` needs: partial sums (2 x NPART
+// doubles per chunk), then the publication flags (one word per chunk), then
+// the exchange records (2 x XW floats per row), each part 256-byte aligned.
+size_t static_scratch_layout(const fem_system* s, size_t* flags_off, size_t* xw_off) {
+  auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+  const size_t part = up(sizeof(double) * 2 * NPART * (size_t)s->nchunks);
+  const size_t flags = up(sizeof(unsigned) * (size_t)s->nchunks);
+  *flags_off = part;
+  *xw_off = part + flags;
+  return part + flags + sizeof(float) * 2 * XW * (size_t)s->P;
+}
 
-int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
-                   float* qd_out, float dt, void* stream) {
+int step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out, float* qd_out,
+               float dt, void* stream, bool use_static) {
   if (!s || !q_in || !qd_in || !f_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   CgArgs ca = cg_args(s);
+  if (use_static) {
+    size_t flags_off = 0, xw_off = 0;
+    const size_t need = static_scratch_layout(s, &flags_off, &xw_off);
+    if (need > STATIC_SCRATCH_BYTES) {
+      char buf[200];
+      std::snprintf(buf, sizeof buf,
+                    "the solve needs %zu bytes of static scratch (%d rows, %d CG chunks); the limit is %zu bytes",
+                    need, s->P, s->nchunks, STATIC_SCRATCH_BYTES);
+      return fail(FEM_ERR_UNSUPPORTED, buf);
+    }
+    if (!s->static_scratch) return fail(FEM_ERR_STATE, "static scratch address not initialised");
+    ca.part = reinterpret_cast<double*>(s->static_scratch);
+    ca.flags = reinterpret_cast<unsigned*>(s->static_scratch + flags_off);
+    ca.xw = reinterpret_cast<float*>(s->static_scratch + xw_off);
+  }
   ca.rhs = f_in;
   if (s->cs_scale) {
     // Scaled copies (ComponentScaling.prepare, solver.py:1300-1304), then the
@@ -3020,6 +3079,19 @@ int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const f
   ca.qd_out = qd_out;
   ca.h = dt;
   return launch_cg(s, ca, (hipStream_t)stream);
+}
+}  // namespace
+
+extern "C" {
+
+int fem_step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
+                   float* qd_out, float dt, void* stream) {
+  return step_solve(s, q_in, qd_in, f_in, q_out, qd_out, dt, stream, false);
+}
+
+int fem_step_solve_static(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out,
+                          float* qd_out, float dt, void* stream) {
+  return step_solve(s, q_in, qd_in, f_in, q_out, qd_out, dt, stream, true);
 }
 
 int fem_solve(fem_system* s, const float* b, float* x, float tolerance, int32_t maxiter, int32_t kind,

@@ -12,6 +12,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libfem_hip.so"
 
 FEM_OK = 0
+FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_UNSUPPORTED, FEM_ERR_STATE = 1, 2, 3, 4
 FEM_TERM_TET, FEM_TERM_HEX, FEM_TERM_GROUND, FEM_TERM_GRAVITY, FEM_TERM_DAMPING = 1, 2, 4, 8, 16
 FEM_TERM_SPRING, FEM_TERM_TRI = 32, 64
 
@@ -88,6 +89,7 @@ SIGNATURES = {
     "fem_step": (ctypes.c_int, [ctypes.c_void_p] * 8 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_step_rhs": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_step_solve": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
+    "fem_step_solve_static": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_float, ctypes.c_void_p]),
     "fem_eval_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 5),
     "fem_solve_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _ip, _fp, _fp, _ip]),
     "fem_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,

@@ -1,0 +1,98 @@
+"""fem_step_solve_static: the allocation-free solve through the C-ABI.
+
+Two systems are solved in a row with the static-scratch entry point, the
+second smaller than the first (a 1331-node drop mesh split into four CG
+chunks, then a 27-node cube), each from the same inputs as the library's
+fem_step_solve: the increments and output states must be equal bit for bit.
+A system whose scratch would exceed the static array is rejected before any
+launch.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cube
+from newton_solver_fem_amd import ModelBuilder, _lib, build_tet_model, meshes
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def solver_for(model, **kw):
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    base = dict(solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0,
+                linear_solver_tolerance=1e-6, linear_solver_maxiter=200)
+    base.update(kw)
+    return SolverFEM(model, **base)
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _solve_both(s, model, dt, seed):
+    """(dv, q, qd) of fem_step_solve and of fem_step_solve_static from the same
+    right-hand side and warm start."""
+    lib, h = s._lib, s._sys
+    rng = np.random.default_rng(seed)
+    st = model.state()
+    st.particle_q.add_(torch.tensor(rng.normal(0, 1e-3, st.particle_q.shape), dtype=torch.float32, device=DEV))
+    st.particle_qd.copy_(torch.tensor(rng.normal(0, 0.05, st.particle_qd.shape), dtype=torch.float32))
+    s.refresh_tangent(st.particle_q)
+    rhs = torch.empty_like(st.particle_q)
+    _lib.check(lib.fem_step_rhs(h, st.particle_q.data_ptr(), st.particle_qd.data_ptr(), rhs.data_ptr(), None, None,
+                                ctypes.c_float(dt), _stream()), "fem_step_rhs")
+    warm = torch.tensor(rng.normal(0, 0.01, st.particle_q.shape), dtype=torch.float32, device=DEV)
+    out = []
+    for name in ("fem_step_solve", "fem_step_solve_static"):
+        s.dv.copy_(warm)
+        q1, qd1 = torch.full_like(st.particle_q, -1.0), torch.full_like(st.particle_q, -1.0)
+        fn = getattr(lib, name)
+        _lib.check(fn(h, st.particle_q.data_ptr(), st.particle_qd.data_ptr(), rhs.data_ptr(), q1.data_ptr(),
+                      qd1.data_ptr(), ctypes.c_float(dt), _stream()), name)
+        it = s.solve_stats()["iterations"][0]
+        out.append((s.dv.cpu().numpy().copy(), q1.cpu().numpy(), qd1.cpu().numpy(), it))
+    return out
+
+
+def test_static_scratch_solve_equals_library_scratch_solve():
+    print("case static-scratch solve", flush=True)
+    dt = 1 / 600
+    big = build_tet_model(meshes.tet_drop_scene(), device=DEV)
+    sb = solver_for(big, dt=dt, ground_plane=(0, 0, 1, 0), ground_ke=100.0, ground_kd=0.1, ground_kf=1.0,
+                    ground_mu=0.5)
+    sb.set_launch(0, 0, 4)
+    assert sb.launch_config()["cg_workgroups"] == 4
+    small = cube("tet", device=DEV, material=(500.0, 1000.0, 0.1), gravity=(0.0, 0.0, -9.81))
+    ss = solver_for(small, dt=dt)
+    for s, m, seed in ((sb, big, 1), (ss, small, 2)):
+        (dv_a, q_a, qd_a, it_a), (dv_b, q_b, qd_b, it_b) = _solve_both(s, m, dt, seed)
+        print(f"  {m.particle_count} rows: iterations {it_a} / {it_b}", flush=True)
+        assert it_a == it_b and it_a > 1
+        assert np.isfinite(q_b).all()
+        np.testing.assert_array_equal(dv_b, dv_a)
+        np.testing.assert_array_equal(q_b, q_a)
+        np.testing.assert_array_equal(qd_b, qd_a)
+
+
+def test_static_scratch_rejects_systems_beyond_its_size():
+    print("case static-scratch limit", flush=True)
+    n = 360_000  # 96 bytes of records per row: beyond 32 MiB
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    pts = np.zeros((n, 3))
+    pts[:, 0] = np.arange(n) * 1e-3
+    b.add_particles(pts, np.zeros_like(pts), np.ones(n), radius=0.0)
+    model = b.finalize(DEV)
+    s = solver_for(model, dt=1 / 600)
+    st0, st1 = model.state(), model.state()
+    f = torch.zeros_like(st0.particle_q)
+    rc = s._lib.fem_step_solve_static(s._sys, st0.particle_q.data_ptr(), st0.particle_qd.data_ptr(), f.data_ptr(),
+                                      st1.particle_q.data_ptr(), st1.particle_qd.data_ptr(), ctypes.c_float(1 / 600),
+                                      _stream())
+    assert rc == _lib.FEM_ERR_UNSUPPORTED
+    assert b"static scratch" in s._lib.fem_last_error()
