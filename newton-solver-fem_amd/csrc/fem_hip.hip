@@ -961,7 +961,10 @@ __device__ __forceinline__ void publish(unsigned* flag, unsigned epoch) {
 }
 
 // Sum of N floats over the workgroup (fp32 within waves, fp64 across waves,
-// fixed order); every thread receives the result.
+// fixed order); thread 0 receives the result (`out` is unset elsewhere).
+// One barrier: the LDS words `red` are written again only by wait_sum, which
+// every thread reaches through publish()'s barrier, after thread 0 has read
+// them here.
 template <int NT, int N>
 __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* red) {
 #pragma unroll
@@ -973,14 +976,15 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
 #pragma unroll
     for (int n = 0; n < N; ++n) red[w * N + n] = (double)v[n];
   __syncthreads();
+  if (threadIdx.x == 0) {
 #pragma unroll
-  for (int n = 0; n < N; ++n) {
-    double t = red[n];
+    for (int n = 0; n < N; ++n) {
+      double t = red[n];
 #pragma unroll
-    for (int w = 1; w < NT / 64; ++w) t += red[w * N + n];
-    out[n] = t;
+      for (int w = 1; w < NT / 64; ++w) t += red[w * N + n];
+      out[n] = t;
+    }
   }
-  __syncthreads();
 }
 
 // Start of a phase: wait until every chunk of the instance has published
@@ -988,7 +992,9 @@ __device__ __forceinline__ void wg_sum(float (&v)[N], double (&out)[N], double* 
 // every workgroup (same data, same reduction tree).  Thread t waits for
 // chunks cb+t, cb+t+NT, ... and reads their sums as soon as their flags are
 // up.  A single-chunk instance reads its own sums without waiting.  Returns
-// false (every thread) when a wait timed out.
+// false (every thread) when a wait timed out.  No trailing barrier: `red` is
+// written again only by wg_sum, which every thread reaches through the
+// SpMV's barriers, after all have read it here.
 template <int NT, int N>
 __device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* part, int cb, int ce, unsigned epoch,
                                          const int (&slot)[N], double (&out)[N], double* red, int* err,
@@ -1021,7 +1027,6 @@ __device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* pa
     for (int w = 1; w < NT / 64; ++w) t += red[w * N + n];
     out[n] = t;
   }
-  __syncthreads();
   return bad == 0;
 }
 
@@ -1357,7 +1362,7 @@ __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-
 //   rr_k = rr_{k-1} - 2 alpha q.r + alpha^2 q.q       symmetric) give beta_{k-1};
 //   x += alpha p, r -= alpha q, z -= alpha w, p = z + beta p   (own rows);
 //   q = A p, gathering neighbour p from the exchange records (z, w, p) of
-//   phase k-1; w = M^-1 q; publish (z, w, p) and the partial sums
+//   phase k-1; w = M^-1 q and z = M^-1 r afresh; publish (z, w, p) and the partial sums
 //   p.q, q.z, q.w, q.r, q.q, r.z, r.r; barrier.
 // The direct r.z and r.r of phase k replace the recurrence values at phase
 // k+1, so recurrence error does not accumulate.  Exchange records and
@@ -1605,12 +1610,19 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     // w = M^-1 q for every row before any record store: a load waited for
     // after a write-through store would also wait for that store (vector
     // memory operations complete in issue order).
+    // z = M^-1 r is applied afresh too (as warp's cg preconditions the
+    // residual in every iteration), so the recurrence z -= alpha w of the
+    // next phase carries one step of rounding instead of accumulating drift
+    // that stalls a tight solve; the record publishes this z.
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (own[k]) {
-        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]};
-        const v3 ww = MRES ? minv_mul(k, R0 + threadIdx.x + NT * k, qq) : mv(Mi[MRES ? 0 : k], qq);
+        const v3 qq = {qv[k][0], qv[k][1], qv[k][2]}, rv = {r[k][0], r[k][1], r[k][2]};
+        const int i = R0 + threadIdx.x + NT * k;
+        const v3 ww = MRES ? minv_mul(k, i, qq) : mv(Mi[MRES ? 0 : k], qq);
+        const v3 zz = MRES ? minv_mul(k, i, rv) : mv(Mi[MRES ? 0 : k], rv);
         w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
+        z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
       }
     }
 #pragma unroll
