@@ -327,6 +327,12 @@ constexpr size_t ASM_LDS_BUDGET = 38 * 1024;
 // register pressure); EL_ANY kernels handle every element kind.
 enum { EL_TET = 1, EL_ANY = 7 };
 
+// Per-group record of the assembly (host-built, one scalar load per
+// workgroup instead of a chain of dependent index loads): first/end row,
+// first/end tet incidence, first/end hex incidence, first/end slot, first/end
+// contribution.
+constexpr int ASM_HDR = 12;
+
 template <int RB, int EL>
 __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __restrict__ groups,
                                                     const float* __restrict__ x, float h, float h2,
@@ -335,17 +341,20 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
   __shared__ int roff[RB + 1], rtb[RB], rhb[RB], rdiag[RB];
   __shared__ float rmass[RB];
   const int blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int r0 = groups[blk], r1 = groups[blk + 1];
+  const int4* hdr = reinterpret_cast<const int4*>(groups + ASM_HDR * (size_t)blk);
+  const int4 g0 = hdr[0], g1 = hdr[1], g2 = hdr[2];
+  const int r0 = g0.x, r1 = g0.y;
   if (r0 >= r1) return;  // uniform over the workgroup (padding groups are empty)
   const int nr = r1 - r0;
-  const int tb = a.tinc_ptr[r0], nt = a.tinc_ptr[r1] - tb;
-  const int hb = a.hinc_ptr[r0], nh = a.hinc_ptr[r1] - hb;
-  const int kb = a.offsets[r0], ke = a.offsets[r1];
-  const int cb = a.slot_ptr[kb], nc = a.slot_ptr[ke] - cb;
+  const int tb = g0.z, nt = g0.w - tb;
+  const int hb = g1.x, nh = g1.y - hb;
+  const int kb = g1.z, ke = g1.w;
+  const int cb = g2.x, nc = g2.y - cb;
   float* st_t = lds;                                                  // [nt][ST_TET]: 4 blocks
   float* st_h = st_t + ST_TET * (size_t)nt;                           // [nh][ST_HEX]: 8 blocks
   int* sptr = reinterpret_cast<int*>(st_h + ST_HEX * (size_t)nh);     // [ke-kb+1] slot -> first contribution
   uint16_t* codes = reinterpret_cast<uint16_t*>(sptr + (ke - kb + 1));  // [nc]
+  uint8_t* srow = reinterpret_cast<uint8_t*>(codes + nc);             // [ke-kb] slot -> local row
   if (threadIdx.x <= nr) roff[threadIdx.x] = a.offsets[r0 + threadIdx.x] - kb;
   if (threadIdx.x < nr) {
     const int i = r0 + threadIdx.x;
@@ -353,6 +362,8 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
     rhb[threadIdx.x] = a.hinc_ptr[i] - hb;
     rdiag[threadIdx.x] = a.diag_rel[i];
     rmass[threadIdx.x] = a.mass[i];
+    const int s0 = a.offsets[i] - kb, s1 = a.offsets[i + 1] - kb;
+    for (int k = s0; k < s1; ++k) srow[k] = (uint8_t)threadIdx.x;
   }
   // Phase 2's slot pointers and contribution codes: the first ASM_PF per
   // thread are loaded into registers now, in flight with phase 1's element
@@ -400,8 +411,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
   for (int j = threadIdx.x + ASM_PF * BLOCK; j < nc; j += BLOCK) codes[j] = a.contrib[cb + j];
   __syncthreads();
   for (int kk = threadIdx.x; kk < ke - kb; kk += BLOCK) {
-    int lr = 0;
-    while (roff[lr + 1] <= kk) ++lr;
+    const int lr = srow[kk];
     const bool diag = (kk - roff[lr]) == rdiag[lr];
     float acc[9];
     const float m = diag ? rmass[lr] : 0.f;
@@ -423,11 +433,34 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
       acc[8] += cst;
     }
     const int tbase = rtb[lr], hbase = rhb[lr];
-    const int j1 = sptr[kk + 1];
-    for (int j = sptr[kk]; j < j1; ++j) {
-      const uint32_t code = codes[j];
+    auto src_of = [&](uint32_t code) -> const float* {
       const uint32_t t = (code >> 3) & 4095u, b = code & 7u;
-      const float* src = (code & 0x8000u) ? st_h + ST_HEX * (hbase + t) + 9 * b : st_t + ST_TET * (tbase + t) + 9 * b;
+      return (code & 0x8000u) ? st_h + ST_HEX * (hbase + t) + 9 * b : st_t + ST_TET * (tbase + t) + 9 * b;
+    };
+    // Contributions in list order, four at a time: their codes and blocks are
+    // read together (independent LDS reads), the sums stay sequential.
+    const int j1 = sptr[kk + 1];
+    int j = sptr[kk];
+    for (; j + 4 <= j1; j += 4) {
+      const float* s0 = src_of(codes[j]);
+      const float* s1 = src_of(codes[j + 1]);
+      const float* s2 = src_of(codes[j + 2]);
+      const float* s3 = src_of(codes[j + 3]);
+      float v[4][9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        v[0][c] = s0[c];
+        v[1][c] = s1[c];
+        v[2][c] = s2[c];
+        v[3][c] = s3[c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[c] += v[u][c];
+    }
+    for (; j < j1; ++j) {
+      const float* src = src_of(codes[j]);
 #pragma unroll
       for (int c = 0; c < 9; ++c) acc[c] += src[c];
     }
@@ -2003,7 +2036,7 @@ struct fem_system {
   fem_params prm{};
   int elem_g = 4, asm_rb = 16, cg_wg_request = 0;
   size_t asm_lds = 0;
-  int* d_asm_groups = nullptr;  // [asm_ngroups + 1] first row of each assembly workgroup
+  int* d_asm_groups = nullptr;  // [asm_ngroups][ASM_HDR] record of each assembly workgroup
   int asm_ngroups = 0;
   int asm_cap = 0;  // fixed assembly group count of a reserved topology pool (0: none)
   int max_resident = 0;
@@ -2242,7 +2275,28 @@ struct fem_system {
     const size_t nh = (size_t)(pt.hinc_ptr[r1] - pt.hinc_ptr[r0]);
     const size_t ns = (size_t)(pt.offsets[r1] - pt.offsets[r0]);
     const size_t nc = (size_t)(pt.slot_ptr[pt.offsets[r1]] - pt.slot_ptr[pt.offsets[r0]]);
-    return ((size_t)ST_TET * nt + (size_t)ST_HEX * nh + ns + 1) * sizeof(float) + nc * sizeof(uint16_t);
+    return ((size_t)ST_TET * nt + (size_t)ST_HEX * nh + ns + 1) * sizeof(float) + nc * sizeof(uint16_t) + ns;
+  }
+
+  // Group records of the assembly (ASM_HDR ints per group, padding groups
+  // empty at row P) for the row boundaries g.
+  std::vector<int32_t> assemble_headers(const std::vector<int32_t>& g, size_t ngroups) const {
+    std::vector<int32_t> hd(ASM_HDR * ngroups, 0);
+    for (size_t k = 0; k < ngroups; ++k) {
+      const int r0 = k + 1 < g.size() ? g[k] : P, r1 = k + 1 < g.size() ? g[k + 1] : P;
+      int32_t* o = &hd[ASM_HDR * k];
+      o[0] = r0;
+      o[1] = r1;
+      o[2] = pt.tinc_ptr[r0];
+      o[3] = pt.tinc_ptr[r1];
+      o[4] = pt.hinc_ptr[r0];
+      o[5] = pt.hinc_ptr[r1];
+      o[6] = (int32_t)pt.offsets[r0];
+      o[7] = (int32_t)pt.offsets[r1];
+      o[8] = pt.slot_ptr[pt.offsets[r0]];
+      o[9] = pt.slot_ptr[pt.offsets[r1]];
+    }
+    return hd;
   }
 
   // Row groups of the assembly: consecutive rows packed greedily, at most rb
@@ -2280,14 +2334,16 @@ struct fem_system {
       if (n > asm_cap || worst > asm_lds)
         return fail(FEM_ERR_UNSUPPORTED, "the edited topology exceeds the reserved assembly launch");
       g.resize(asm_cap + 1, P);
-      HIP_TRY(hipMemcpy(d_asm_groups, g.data(), g.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      const std::vector<int32_t> hd = assemble_headers(g, asm_cap);
+      HIP_TRY(hipMemcpy(d_asm_groups, hd.data(), hd.size() * sizeof(int32_t), hipMemcpyHostToDevice));
       asm_rb = rb;
       return FEM_OK;
     }
     const int padded = n > 8 ? (n + 7) & ~7 : std::max(n, 1);
     while ((int)g.size() < padded + 1) g.push_back(P);
+    const std::vector<int32_t> hd = assemble_headers(g, padded);
     int* dg = nullptr;
-    if (int rc = upload(&dg, g.data(), g.size())) return rc;
+    if (int rc = upload(&dg, hd.data(), hd.size())) return rc;
     drop(d_asm_groups);
     d_asm_groups = dg;
     asm_ngroups = padded;
@@ -2303,7 +2359,7 @@ struct fem_system {
   int reserve_assemble(int rb) {
     const double inc_total = 4.0 * T;
     const double lds_total = (double)(ST_TET * inc_total + pt.offsets[P] + 2.0 * P) * sizeof(float) +
-                             16.0 * T * sizeof(uint16_t);
+                             16.0 * T * sizeof(uint16_t) + (double)pt.offsets[P];
     std::vector<int32_t> g;
     size_t worst = 0;
     assemble_groups(rb, g, worst);
@@ -2312,7 +2368,7 @@ struct fem_system {
     cap = std::max<long>(cap, (long)g.size() - 1);
     cap = (cap + 7) & ~7L;
     int* dg = nullptr;
-    if (int rc = alloc(&dg, (size_t)cap + 1)) return rc;
+    if (int rc = alloc(&dg, (size_t)ASM_HDR * cap)) return rc;
     drop(d_asm_groups);
     d_asm_groups = dg;
     asm_cap = (int)cap;
