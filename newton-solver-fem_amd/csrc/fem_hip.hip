@@ -964,7 +964,10 @@ __device__ __forceinline__ bool spin_until(const unsigned* p, unsigned target, i
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f3v __attribute__((ext_vector_type(3), aligned(4)));
+// Four floats at 4-byte alignment (exchange records are packed at 36 bytes).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 __device__ __forceinline__ f4v ld_shared(const f4v* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ f4u ld_shared_u(const f4u* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float ld_shared(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ double ld_shared(const double* p) { return __builtin_nontemporal_load(p); }
 // The stores carry no memory clobber, so that independent loads (inverse
@@ -1073,9 +1076,10 @@ constexpr size_t STATIC_SCRATCH_BYTES = size_t(32) << 20;
 static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
 
 constexpr int CG_J = 8;  // blocks per thread per tile (a tile is CG_J * NT blocks)
-// Exchange record of one row: z(3) w(3) p(3) + pad, 48 bytes (three 16-byte
-// aligned loads per gathered column).
-constexpr int XW = 12;
+// Exchange record of one row: z(3) w(3) p(3), packed at 36 bytes (two 16-byte
+// loads and one 4-byte load per gathered column; the halo gather moves about
+// 2.5 records per row every iteration, so no padding is carried).
+constexpr int XW = 9;
 
 // q[k] = sum_j A_ij u_j for the thread's own rows i = R0 + t + NT k.
 // Row-aligned tiles of <= (CG_J * NT) blocks.  Per tile, the operand u of every
@@ -1140,8 +1144,8 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
         if (PROLOGUE) {
           u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, c) * s;
         } else {
-          const f4v* e = reinterpret_cast<const f4v*>(xw + XW * (size_t)c);
-          const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
+          const f4u* e = reinterpret_cast<const f4u*>(xw + XW * (size_t)c);
+          const f4u e0 = ld_shared_u(e), e1 = ld_shared_u(e + 1);
           const float p2 = ld_shared(xw + XW * (size_t)c + 8);
           u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
                fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};
@@ -1266,8 +1270,8 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
         u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, (int)c) * s;
       } else {
         const float* rec = xw + (size_t)(c * (uint32_t)XW);
-        const f4v* e = reinterpret_cast<const f4v*>(rec);
-        const f4v e0 = ld_shared(e), e1 = ld_shared(e + 1);
+        const f4u* e = reinterpret_cast<const f4u*>(rec);
+        const f4u e0 = ld_shared_u(e), e1 = ld_shared_u(e + 1);
         const float p2 = ld_shared(rec + 8);
         u = {fmaf(beta, e1.z, fmaf(-alpha, e0.w, e0.x)), fmaf(beta, e1.w, fmaf(-alpha, e1.x, e0.y)),
              fmaf(beta, p2, fmaf(-alpha, e1.y, e0.z))};

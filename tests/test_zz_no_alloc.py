@@ -82,7 +82,7 @@ def test_static_scratch_solve_equals_library_scratch_solve():
 
 def test_static_scratch_rejects_systems_beyond_its_size():
     print("case static-scratch limit", flush=True)
-    n = 360_000  # 96 bytes of records per row: beyond 32 MiB
+    n = 480_000  # 72 bytes of exchange records per row: beyond 32 MiB
     b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
     pts = np.zeros((n, 3))
     pts[:, 0] = np.arange(n) * 1e-3
