@@ -141,12 +141,14 @@ int fem_bind_gravity(fem_system* sys, const float* gravity_dev);
  * element incidences fit one pass of the workgroup), and the persistent CG
  * workgroup count (0 = automatic). */
 int fem_set_launch(fem_system* sys, int32_t elem_lanes, int32_t assemble_rows, int32_t cg_workgroups);
-/* Current launch shapes; cg_launch_groups > 1 when the instances' CG
- * workgroups do not all fit on the device at once and the solve runs as
- * consecutive launches over groups of whole instances; cg_variant is
+/* Current launch shapes.  cg_workgroups counts the CG chunks (one workgroup
+ * solves a chunk); the solve is one persistent launch of
+ * cg_launch_workgroups workgroups: one per chunk when they all fit on the
+ * device at once, else as many as fit, taking chunks of whole instances from
+ * a device queue as their instances converge.  cg_variant is
  * FEM_CG_STREAMED or FEM_CG_RESIDENT. */
 int fem_get_launch(const fem_system* sys, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups, int32_t* cg_variant);
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_workgroups, int32_t* cg_variant);
 
 /* Persistent CG kernel variant (the same iteration as warp.optim.linear.cg's
  * block-Jacobi PCG, linear.py cg, as called by SolverFEM._solve_linear_system

@@ -133,7 +133,8 @@ __global__ void k_prepare(const int* __restrict__ fixed, int nfixed, const float
 // Per-solve reset of the CG barrier counters, XCC epochs and registration
 // and of the device error word (the flat barrier's generation word keeps
 // counting across launches).
-// Clears the per-chunk publication flags of the persistent CG and the error word.
+// Clears the per-chunk publication flags of the persistent CG, the chunk
+// queue word after them and the error word.
 __global__ void k_cg_reset(unsigned* flags, int nflags, int* err) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nflags) flags[t] = 0u;
@@ -916,13 +917,14 @@ struct CgArgs {
   const unsigned long long* res_mask;
   const int2* res_row;
   const unsigned* rhs_max;
-  int chunk_base;     // first chunk of this launch group
-  int nchunks_total;  // chunks of all groups (stride of the double-buffered partials)
+  int chunk_base;          // first chunk of this launch group
+  int nchunks_total;       // chunks of all groups (stride of the double-buffered partials)
+  const int* chunk_order;  // queued launch: chunks in the order the queue hands them out (else null)
   float tolerance;
   int maxiter;
   int early_exit;
   double* part;
-  unsigned* flags;  // [nchunks_total] last phase each chunk has published
+  unsigned* flags;  // [nchunks_total] last phase each chunk has published, then the chunk queue word
   int* stats;
   int* err;
   long spin_limit;  // sleep-polls a wait may take before it gives up
@@ -1075,6 +1077,21 @@ __device__ __forceinline__ bool wait_sum(const unsigned* flags, const double* pa
 constexpr size_t STATIC_SCRATCH_BYTES = size_t(32) << 20;
 static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
 
+// From here to the end of the CG kernel no multiply-add is fused implicitly:
+// every fused one is an explicit fmaf, so each kernel shape (rows per
+// thread, queued or not) rounds every chunk alike and a chunk's solve does
+// not depend on the launch it runs in.
+#pragma clang fp contract(off)
+__device__ __forceinline__ float dot_cg(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ v3 mv_cg(const float m[9], v3 x) {
+  return {fmaf(m[2], x.z, fmaf(m[1], x.y, m[0] * x.x)), fmaf(m[5], x.z, fmaf(m[4], x.y, m[3] * x.x)),
+          fmaf(m[8], x.z, fmaf(m[7], x.y, m[6] * x.x))};
+}
+__device__ __forceinline__ v3 scale_cg(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 sub_cg(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 add_cg(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 fma_cg(v3 a, float s, v3 b) { return {fmaf(a.x, s, b.x), fmaf(a.y, s, b.y), fmaf(a.z, s, b.z)}; }
+
 constexpr int CG_J = 8;  // blocks per thread per tile (a tile is CG_J * NT blocks)
 // Exchange record of one row: z(3) w(3) p(3), packed at 36 bytes (two 16-byte
 // loads and one 4-byte load per gathered column; the halo gather moves about
@@ -1142,7 +1159,7 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
         const int c = ucol[j];
         v3 u;
         if (PROLOGUE) {
-          u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, c) * s;
+          u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : scale_cg(ld3(a.dv, c), s);
         } else {
           const f4u* e = reinterpret_cast<const f4u*>(xw + XW * (size_t)c);
           const f4u e0 = ld_shared_u(e), e1 = ld_shared_u(e + 1);
@@ -1164,9 +1181,9 @@ __device__ __forceinline__ void cg_spmv(const CgArgs& a, int R0, int nr, int t0,
         const float* uu = u_l + 3 * li[j];
         const float ux = uu[0], uy = uu[1], uz = uu[2];
         float* o = part_l + 3 * (threadIdx.x + NT * j);
-        o[0] = bv[j][0] * ux + bv[j][1] * uy + bv[j][2] * uz;
-        o[1] = bv[j][3] * ux + bv[j][4] * uy + bv[j][5] * uz;
-        o[2] = bv[j][6] * ux + bv[j][7] * uy + bv[j][8] * uz;
+        o[0] = fmaf(bv[j][2], uz, fmaf(bv[j][1], uy, bv[j][0] * ux));
+        o[1] = fmaf(bv[j][5], uz, fmaf(bv[j][4], uy, bv[j][3] * ux));
+        o[2] = fmaf(bv[j][8], uz, fmaf(bv[j][7], uy, bv[j][6] * ux));
       }
     }
     __syncthreads();
@@ -1240,7 +1257,7 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
     if (own[k]) {
       f4v u = f4v{p[k][0], p[k][1], p[k][2], 0.f};
       if (PROLOGUE) {
-        const v3 d = fx[k] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, rt.R0 + (int)threadIdx.x + NT * k) * s;
+        const v3 d = fx[k] ? v3{0.f, 0.f, 0.f} : scale_cg(ld3(a.dv, rt.R0 + (int)threadIdx.x + NT * k), s);
         u = f4v{d.x, d.y, d.z, 0.f};
       }
       u_l[rt.own_lo + threadIdx.x + NT * k] = u;
@@ -1267,7 +1284,7 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
       const uint32_t c = live ? (uint32_t)hc[i] : 0u;
       v3 u;
       if (PROLOGUE) {
-        u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : ld3(a.dv, (int)c) * s;
+        u = a.row_fixed[c] ? v3{0.f, 0.f, 0.f} : scale_cg(ld3(a.dv, (int)c), s);
       } else {
         const float* rec = xw + (size_t)(c * (uint32_t)XW);
         const f4u* e = reinterpret_cast<const f4u*>(rec);
@@ -1325,9 +1342,9 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
   asm volatile("" : "+v"(mlo), "+v"(mhi), "+v"(run), "+v"(cnt));
   auto block = [&](int j, const f4v& u, float a0, float a1, float a2, float a3, float a4, float a5, float a6,
                    float a7, float a8) {
-    acc.x += a0 * u.x + a1 * u.y + a2 * u.z;
-    acc.y += a3 * u.x + a4 * u.y + a5 * u.z;
-    acc.z += a6 * u.x + a7 * u.y + a8 * u.z;
+    acc.x = fmaf(a2, u.z, fmaf(a1, u.y, fmaf(a0, u.x, acc.x)));
+    acc.y = fmaf(a5, u.z, fmaf(a4, u.y, fmaf(a3, u.x, acc.y)));
+    acc.z = fmaf(a8, u.z, fmaf(a7, u.y, fmaf(a6, u.x, acc.z)));
     seg_l[j < cnt ? run : RES_SEG] = acc;
     const bool e = ((((uint64_t)mhi << 32) | mlo) >> j) & 1u;
     run += e;
@@ -1409,12 +1426,16 @@ __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-
 // RES selects the resident SpMV (res_spmv: blocks kept on chip, RES_NT
 // threads, RES_K rows per thread) instead of the streamed tiles (cg_spmv).
 template <int K, int NT, bool RES>
-__global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
-  static_assert(!RES || (NT == RES_NT && K == RES_K), "resident CG shape");
-  constexpr size_t SM_STREAM = sizeof(float) * (6 * CG_J * NT + K * NT + 1);
-  constexpr size_t SM_RES = sizeof(f4v) * (RES_UC * NT + RES_SEG + 1) + sizeof(float) * 9 * RES_JL * NT +
-                            sizeof(uint16_t) * RES_JL * NT;
-  __shared__ __align__(16) unsigned char smem[RES ? SM_RES : SM_STREAM];
+constexpr size_t cg_smem_bytes() {
+  return RES ? sizeof(f4v) * (RES_UC * NT + RES_SEG + 1) + sizeof(float) * 9 * RES_JL * NT +
+                   sizeof(uint16_t) * RES_JL * NT
+             : sizeof(float) * (6 * CG_J * NT + K * NT + 1);
+}
+
+// The solve of chunk c by one workgroup (smem: cg_smem_bytes, red: the
+// workgroup reduction slots).
+template <int K, int NT, bool RES>
+__device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned char* smem, double* red) {
   // streamed tiles
   float* part_l = reinterpret_cast<float*>(smem);
   float* u_l = part_l + 3 * CG_J * NT;
@@ -1424,8 +1445,6 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
   f4v* seg_l = ures_l + RES_UC * NT;
   float* mat_l = reinterpret_cast<float*>(seg_l + RES_SEG + 1);
   uint16_t* lidx_l = reinterpret_cast<uint16_t*>(mat_l + 9 * RES_JL * NT);
-  __shared__ double red[(NT / 64) * NPART];
-  const int c = a.chunk_base + xcd_swizzle(blockIdx.x, gridDim.x);
   const int inst = a.chunk_inst[c];
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
@@ -1459,11 +1478,11 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     }
   }
   auto minv_mul = [&](int k, int i, v3 v) -> v3 {
-    if (MRES) return mv(Mk[MRES ? k : 0], v);
+    if (MRES) return mv_cg(Mk[MRES ? k : 0], v);
     float m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = a.minv[9 * (size_t)i + e];
-    return mv(m, v);
+    return mv_cg(m, v);
   };
   int ucol[CG_J];
   // Resident blocks, their chunk-local column numbers (two per word), the
@@ -1556,17 +1575,17 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     for (int d = 0; d < 3; ++d) x[k][d] = r[k][d] = z[k][d] = p[k][d] = w[k][d] = 0.f;
     if (own[k]) {
       const int i = R0 + threadIdx.x + NT * k;
-      const v3 b = ld3(a.rhs, i) * s;
-      const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : ld3(a.dv, i) * s;
-      const v3 rr0 = b - v3{qv[k][0], qv[k][1], qv[k][2]};
+      const v3 b = scale_cg(ld3(a.rhs, i), s);
+      const v3 x0 = (zero_start || fx[k]) ? v3{0.f, 0.f, 0.f} : scale_cg(ld3(a.dv, i), s);
+      const v3 rr0 = sub_cg(b, v3{qv[k][0], qv[k][1], qv[k][2]});
       const v3 zz = minv_mul(k, i, rr0);
       x[k][0] = x0.x; x[k][1] = x0.y; x[k][2] = x0.z;
       r[k][0] = rr0.x; r[k][1] = rr0.y; r[k][2] = rr0.z;
       z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
       st_xw(a.xw, i, z[k], w[k], p[k]);
-      pb[0] += dot(b, b);
-      pb[1] += dot(rr0, rr0);
-      pb[2] += dot(rr0, zz);
+      pb[0] += dot_cg(b, b);
+      pb[1] += dot_cg(rr0, rr0);
+      pb[2] += dot_cg(rr0, zz);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) qv[k][d] = 0.f;
@@ -1656,8 +1675,8 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
       if (own[k]) {
         const v3 qq = {qv[k][0], qv[k][1], qv[k][2]}, rv = {r[k][0], r[k][1], r[k][2]};
         const int i = R0 + threadIdx.x + NT * k;
-        const v3 ww = MRES ? minv_mul(k, i, qq) : mv(Mi[MRES ? 0 : k], qq);
-        const v3 zz = MRES ? minv_mul(k, i, rv) : mv(Mi[MRES ? 0 : k], rv);
+        const v3 ww = MRES ? minv_mul(k, i, qq) : mv_cg(Mi[MRES ? 0 : k], qq);
+        const v3 zz = MRES ? minv_mul(k, i, rv) : mv_cg(Mi[MRES ? 0 : k], rv);
         w[k][0] = ww.x; w[k][1] = ww.y; w[k][2] = ww.z;
         z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
       }
@@ -1670,13 +1689,13 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
         st_xw(xw_next, i, z[k], w[k], p[k]);
         const v3 pv = {p[k][0], p[k][1], p[k][2]}, zv = {z[k][0], z[k][1], z[k][2]},
                  rv = {r[k][0], r[k][1], r[k][2]};
-        pp[0] += dot(pv, qq);
-        pp[1] += dot(qq, zv);
-        pp[2] += dot(qq, ww);
-        pp[3] += dot(qq, rv);
-        pp[4] += dot(qq, qq);
-        pp[5] += dot(rv, zv);
-        pp[6] += dot(rv, rv);
+        pp[0] += dot_cg(pv, qq);
+        pp[1] += dot_cg(qq, zv);
+        pp[2] += dot_cg(qq, ww);
+        pp[3] += dot_cg(qq, rv);
+        pp[4] += dot_cg(qq, qq);
+        pp[5] += dot_cg(rv, zv);
+        pp[6] += dot_cg(rv, rv);
       }
     }
     {
@@ -1704,15 +1723,15 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (own[k] && a.x_out) {
-      const v3 xo = failed ? v3{nan, nan, nan} : v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
+      const v3 xo = failed ? v3{nan, nan, nan} : scale_cg(v3{x[k][0], x[k][1], x[k][2]}, 1.f / s);
       st3(a.x_out, R0 + threadIdx.x + NT * k, xo);
     } else if (own[k]) {
       const int i = R0 + threadIdx.x + NT * k;
-      const v3 dvi = failed ? v3{nan, nan, nan} : v3{x[k][0], x[k][1], x[k][2]} * (1.f / s);
+      const v3 dvi = failed ? v3{nan, nan, nan} : scale_cg(v3{x[k][0], x[k][1], x[k][2]}, 1.f / s);
       st3(a.dv, i, dvi);
       if (a.q_out) {
-        v3 vel = ld3(a.qd_in, i) + dvi;
-        v3 qo = ld3(a.q_in, i) + vel * a.h;
+        v3 vel = add_cg(ld3(a.qd_in, i), dvi);
+        v3 qo = fma_cg(vel, a.h, ld3(a.q_in, i));
         if (fx[k] && !failed) {
           qo = ld3(a.anchor, i);
           vel = v3{0.f, 0.f, 0.f};
@@ -1728,6 +1747,40 @@ __global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
     a.stats[4 * inst + 2] = __float_as_int((float)atol_sq);
   }
 }
+
+// One persistent launch.  Without QUEUE every chunk has a workgroup of its
+// own: workgroup b solves chunk xcd_swizzle(b, n) (an XCC holds consecutive
+// chunks).  With QUEUE the launch holds as many workgroups as fit at once and
+// they take chunks from a queue (the word after the chunk flags), in the
+// order chunk_order gives: whole instances, largest first.  A workgroup
+// moves to the next chunk when its instance has converged.  The order keeps
+// the queue from deadlocking: every instance before the one at the queue's
+// head has all its chunks running, so it converges and frees workgroups for
+// the rest of the head's chunks (an instance never needs more workgroups
+// than fit, which the planner checks).  The queue loop keeps a few more
+// values live, so it is a kernel of its own rather than a branch.
+template <int K, int NT, bool RES, bool QUEUE>
+__global__ __launch_bounds__(NT) void k_cg(CgArgs a) {
+  static_assert(!RES || (NT == RES_NT && K == RES_K), "resident CG shape");
+  __shared__ __align__(16) unsigned char smem[cg_smem_bytes<K, NT, RES>()];
+  __shared__ double red[(NT / 64) * NPART];
+  if constexpr (!QUEUE) {
+    cg_chunk<K, NT, RES>(a, a.chunk_base + xcd_swizzle(blockIdx.x, gridDim.x), smem, red);
+  } else {
+    __shared__ int next_l;
+    for (int item = blockIdx.x; item < a.nchunks_total;) {
+      cg_chunk<K, NT, RES>(a, a.chunk_order[item], smem, red);
+      __syncthreads();  // this chunk's shared memory and the previous next_l are no longer read
+      if (threadIdx.x == 0)
+        next_l = (int)gridDim.x + (int)__hip_atomic_fetch_add(a.flags + a.nchunks_total, 1u, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      item = next_l;
+    }
+  }
+}
+
+#pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------
 // Prepared-step actions (PreparedFEMStep, prepared_step.py:64-302).
@@ -1963,23 +2016,28 @@ struct LaunchVjpRows {
 };
 
 // Rows per thread K and threads per workgroup NT of the persistent CG
-// (res: the resident variant, RES_K rows and RES_NT threads).
-const void* cg_kernel_ptr(int k, int nt, bool res) {
-  if (res) return (k == RES_K && nt == RES_NT) ? (const void*)k_cg<RES_K, RES_NT, true> : nullptr;
+// (res: the resident variant, RES_K rows and RES_NT threads; queue: the
+// workgroups take chunks from the device queue).
+template <bool Q>
+const void* cg_kernel_ptr_q(int k, int nt, bool res) {
+  if (res) return (k == RES_K && nt == RES_NT) ? (const void*)k_cg<RES_K, RES_NT, true, Q> : nullptr;
   if (nt == 512) {
     switch (k) {
-      case 1: return (const void*)k_cg<1, 512, false>;
-      case 2: return (const void*)k_cg<2, 512, false>;
+      case 1: return (const void*)k_cg<1, 512, false, Q>;
+      case 2: return (const void*)k_cg<2, 512, false, Q>;
     }
     return nullptr;
   }
   switch (k) {
-    case 1: return (const void*)k_cg<1, 256, false>;
-    case 2: return (const void*)k_cg<2, 256, false>;
-    case 4: return (const void*)k_cg<4, 256, false>;
-    case 8: return (const void*)k_cg<8, 256, false>;
+    case 1: return (const void*)k_cg<1, 256, false, Q>;
+    case 2: return (const void*)k_cg<2, 256, false, Q>;
+    case 4: return (const void*)k_cg<4, 256, false, Q>;
+    case 8: return (const void*)k_cg<8, 256, false, Q>;
   }
   return nullptr;
+}
+const void* cg_kernel_ptr(int k, int nt, bool res, bool queue) {
+  return queue ? cg_kernel_ptr_q<true>(k, nt, res) : cg_kernel_ptr_q<false>(k, nt, res);
 }
 
 }  // namespace
@@ -2032,7 +2090,10 @@ struct fem_system {
   size_t cap_tile_cols = 0, cap_lidx = 0;  // storage of the current CG plan
   bool cg_res = false;
   int cg_variant_request = 0;  // 0 any, 1 streamed, 2 resident
-  std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups
+  std::vector<int32_t> cg_groups;  // chunk boundaries of the CG launch groups (one group)
+  int* d_chunk_order = nullptr;    // queued CG launch: order of the chunk queue (null: one workgroup per chunk)
+  int cg_grid = 0;                 // workgroups of the CG launch
+  bool cg_queue = false;           // the launch takes chunks from the device queue
   // bound
   int* b_offsets = nullptr;
   int* b_cols = nullptr;
@@ -2645,36 +2706,45 @@ struct fem_system {
       return FEM_OK;
     }
     const int k_rows = res ? RES_K : maxrows <= nt ? 1 : maxrows <= 2 * nt ? 2 : maxrows <= 4 * nt ? 4 : 8;
-    const void* kp = cg_kernel_ptr(k_rows, nt, res);
+    const void* kp = cg_kernel_ptr(k_rows, nt, res, false);
     if (!kp) return fail(FEM_ERR_ARG, "unsupported CG rows per thread");
     int nb = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kp, nt, 0));
-    // Launch groups of whole instances, each small enough that all of its
-    // workgroups are co-resident (instances never span two groups).
-    const int cap = nb * ncu;
-    std::vector<int32_t> groups(1, 0);
-    {
-      int in_group = 0;
-      for (int k = 0; k < ninst; ++k) {
-        const int nk = inst_chunks[k + 1] - inst_chunks[k];
-        if (nk > cap) {
-          char buf[160];
-          std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups for one instance, device holds %d", nk,
-                        cap);
-          return fail(FEM_ERR_UNSUPPORTED, buf);
-        }
-        if (in_group + nk > cap) {
-          groups.push_back(inst_chunks[k]);
-          in_group = 0;
-        }
-        in_group += nk;
+    // One persistent launch.  With more chunks than workgroups fit at once
+    // the queue kernel runs with `cap` workgroups (as many of its own as fit)
+    // that take chunks from a queue: whole instances, the most expensive
+    // first (they bound the solve's length), each instance's chunks
+    // consecutive.  An instance's chunks must all be co-resident.
+    int cap = nb * ncu;
+    const bool queue = n_chunks > cap;
+    if (queue) {
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, cg_kernel_ptr(k_rows, nt, res, true), nt, 0));
+      cap = std::min(cap, nb * ncu);
+    }
+    for (int k = 0; k < ninst; ++k) {
+      const int nk = inst_chunks[k + 1] - inst_chunks[k];
+      if (nk > cap) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "CG needs %d co-resident workgroups for one instance, device holds %d", nk,
+                      cap);
+        return fail(FEM_ERR_UNSUPPORTED, buf);
       }
-      groups.push_back(n_chunks);
+    }
+    const std::vector<int32_t> groups = {0, n_chunks};
+    std::vector<int32_t> order;
+    if (queue) {
+      std::vector<int32_t> inst_order(ninst);
+      for (int k = 0; k < ninst; ++k) inst_order[k] = k;
+      std::stable_sort(inst_order.begin(), inst_order.end(), [&](int u, int v) { return cost[u] > cost[v]; });
+      order.reserve(n_chunks);
+      for (const int k : inst_order)
+        for (int c = inst_chunks[k]; c < inst_chunks[k + 1]; ++c) order.push_back(c);
     }
     int *n_rows = nullptr, *n_inst = nullptr, *n_ichunks = nullptr, *n_tiles = nullptr, *n_trows = nullptr,
         *n_tcptr = nullptr, *n_tcols = nullptr, *n_rthr = nullptr, *n_rrow = nullptr;
     uint64_t* n_rmask = nullptr;
     uint16_t* n_lidx = nullptr;
+    int* n_order = nullptr;
     double* n_part = nullptr;
     unsigned* n_flags = nullptr;
     int rc;
@@ -2690,11 +2760,12 @@ struct fem_system {
         (rc = upload(&n_ichunks, inst_chunks.data(), inst_chunks.size())) ||
         (rc = upload(&n_tiles, chunk_tiles.data(), chunk_tiles.size())) ||
         (rc = upload(&n_trows, tile_rows.data(), tile_rows.size())) || (rc = alloc(&n_part, 2 * (size_t)NPART * n_chunks)) ||
-        (rc = alloc(&n_flags, (size_t)n_chunks)))
+        (rc = alloc(&n_flags, (size_t)n_chunks + 1)) || (!order.empty() && (rc = upload(&n_order, order.data(), order.size()))))
       return rc;
     for (void* p : {(void*)d_chunk_rows, (void*)d_chunk_inst, (void*)d_inst_chunks, (void*)d_part,
                     (void*)d_chunk_tiles, (void*)d_tile_rows, (void*)d_tile_cptr, (void*)d_tile_cols,
-                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row, (void*)d_cgflags})
+                    (void*)d_lidx, (void*)d_res_thr, (void*)d_res_mask, (void*)d_res_row, (void*)d_cgflags,
+                    (void*)d_chunk_order})
       drop(p);
     d_chunk_rows = n_rows;
     d_chunk_inst = n_inst;
@@ -2710,6 +2781,9 @@ struct fem_system {
     cg_res = res;
     d_part = n_part;
     d_cgflags = n_flags;
+    d_chunk_order = n_order;
+    cg_queue = !order.empty();
+    cg_grid = cg_queue ? cap : n_chunks;
     nchunks = n_chunks;
     h_chunk_rows = chunk_rows;
     cap_tile_cols = tile_cols.size();
@@ -2949,10 +3023,10 @@ int fem_set_launch(fem_system* s, int32_t elem_lanes, int32_t assemble_rows, int
 }
 
 int fem_get_launch(const fem_system* s, int32_t* elem_lanes, int32_t* assemble_rows, int32_t* cg_workgroups,
-                   int32_t* cg_rows_per_thread, int32_t* cg_launch_groups, int32_t* cg_variant) {
+                   int32_t* cg_rows_per_thread, int32_t* cg_launch_workgroups, int32_t* cg_variant) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   if (cg_variant) *cg_variant = s->cg_res ? FEM_CG_RESIDENT : FEM_CG_STREAMED;
-  if (cg_launch_groups) *cg_launch_groups = s->cg_groups.empty() ? 0 : (int32_t)s->cg_groups.size() - 1;
+  if (cg_launch_workgroups) *cg_launch_workgroups = s->cg_grid;
   if (elem_lanes) *elem_lanes = s->elem_g;
   if (assemble_rows) *assemble_rows = s->asm_rb;
   if (cg_workgroups) *cg_workgroups = s->nchunks;
@@ -3074,30 +3148,30 @@ CgArgs cg_args(const fem_system* s) {
   return ca;
 }
 
-// Barrier reset + one persistent launch per group of whole instances.
+// Flag and queue reset + the persistent launch.
 int launch_cg(const fem_system* s, CgArgs ca, hipStream_t st) {
-  const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt, s->cg_res);
+  const void* kp = cg_kernel_ptr(s->cg_k, s->cg_nt, s->cg_res, s->cg_queue);
   if (!kp) return fail(FEM_ERR_ARG, "unsupported CG launch shape");
-  hipLaunchKernelGGL(k_cg_reset, dim3((s->nchunks + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ca.flags,
-                     s->nchunks, s->d_err);
+  hipLaunchKernelGGL(k_cg_reset, dim3((s->nchunks + 1 + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ca.flags,
+                     s->nchunks + 1, s->d_err);
   HIP_TRY(hipGetLastError());
-  for (size_t g = 0; g + 1 < s->cg_groups.size(); ++g) {
-    ca.chunk_base = s->cg_groups[g];
-    void* args[] = {&ca};
-    HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_groups[g + 1] - s->cg_groups[g]), dim3(s->cg_nt), args, 0, st));
-  }
+  ca.chunk_base = 0;
+  ca.chunk_order = s->d_chunk_order;
+  void* args[] = {&ca};
+  HIP_TRY(hipLaunchKernel(kp, dim3(s->cg_grid), dim3(s->cg_nt), args, 0, st));
   return FEM_OK;
 }
 }  // namespace
 
 namespace {
 // Bytes of static scratch a solve of `s` needs: partial sums (2 x NPART
-// doubles per chunk), then the publication flags (one word per chunk), then
+// doubles per chunk), then the publication flags (one word per chunk and the
+// chunk queue word), then
 // the exchange records (2 x XW floats per row), each part 256-byte aligned.
 size_t static_scratch_layout(const fem_system* s, size_t* flags_off, size_t* xw_off) {
   auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
   const size_t part = up(sizeof(double) * 2 * NPART * (size_t)s->nchunks);
-  const size_t flags = up(sizeof(unsigned) * (size_t)s->nchunks);
+  const size_t flags = up(sizeof(unsigned) * ((size_t)s->nchunks + 1));
   *flags_off = part;
   *xw_off = part + flags;
   return part + flags + sizeof(float) * 2 * XW * (size_t)s->P;

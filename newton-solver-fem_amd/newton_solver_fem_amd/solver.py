@@ -526,8 +526,12 @@ class SolverFEM:
     def launch_config(self) -> dict:
         v = [ctypes.c_int32() for _ in range(6)]
         _lib.check(self._lib.fem_get_launch(self._sys, *[ctypes.byref(x) for x in v]), "fem_get_launch")
-        keys = ("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread", "cg_launch_groups")
+        keys = ("elem_lanes", "assemble_rows", "cg_workgroups", "cg_rows_per_thread", "cg_launch_workgroups")
         out = dict(zip(keys, (x.value for x in v[:5])))
+        # One persistent CG launch; fewer workgroups than chunks means they
+        # take the chunks of whole instances from a device queue.
+        out["cg_launch_groups"] = 1
+        out["cg_queued"] = out["cg_launch_workgroups"] < out["cg_workgroups"]
         out["cg_variant"] = {1: "streamed", 2: "resident"}[v[5].value]
         return out
 

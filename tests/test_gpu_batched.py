@@ -2,8 +2,9 @@
 
 A batch packs uneven Tet4 meshes back to back (``instance_offsets``); every
 instance runs its own PCG (own dots, alpha/beta, residual exit and
-statistics) inside one persistent launch, or inside consecutive launches over
-groups of whole instances when its workgroups do not fit at once.  With the
+statistics) inside one persistent launch; when the chunks outnumber the
+workgroups that fit at once, the workgroups take whole instances' chunks from
+a device queue.  With the
 same chunking per instance, the batch reproduces the meshes stepped one by
 one (looped) bit for bit.
 """
@@ -90,27 +91,38 @@ def test_batched_bit_identical_to_looped(batch, variant):
         np.testing.assert_array_equal(np.asarray(t.solve_stats()["iterations"]), np.asarray(it_b))
 
 
-def test_batched_launch_groups(batch):
-    """More workgroups than fit at once: the solve runs as several launches over whole instances.
+def test_batched_chunk_queue(batch):
+    """More chunks than workgroups fit at once: one launch whose workgroups take chunks from a device queue.
 
     Chunking differs from the default launch, so the fp32 reduction order
     differs; both solves are converged to tol 1e-6, and two solutions with
     relative residual <= tol differ by at most ~2 tol cond(M^-1 A) (< 1e-3 here).
+    Which workgroup takes which chunk changes from run to run, the chunks'
+    arithmetic does not: two queued solvers agree bit for bit.
     """
     scenes, merged, offs = batch
-    print("case launch groups", flush=True)
+    print("case chunk queue", flush=True)
     dt = 1 / 600
     kw = dict(dt=dt, linear_solver_tolerance=1e-6, linear_solver_maxiter=400, instance_offsets=offs)
     model = build_tet_model(merged, device=DEV)
     a = solver_for(model, **kw)
     b = solver_for(model, **kw)
-    b.set_launch(0, 0, 1100)
-    assert b.launch_config()["cg_launch_groups"] >= 2
-    qa, qb = np_(rollout(a, model, 1, dt).particle_q), np_(rollout(b, model, 1, dt).particle_q)
+    c = solver_for(model, **kw)
+    for s in (b, c):
+        s.set_launch(0, 0, 1100)
+        cfg = s.launch_config()
+        assert cfg["cg_launch_groups"] == 1 and cfg["cg_queued"]
+        assert cfg["cg_launch_workgroups"] < cfg["cg_workgroups"]
+    assert not a.launch_config()["cg_queued"]
+    qa, qb = np_(rollout(a, model, 2, dt).particle_q), np_(rollout(b, model, 2, dt).particle_q)
+    qc = np_(rollout(c, model, 2, dt).particle_q)
     assert rel(np_(b.dv), np_(a.dv)) < 1e-3
     assert rel(qb, qa) < 1e-6
+    np.testing.assert_array_equal(np_(c.dv), np_(b.dv))
+    np.testing.assert_array_equal(qc, qb)
     sa, sb = a.solve_stats(), b.solve_stats()
     assert len(sb["iterations"]) == len(scenes)
+    assert sb["iterations"] == c.solve_stats()["iterations"]
     assert (np.asarray(sb["residual_sq"]) <= np.asarray(sb["atol_sq"]) * 1.0001).all()
 
 
