@@ -199,8 +199,9 @@ int fem_step_solve(fem_system* sys, const float* q_in_dev, const float* qd_in_de
  * counterpart in the reference.  The persistent CG's partial sums,
  * publication flags and exchange records live in a static device array of
  * 32 MiB per GPU (held until the process ends) instead of the system's own
- * buffers: 72 bytes per row plus 132 bytes per CG chunk, so up to about
- * 460,000 rows; a larger system returns FEM_ERR_UNSUPPORTED before any
+ * buffers: 72 bytes per row plus 132 bytes per CG chunk (and the chunk
+ * queue word), so up to about 460,000 rows; a larger system returns
+ * FEM_ERR_UNSUPPORTED before any
  * launch.  The call allocates and frees nothing; the flags it needs zeroed
  * are cleared by a kernel on the stream.  Every call on a GPU shares that
  * GPU's array: calls that overlap on one GPU (other streams, other systems)
