@@ -55,8 +55,10 @@ def build_large(device):
 
 def cpu_baseline(scene, model, substeps: int):
     """FP32 CPU restatement of the reference substep, timed on one host thread
-    and on all host threads (up to 16; the CG loops run in parallel, the element
-    loops serially); the faster of the two is the reported baseline."""
+    and on all host threads (up to 16: the CG loops, element tangent blocks,
+    block inverses and predictor run in parallel; the triplet accumulation and
+    the element force scatter stay serial); the faster of the two is the
+    reported baseline."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
 
@@ -92,7 +94,56 @@ def cpu_baseline(scene, model, substeps: int):
         "threads": threads,
         "sample": f"{substeps} substeps (tangent refresh + forces/predictor + 50 PCG iterations) of the same "
                   f"1,008,000-tet workload, fp32 C++ restatement: one thread {el1:.1f} s; {threads} threads "
-                  f"(CG loops in parallel, element loops serial) {elm:.1f} s",
+                  f"(CG loops, element tangent blocks, block inverses and predictor in parallel; triplet "
+                  f"accumulation and element force scatter serial) {elm:.1f} s",
+    }
+
+
+def cpu_baseline_adjoint_check(ctx, directions: int = 3, eps: float = 1e-4):
+    """Part of the CPU-baseline leg: the FP64 oracle as the checker of the
+    bench's GPU activation gradient.  Central finite differences of the
+    oracle's 60-substep rollout loss along random directions d (seed 4),
+    against the adjoint's g.d; also the oracle loss against the GPU loss."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+
+    import oracle
+
+    sc, host, gravity = ctx["scene"], ctx["host"], ctx["gravity"]
+    rest = host["particle_q"].astype(np.float64)
+    tip, target, steps = ctx["tip"], ctx["target"], ctx["steps"]
+    o = oracle.OracleSolver(host, bits=64, dt=DT, gravity=gravity, fixed=sc.fixed)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    o.set_threads(threads)
+    mat = sc.materials.astype(np.float64)
+
+    def loss(acts):
+        m = mat.copy()
+        m[:, 0] = mat[:, 0] * (1.0 + acts)
+        o.set_tet_materials(m)
+        o.refresh_tangent(rest)
+        q, v = rest.copy(), np.zeros_like(rest)
+        for _ in range(steps):
+            q, v, _ = o.diff_step(q, v)
+        return float(np.sum((q[tip].mean(0) - target) ** 2))
+
+    t0 = time.perf_counter()
+    a0 = sc.activations.astype(np.float64)
+    base = loss(a0)
+    rng = np.random.default_rng(4)
+    errs = []
+    for _ in range(directions):
+        d = rng.normal(size=len(a0))
+        fd = (loss(a0 + eps * d) - loss(a0 - eps * d)) / (2 * eps)
+        errs.append(abs(float(np.dot(ctx["grad"], d)) - fd) / abs(fd))
+    return {
+        "method": f"FP64 oracle rollout, central differences (eps {eps:g}) along {directions} random directions "
+                  f"vs the adjoint's gradient . direction; {threads} host threads, {time.perf_counter() - t0:.1f} s",
+        "loss_gpu": ctx["loss"],
+        "loss_oracle": base,
+        "loss_rel_err": abs(ctx["loss"] - base) / base,
+        "gradient_rel_err": errs,
+        "gradient_rel_err_max": max(errs),
     }
 
 
@@ -160,10 +211,16 @@ def showcase_model(dev):
     return b.finalize(dev)
 
 
-def bench_showcase(dev, frames: int = 100, warmup: int = 30):
+def bench_showcase(dev, frames: int = 40, rest_frames: int = 100, rest_warmup: int = 30):
     """Frames of the showcase-sized scene: one tangent refresh + 10 substeps (h = 1/600,
     block-Jacobi PCG, tol 1e-2, <= 50 iterations) per frame, captured in a HIP graph,
-    against the reference's published 46.758 ms/frame (NVIDIA GB10, showcase.json)."""
+    against the reference's published 46.758 ms/frame (NVIDIA GB10, showcase.json).
+
+    The bodies start 0.4 m above the ground at rest: the first `frames` frames
+    (0.67 s: the fall, the impact and the rebound) are the moving-scene number
+    that the ratio uses; `rest_frames` frames after another `rest_warmup`
+    are reported apart as the resting scene.  CG iterations per substep come
+    from an untimed eager pass over the same frames."""
     import torch
 
     from newton_solver_fem_amd.solver import SolverFEM
@@ -183,6 +240,7 @@ def bench_showcase(dev, frames: int = 100, warmup: int = 30):
             solver.step(a, b, None, None, dt)
             a, b = b, a  # an even substep count ends in s0
 
+    q0, qd0 = s0.particle_q.clone(), s0.particle_qd.clone()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -191,16 +249,51 @@ def bench_showcase(dev, frames: int = 100, warmup: int = 30):
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         frame()
-    for _ in range(warmup):
-        g.replay()
-    torch.cuda.synchronize()
+
+    def restart():
+        s0.particle_q.copy_(q0)
+        s0.particle_qd.copy_(qd0)
+        solver.dv.zero_()
+        torch.cuda.synchronize()
+
+    # Moving scene: the first frames from rest above the ground, timed back to back.
+    g.replay()  # one untimed replay (first-replay setup)
+    restart()
     t0 = time.perf_counter()
     for _ in range(frames):
         g.replay()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / frames * 1e3
+    moving = s0.particle_q.clone()
+    # Resting scene: later frames, once the bodies have settled.
+    for _ in range(rest_warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rest_frames):
+        g.replay()
+    torch.cuda.synchronize()
+    ms_rest = (time.perf_counter() - t0) / rest_frames * 1e3
     if not torch.isfinite(s0.particle_q).all():
         raise RuntimeError("non-finite showcase state")
+    # The same frames once more, eagerly (the same kernels as the graph), reading
+    # every substep's solve statistics (untimed).
+    restart()
+    iters = []
+    for f in range(frames + rest_warmup + rest_frames):
+        solver.refresh_tangent(s0.particle_q, dt)
+        a, b = s0, s1
+        for _ in range(sub):
+            solver.step(a, b, None, None, dt)
+            iters.append(solver.solve_stats()["iterations"][0])
+            a, b = b, a
+        if f == frames - 1:
+            same = bool(torch.equal(s0.particle_q, moving))
+    it_move, it_rest = iters[:frames * sub], iters[-rest_frames * sub:]
+
+    def it_stats(v):
+        return {"mean": float(sum(v) / len(v)), "max": max(v), "min": min(v)}
+
     published = 46.757766123240195
     return {
         "workload": "showcase-sized scene: 3 Tet4 soft meshes (add_soft_mesh: tets, open-face triangles, edge springs) "
@@ -211,9 +304,15 @@ def bench_showcase(dev, frames: int = 100, warmup: int = 30):
         "hexes": solver.hex_count,
         "springs": model.spring_count,
         "triangles": model.tri_count,
+        "frames": f"first {frames} frames from rest 0.4 m above the ground (fall, impact, rebound)",
         "ms_per_frame": ms,
         "ms_per_substep": ms / sub,
-        "cg_iterations_last": solver.solve_stats()["iterations"][0],
+        "cg_iterations_per_substep": it_stats(it_move),
+        "eager_equals_graph": same,
+        "resting_scene": {"frames": f"{rest_frames} frames after frame {frames + rest_warmup}",
+                          "ms_per_frame": ms_rest,
+                          "cg_iterations_per_substep": it_stats(it_rest),
+                          "speedup_vs_reference": published / ms_rest},
         "reference_ms_per_frame": published,
         "reference_hardware": "NVIDIA GB10 (docs/validation/showcase.json)",
         "speedup_vs_reference": published / ms,
@@ -496,6 +595,30 @@ def bench_adjoint(dev, repeats: int = 3):
     if not torch.isfinite(g).all():
         raise RuntimeError("non-finite activation gradient")
     fwd, ck, bwd = min(tf), min(tc), min(tb)
+
+    # Iterations of every checked solve (kind 0: forward dv, kind 1: adjoint),
+    # read after each solve in one more gradient evaluation outside the timing.
+    from newton_solver_fem_amd import differentiation as diff_mod
+
+    counts = {0: [], 1: []}
+    plain_solve = diff_mod._solve
+
+    def counting_solve(s_, b_, kind):
+        x_ = plain_solve(s_, b_, kind)
+        counts[kind].append(s_.solve_stats()["iterations"][0])
+        return x_
+
+    diff_mod._solve = counting_solve
+    try:
+        _, _, g = gradient()
+    finally:
+        diff_mod._solve = plain_solve
+    loss_val = float(forward())
+    solver.check_differentiation()
+    fi, ai = np.asarray(counts[0]), np.asarray(counts[1])
+    ctx = {"scene": sc, "host": model.host, "gravity": model.gravity, "tip": tip.cpu().numpy(),
+           "target": target.double().cpu().numpy(), "steps": steps, "grad": g.double().cpu().numpy(),
+           "loss": loss_val}
     return {
         "workload": "differentiable Tet4 bar 40x8x8 cells (12,800 tets), x=0 face fixed, activations scaling mu, "
                     "d(tip-target loss)/d(activations), 60 substeps, checkpoint every 10, checked solves tol 1e-8",
@@ -508,7 +631,13 @@ def bench_adjoint(dev, repeats: int = 3):
         # the whole gradient evaluation (checkpointed forward + backward) over the plain forward
         "gradient_over_forward": (ck + bwd) / fwd,
         "residuals": res,
-    }
+        # per checked solve (tol 1e-8): the checkpointed gradient runs 60 forward solves, the 60 recomputed
+        # ones of the checkpoint segments and 60 adjoint solves
+        "solve_iterations": {
+            "forward_mean": float(fi.mean()), "forward_max": int(fi.max()), "forward_solves": int(fi.size),
+            "adjoint_mean": float(ai.mean()), "adjoint_max": int(ai.max()), "adjoint_solves": int(ai.size),
+        },
+    }, ctx
 
 
 def main():
@@ -651,8 +780,9 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = bench_batched(dev, args.batched_replays)
+    adj_ctx = None
     if not args.no_adjoint:
-        out["adjoint"] = bench_adjoint(dev)
+        out["adjoint"], adj_ctx = bench_adjoint(dev)
     if not args.no_showcase:
         out["showcase"] = bench_showcase(dev)
     if not args.no_cut:
@@ -660,6 +790,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, model, args.cpu_substeps)
         out["cpu_baseline"]["host_cpus"] = os.cpu_count()
+        if adj_ctx is not None:
+            out["adjoint"]["fd_check"] = cpu_baseline_adjoint_check(adj_ctx)
     print(json.dumps(out), flush=True)
 
 
