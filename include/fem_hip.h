@@ -115,6 +115,11 @@ const char* fem_last_error(void);
 /* Number of stored 3x3 blocks (A_bsr.nnz) and the largest row length. */
 int fem_pattern_size(const fem_system* sys, int64_t* nnz, int32_t* max_row_blocks);
 
+/* Device allocations the library holds for this system and their bytes (no
+ * reference counterpart: lets a caller check that refreshes allocate
+ * nothing). */
+int fem_memory_stats(const fem_system* sys, int64_t* allocations, int64_t* bytes);
+
 /* Binds caller-owned device buffers that stay valid for the system's life:
  * offsets [P+1] int32, columns [capacity] int32 (offsets and the first nnz
  * columns filled here, synchronously), values [capacity][9], inv_diag [P][9]
@@ -356,9 +361,18 @@ int fem_prepare_topology(fem_system* sys, int32_t original_node_count);
  * when the resident kernel holds the reserved rows.  Without a pool the
  * exact pattern of the edited model is built again (within the bound block
  * capacity; offsets and columns are rewritten and fem_pattern_size reports the
- * new nnz); when the pattern changes, device arrays the previous pattern used
- * stay allocated until fem_destroy (an unchanged topology allocates
- * nothing).  Synchronizes the stream.  Not for batched systems. */
+ * new nnz), as the reference re-sorts into the same A_bsr storage
+ * (solver.py:920-935).  When the bound capacity exceeds the initial pattern
+ * (fem_bind), the pattern arrays, the resident CG plan's storage and the
+ * assembly launch are sized for it, and an edit that fits them is written in
+ * place: the same device arrays and launches, so launches captured before the
+ * edit replay the edited topology, and nothing is allocated.  An edit that
+ * does not fit (the streamed CG plan, more assembly groups or CG columns than
+ * reserved) gets new arrays; the old ones stay allocated until fem_destroy
+ * (launches captured before still name them and step the old topology, so
+ * they must be captured again) and stats[0] is set to 1 (else 0; the other
+ * counters are 0 without a pool).  An unchanged topology allocates nothing.
+ * Synchronizes the stream.  Not for batched systems. */
 int fem_update_topology(fem_system* sys, const int32_t* tet_indices_dev, const float* tet_poses_dev,
                         const float* tet_materials_dev, const float* particle_mass_dev,
                         const float* particle_inv_mass_dev, const int32_t* particle_flags_dev, const float* q_dev,

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "fem_hip.h"
@@ -2106,6 +2107,7 @@ struct fem_system {
   int asm_cap = 0;  // fixed assembly group count of a reserved topology pool (0: none)
   int max_resident = 0;
   std::vector<void*> allocs;
+  std::unordered_map<const void*, size_t> alloc_bytes;  // size of every live allocation
   bool keep_replaced = false;
   int64_t b_cap = 0;  // block capacity of the bound offsets/columns/values storage
   // Host copies of the element data a topology rebuild needs besides the tets.
@@ -2177,6 +2179,7 @@ struct fem_system {
     if (n == 0) n = 1;
     HIP_TRY(hipMalloc((void**)dst, n * sizeof(T_)));
     allocs.push_back(*dst);
+    alloc_bytes[*dst] = n * sizeof(T_);
     if (src) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T_), hipMemcpyHostToDevice));
     else HIP_TRY(hipMemset(*dst, 0, n * sizeof(T_)));
     return FEM_OK;
@@ -2192,6 +2195,25 @@ struct fem_system {
     if (!p || keep_replaced) return;
     (void)hipFree(p);
     allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+    alloc_bytes.erase(p);
+  }
+  // Whether allocation p holds at least `bytes`.
+  bool holds(const void* p, size_t bytes) const {
+    if (!p) return bytes == 0;
+    const auto it = alloc_bytes.find(p);
+    return it != alloc_bytes.end() && it->second >= bytes;
+  }
+  // n elements of src into the existing allocation *dst (which holds them).
+  template <typename T_>
+  int write(T_* dst, const T_* src, size_t n) {
+    if (n) HIP_TRY(hipMemcpy(dst, src, n * sizeof(T_), hipMemcpyHostToDevice));
+    return FEM_OK;
+  }
+  // Device copy of n elements into a new allocation of at least cap elements.
+  template <typename T_>
+  int upload_cap(T_** dst, const T_* src, size_t n, size_t cap) {
+    if (int rc = upload(dst, (const T_*)nullptr, std::max(n, cap))) return rc;
+    return write(*dst, src, n);
   }
   // Device copy of n elements of src into a new allocation of at least cap
   // elements; the previous allocation *dst (if any) is dropped.
@@ -2300,7 +2322,7 @@ struct fem_system {
   // Device copies of everything derived from the pattern: rows, incidences,
   // their slots, per-slot contribution lists and the constant spring blocks.
   // tinc_cap / contrib_cap reserve room for later incidence updates.
-  int upload_pattern(size_t tinc_cap = 0, size_t contrib_cap = 0) {
+  int upload_pattern(size_t tinc_cap = 0, size_t contrib_cap = 0, size_t block_cap = 0) {
     std::vector<int32_t> off32(P + 1), tptr(P + 1), hptr(P + 1), sptr(P + 1), rptr(P + 1);
     for (int i = 0; i <= P; ++i) {
       off32[i] = (int32_t)pt.offsets[i];
@@ -2311,7 +2333,7 @@ struct fem_system {
     }
     int rc;
     if ((rc = replace(&d_offsets, off32.data(), off32.size())) ||
-        (rc = replace(&d_cols, pt.columns.data(), pt.columns.size())) ||
+        (rc = replace(&d_cols, pt.columns.data(), pt.columns.size(), block_cap)) ||
         (rc = replace(&d_diag, pt.diag_rel.data(), pt.diag_rel.size())) ||
         (rc = replace(&d_tinc_ptr, tptr.data(), tptr.size())) ||
         (rc = replace(&d_tinc, pt.tinc.data(), pt.tinc.size(), tinc_cap)) ||
@@ -2319,7 +2341,7 @@ struct fem_system {
         (rc = replace(&d_hinc_ptr, hptr.data(), hptr.size())) ||
         (rc = replace(&d_hinc, pt.hinc.data(), pt.hinc.size())) ||
         (rc = replace(&d_hslots, pt.hslots.data(), pt.hslots.size())) ||
-        (rc = replace(&d_slot_ptr, pt.slot_ptr.data(), pt.slot_ptr.size())) ||
+        (rc = replace(&d_slot_ptr, pt.slot_ptr.data(), pt.slot_ptr.size(), block_cap ? block_cap + 1 : 0)) ||
         (rc = replace(&d_contrib, pt.contrib.data(), pt.contrib.size(), contrib_cap)))
       return rc;
     if (S > 0 || R > 0) {
@@ -2328,7 +2350,7 @@ struct fem_system {
           (rc = replace(&d_sinc, pt.sinc.data(), pt.sinc.size())) ||
           (rc = replace(&d_rinc_ptr, rptr.data(), rptr.size())) ||
           (rc = replace(&d_rinc, pt.rinc.data(), pt.rinc.size())) ||
-          (rc = replace(&d_slot_spring, slot_spring.data(), slot_spring.size())))
+          (rc = replace(&d_slot_spring, slot_spring.data(), slot_spring.size(), S > 0 ? block_cap : 0)))
         return rc;
     }
     return FEM_OK;
@@ -2404,7 +2426,12 @@ struct fem_system {
       asm_rb = rb;
       return FEM_OK;
     }
-    const int padded = n > 8 ? (n + 7) & ~7 : std::max(n, 1);
+    // A few spare (empty) groups and, when the bound storage leaves room for
+    // more blocks, the full staging budget: a topology edit without a pool
+    // that still fits this launch is then written in place.
+    const bool room = b_cap > (int64_t)pt.offsets[P];
+    const int spare = std::max(8, n / 256);
+    const int padded = n > 8 ? (n + spare + 7) & ~7 : std::max(n, 1) + spare;
     while ((int)g.size() < padded + 1) g.push_back(P);
     const std::vector<int32_t> hd = assemble_headers(g, padded);
     int* dg = nullptr;
@@ -2413,7 +2440,7 @@ struct fem_system {
     d_asm_groups = dg;
     asm_ngroups = padded;
     asm_rb = rb;
-    asm_lds = worst;
+    asm_lds = room ? std::max(worst, ASM_LDS_BUDGET) : worst;
     return FEM_OK;
   }
 
@@ -2462,6 +2489,68 @@ struct fem_system {
     const int rc2 = plan_cg_nt(256, false);
     if (rc2 != FEM_OK) g_error = why + "; " + g_error;
     return rc2;
+  }
+
+  // Topology edit without a reserved pool, in place: the edited pattern `np`
+  // goes into the existing pattern arrays, the assembly keeps its launch
+  // (group count and staging) with the groups rewritten, and the resident CG
+  // is re-planned over its chunk rows within its storage -- the same device
+  // arrays and launches as before, so a graph captured before the edit
+  // replays the edited topology (as the reference re-sorts into the same
+  // A_bsr storage, solver.py:920-935).  Returns FEM_ERR_UNSUPPORTED, with
+  // nothing changed, when the edit does not fit (streamed CG plan, arrays or
+  // launches too small); the caller then reallocates.
+  int edit_pattern_in_place(Pattern& np) {
+    const size_t nnz = (size_t)np.offsets[P];
+    if (!cg_res || compact || h_chunk_rows.empty())
+      return fail(FEM_ERR_UNSUPPORTED, "in-place topology edits need the resident CG plan");
+    if (!holds(d_cols, 4 * nnz) || !holds(d_slot_ptr, 4 * (nnz + 1)) || !holds(d_tinc, 4 * np.tinc.size()) ||
+        !holds(d_tslots, 4 * np.tslots.size()) || !holds(d_hinc, 4 * np.hinc.size()) ||
+        !holds(d_hslots, 4 * np.hslots.size()) || !holds(d_contrib, 2 * np.contrib.size()) ||
+        !holds(d_sinc, 4 * np.sinc.size()) || !holds(d_rinc, 4 * np.rinc.size()) ||
+        (S > 0 && !holds(d_slot_spring, sizeof(float2) * nnz)))
+      return fail(FEM_ERR_UNSUPPORTED, "the edited pattern exceeds the pattern storage");
+    std::swap(pt, np);  // plan over the edited pattern; swapped back on failure
+    std::vector<int32_t> g;
+    size_t worst = 0;
+    assemble_groups(asm_rb, g, worst);
+    int rc = FEM_OK;
+    if ((int)g.size() - 1 > asm_ngroups || worst > asm_lds)
+      rc = fail(FEM_ERR_UNSUPPORTED, "the edited topology exceeds the assembly launch");
+    else
+      rc = plan_cg_nt(cg_nt, true, true);  // checks everything before it writes
+    if (rc != FEM_OK) {
+      std::swap(pt, np);
+      return rc == FEM_ERR_HIP ? rc : fail(FEM_ERR_UNSUPPORTED, g_error);
+    }
+    std::vector<int32_t> off32(P + 1), tptr(P + 1), hptr(P + 1), sptr(P + 1), rptr(P + 1);
+    for (int i = 0; i <= P; ++i) {
+      off32[i] = (int32_t)pt.offsets[i];
+      tptr[i] = (int32_t)pt.tinc_ptr[i];
+      hptr[i] = (int32_t)pt.hinc_ptr[i];
+      sptr[i] = (int32_t)pt.sinc_ptr[i];
+      rptr[i] = (int32_t)pt.rinc_ptr[i];
+    }
+    if ((rc = write(d_offsets, off32.data(), off32.size())) || (rc = write(d_cols, pt.columns.data(), nnz)) ||
+        (rc = write(d_diag, pt.diag_rel.data(), pt.diag_rel.size())) ||
+        (rc = write(d_tinc_ptr, tptr.data(), tptr.size())) ||
+        (rc = write(d_tinc, pt.tinc.data(), pt.tinc.size())) ||
+        (rc = write(d_tslots, pt.tslots.data(), pt.tslots.size())) ||
+        (rc = write(d_hinc_ptr, hptr.data(), hptr.size())) || (rc = write(d_hinc, pt.hinc.data(), pt.hinc.size())) ||
+        (rc = write(d_hslots, pt.hslots.data(), pt.hslots.size())) ||
+        (rc = write(d_slot_ptr, pt.slot_ptr.data(), pt.slot_ptr.size())) ||
+        (rc = write(d_contrib, pt.contrib.data(), pt.contrib.size())))
+      return rc;
+    if (S > 0 || R > 0) {
+      const std::vector<float2> slot_spring = S > 0 ? spring_slots() : std::vector<float2>();
+      if ((rc = write(d_sinc_ptr, sptr.data(), sptr.size())) || (rc = write(d_sinc, pt.sinc.data(), pt.sinc.size())) ||
+          (rc = write(d_rinc_ptr, rptr.data(), rptr.size())) || (rc = write(d_rinc, pt.rinc.data(), pt.rinc.size())) ||
+          (rc = write(d_slot_spring, slot_spring.data(), slot_spring.size())))
+        return rc;
+    }
+    g.resize(asm_ngroups + 1, P);
+    const std::vector<int32_t> hd = assemble_headers(g, asm_ngroups);
+    return write(d_asm_groups, hd.data(), hd.size());
   }
 
   // keep_rows: re-plan the resident kernel for the current Krylov pattern over
@@ -2752,9 +2841,15 @@ struct fem_system {
                 (rc = upload(&n_rmask, res_mask.data(), res_mask.size())) ||
                 (rc = upload(&n_rrow, res_row.data(), res_row.size()))))
       return rc;
+    // Column lists and local column numbers with room for an edit of the
+    // pattern that keeps the chunk rows (written in place then): every block
+    // the bound storage can still take adds at most one column to a chunk.
+    const size_t room = (size_t)std::max<int64_t>(0, b_cap - (int64_t)koff[P]);
+    const size_t tcols_cap = tile_cols.size() + tile_cols.size() / 16 + 64 + room;
+    const size_t lidx_cap = lidx.size() + room;
     if ((rc = upload(&n_tcptr, tile_cptr.data(), tile_cptr.size())) ||
-        (rc = upload(&n_tcols, tile_cols.data(), tile_cols.size())) ||
-        (rc = upload(&n_lidx, lidx.data(), lidx.size())) ||
+        (rc = upload_cap(&n_tcols, tile_cols.data(), tile_cols.size(), tcols_cap)) ||
+        (rc = upload_cap(&n_lidx, lidx.data(), lidx.size(), lidx_cap)) ||
         (rc = upload(&n_rows, chunk_rows.data(), chunk_rows.size())) ||
         (rc = upload(&n_inst, chunk_inst.data(), chunk_inst.size())) ||
         (rc = upload(&n_ichunks, inst_chunks.data(), inst_chunks.size())) ||
@@ -2786,8 +2881,8 @@ struct fem_system {
     cg_grid = cg_queue ? cap : n_chunks;
     nchunks = n_chunks;
     h_chunk_rows = chunk_rows;
-    cap_tile_cols = tile_cols.size();
-    cap_lidx = lidx.size();
+    cap_tile_cols = tcols_cap;
+    cap_lidx = lidx_cap;
     cg_groups = groups;
     cg_k = k_rows;
     cg_nt = nt;
@@ -2905,7 +3000,7 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
         if (inst_of[s->pt.columns[k]] != inst_of[i]) return cleanup(fail(FEM_ERR_ARG, "element spans two instances"));
     if ((rc = s->upload(&s->d_row_inst, inst_of.data(), P))) return cleanup(rc);
   }
-  if ((rc = s->upload_pattern())) return cleanup(rc);
+  if ((rc = s->upload_pattern(4 * (size_t)s->T, 16 * (size_t)s->T + 64 * (size_t)s->H))) return cleanup(rc);
   std::vector<unsigned char> fixed_mask(std::max(P, 1), 0);
   for (int k = 0; k < s->nfixed; ++k) {
     int i = d->fixed_indices[k];
@@ -2954,6 +3049,18 @@ int fem_destroy(fem_system* s) {
   return FEM_OK;
 }
 
+int fem_memory_stats(const fem_system* s, int64_t* allocations, int64_t* bytes) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  int64_t total = 0;
+  for (const void* p : s->allocs) {
+    const auto it = s->alloc_bytes.find(p);
+    if (it != s->alloc_bytes.end()) total += (int64_t)it->second;
+  }
+  if (allocations) *allocations = (int64_t)s->allocs.size();
+  if (bytes) *bytes = total;
+  return FEM_OK;
+}
+
 int fem_pattern_size(const fem_system* s, int64_t* nnz, int32_t* max_row_blocks) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   if (nnz) *nnz = s->pt.offsets[s->P];
@@ -2969,7 +3076,18 @@ int fem_bind(fem_system* s, int32_t* offsets, int32_t* columns, float* values, f
   if (block_capacity < nnz) return fail(FEM_ERR_ARG, "block_capacity is smaller than the pattern");
   if (s->cs_scale && block_capacity != s->b_cap)
     return fail(FEM_ERR_STATE, "component scaling storage is bound to the previous block capacity");
+  const bool grow = block_capacity > s->b_cap && block_capacity > nnz && !s->topo_count && s->P > 0 && s->ninst == 1;
   s->b_cap = block_capacity;
+  // Storage that leaves room for more blocks: the pattern arrays, the CG
+  // plan storage and the assembly launch are sized for it now (before any
+  // launch is captured), so that a later topology edit without a reserved
+  // pool that fits is written in place.
+  if (grow) {
+    int rc;
+    if ((rc = s->upload_pattern(4 * (size_t)s->T, 16 * (size_t)s->T + 64 * (size_t)s->H, (size_t)block_capacity)) ||
+        (rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg()))
+      return rc;
+  }
   s->b_offsets = offsets;
   s->b_cols = columns;
   s->b_vals = values;
@@ -3698,17 +3816,26 @@ int fem_update_topology(fem_system* s, const int32_t* tet_indices, const float* 
                       np.tinc_ptr == cur.tinc_ptr && np.hinc == cur.hinc && np.hinc_ptr == cur.hinc_ptr &&
                       np.sinc == cur.sinc && np.sinc_ptr == cur.sinc_ptr;
     if (!same) {  // an unchanged topology keeps every array (and allocates nothing)
-      // Arrays a graph captured earlier may name stay allocated from here on.
-      s->keep_replaced = true;
-      s->pt = std::move(np);
-      int rc;
-      if ((rc = s->upload_pattern())) return rc;
+      int rc = s->edit_pattern_in_place(np);
+      if (rc == FEM_ERR_UNSUPPORTED) {
+        // Does not fit the current arrays and launches: new ones.  Graphs
+        // captured before this edit name the old ones, which therefore stay
+        // allocated (and describe the old topology) until the system is
+        // destroyed; such graphs must be captured again (counters[0] = 1).
+        s->keep_replaced = true;
+        s->pt = std::move(np);
+        if ((rc = s->upload_pattern(4 * (size_t)T, 16 * (size_t)T + 64 * (size_t)s->H, (size_t)s->b_cap)))
+          return rc;
+        if ((rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
+        counters[0] = 1;
+      } else if (rc != FEM_OK) {
+        return rc;
+      }
       std::vector<int32_t> off32(P + 1);
       for (int i = 0; i <= P; ++i) off32[i] = (int32_t)s->pt.offsets[i];
       HIP_TRY(hipMemcpy(s->b_offsets, off32.data(), off32.size() * 4, hipMemcpyHostToDevice));
       if (!s->pt.columns.empty())
         HIP_TRY(hipMemcpy(s->b_cols, s->pt.columns.data(), s->pt.columns.size() * 4, hipMemcpyHostToDevice));
-      if ((rc = s->plan_assemble(s->asm_rb)) || (rc = s->plan_cg())) return rc;
     }
   }
   if (T > 0) {

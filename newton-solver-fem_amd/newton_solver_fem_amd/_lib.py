@@ -78,6 +78,7 @@ SIGNATURES = {
     "fem_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fem_last_error": (ctypes.c_char_p, []),
     "fem_pattern_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip]),
+    "fem_memory_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_int64]),
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "fem_bind_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -207,6 +207,7 @@ class SolverFEM:
         self.hex_count = int(model.fem.hex_count)
         self._diff = None
         self._revision = 0
+        self.topology_reallocated = False
 
         self._lib = _lib.load()
         host = model.host
@@ -315,6 +316,12 @@ class SolverFEM:
         _lib.check(self._lib.fem_pattern_size(self._sys, ctypes.byref(nnz), ctypes.byref(maxrow)), "fem_pattern_size")
         return int(nnz.value)
 
+    def memory_stats(self) -> dict:
+        """Device allocations the library holds for this solver and their bytes."""
+        n, b = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._lib.fem_memory_stats(self._sys, ctypes.byref(n), ctypes.byref(b)), "fem_memory_stats")
+        return {"allocations": int(n.value), "bytes": int(b.value)}
+
     def _view_matrix(self):
         offsets, columns, values = self._A_storage
         n = self._pattern_nnz()
@@ -370,7 +377,11 @@ class SolverFEM:
         valid for an existing graph, and :attr:`last_update` reports the
         reference's update counters; ``element_sources`` names each slot's
         previous element (-1: new).  Otherwise the exact pattern is built
-        again within the reserved matrix storage.  Clears the warm start.
+        again within the reserved matrix storage, in place when it fits the
+        arrays and launches sized at construction (graphs captured before the
+        edit then replay the edited topology); otherwise new arrays are made
+        and :attr:`topology_reallocated` is set (such graphs must be captured
+        again).  Clears the warm start.
         """
         if self.differentiable:
             raise NotImplementedError("Differentiation requires fixed topology without streaming/component scaling")
@@ -408,6 +419,10 @@ class SolverFEM:
             self.last_update = {k: int(stats[i]) for i, k in enumerate(keys)}
             self.last_update["full_numeric_refresh"] = bool(stats[7])
         else:
+            # An edit that did not fit the arrays and launches sized at
+            # construction got new ones: graphs captured before it step the
+            # old topology and must be captured again.
+            self.topology_reallocated = bool(stats[0])
             self._view_matrix()
         self.tet_materials = m.tet_materials.detach().clone()
         self.dv.zero_()

@@ -324,6 +324,82 @@ def test_refresh_topology_without_pool_resorts_in_place():
     np.testing.assert_allclose(np_(out.particle_qd), np_(ref_out.particle_qd), rtol=1e-4, atol=2e-5)
 
 
+def _capture_substep(solver, state, output):
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        solver.refresh_tangent(state.particle_q)
+        solver.step(state, output, None, None, solver.dt)
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        solver.refresh_tangent(state.particle_q)
+        solver.step(state, output, None, None, solver.dt)
+    return g
+
+
+def _fresh_step(records, state):
+    ref_model, ref = pool_solver(records)
+    ref_in, expected = ref_model.state(), ref_model.state()
+    ref_in.particle_q.copy_(state.particle_q)
+    ref_in.particle_qd.copy_(state.particle_qd)
+    ref.refresh_tangent(ref_in.particle_q)
+    ref.step(ref_in, expected, None, None, ref.dt)
+    return expected
+
+
+def test_no_pool_edit_in_place_replays_captured_graph():
+    """Without a pool an edit that fits the storage sized at construction is
+    written into the same arrays and launches: a graph captured before the
+    edit replays the edited topology (bit for bit what the solver now steps
+    eagerly, and a fresh build's step to the solve tolerance) and the
+    library allocates nothing."""
+    print("case no-pool in-place capture", flush=True)
+    model, solver = pool_solver()
+    state, output = model.state(), model.state()
+    state.particle_q.copy_(pool_deformed(model))
+    state.particle_qd.copy_(state.particle_q * torch.tensor((0.1, -0.05, 0.02), device=DEV))
+    g = _capture_substep(solver, state, output)
+    kept, mem = storage(solver), solver.memory_stats()
+    for records in (POOL_SPLIT, POOL_SPLIT[:2], POOL_INITIAL):
+        pool_edit(model, records)
+        solver.refresh_topology(state.particle_q)
+        assert not solver.topology_reallocated
+        assert solver.memory_stats() == mem and storage(solver) == kept
+        g.replay()
+        torch.cuda.synchronize()
+        replayed_q, replayed_qd = np_(output.particle_q), np_(output.particle_qd)
+        solver.dv.zero_()  # the replay started from the warm start the edit cleared
+        solver.refresh_tangent(state.particle_q)
+        solver.step(state, output, None, None, solver.dt)
+        np.testing.assert_array_equal(replayed_q, np_(output.particle_q))
+        np.testing.assert_array_equal(replayed_qd, np_(output.particle_qd))
+        expected = _fresh_step(records, state)
+        np.testing.assert_allclose(replayed_q, np_(expected.particle_q), rtol=2e-5, atol=2e-6)
+        np.testing.assert_allclose(replayed_qd, np_(expected.particle_qd), rtol=1e-4, atol=2e-5)
+
+
+def test_no_pool_edit_that_does_not_fit_reallocates_and_says_so():
+    """The streamed CG plan is not re-planned in place: the edit gets new
+    arrays, topology_reallocated is set, and the solver steps the edited
+    topology (a fresh build's step to the solve tolerance)."""
+    print("case no-pool reallocating edit", flush=True)
+    model, solver = pool_solver()
+    solver.set_cg_variant("streamed")
+    state, output = model.state(), model.state()
+    state.particle_q.copy_(pool_deformed(model))
+    mem = solver.memory_stats()
+    pool_edit(model, POOL_SPLIT)
+    solver.refresh_topology(state.particle_q)
+    assert solver.topology_reallocated
+    assert solver.memory_stats()["allocations"] > mem["allocations"]
+    solver.refresh_tangent(state.particle_q)
+    solver.step(state, output, None, None, solver.dt)
+    expected = _fresh_step(POOL_SPLIT, state)
+    np.testing.assert_allclose(np_(output.particle_q), np_(expected.particle_q), rtol=2e-5, atol=2e-6)
+    np.testing.assert_allclose(np_(output.particle_qd), np_(expected.particle_qd), rtol=1e-4, atol=2e-5)
+
+
 def test_topology_argument_rules():
     print("case topology rules", flush=True)
     model, solver = pool_solver(streaming=True)
@@ -605,8 +681,10 @@ def test_unchanged_topology_refresh_keeps_every_array():
     solver.refresh_tangent(positions)
     before = solver.A_bsr.dense()
     pointers = storage(solver)
+    mem = solver.memory_stats()
     for _ in range(5):
         solver.refresh_topology(positions)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(solver.A_bsr.dense(), before)
     assert storage(solver) == pointers
+    assert solver.memory_stats() == mem  # the library allocated nothing either
