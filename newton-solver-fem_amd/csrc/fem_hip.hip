@@ -1395,6 +1395,60 @@ __device__ __forceinline__ void res_spmv(const CgArgs& a, const ResThread& rt, c
   // The next call writes u_l first and seg_l only after its own barrier.
 }
 
+// Exchange records of the resident kernel, written in two steps so that the
+// stores of every row fill whole cache lines.  For each k, a wave's rows are
+// 64 consecutive rows of the chunk (local rows 256 k + 64 w .. + 63), whose
+// records form one contiguous region.  xw_stage: each lane puts its 36-byte
+// record into the wave's LDS area for k (576 floats at stride 9: no bank
+// conflicts).  xw_flush: the wave writes the region as consecutive 16-byte
+// pieces, instead of three partial writes of every 64-byte segment.  LDS
+// operations of one wave complete in issue order and the compiler barriers
+// keep that order, so no workgroup barrier is needed; every k is staged
+// before the first is flushed, so one LDS round trip serves all four.
+// Wave-uniform values are kept in scalar registers; lanes past the chunk's
+// rows (own == false) stage nothing.
+__device__ __forceinline__ float* xw_area(float* stage, int k) {
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  return stage + 64 * XW * (4 * k + wv);
+}
+
+__device__ __forceinline__ void xw_stage(float* stage, int k, bool own, const float (&z)[3], const float (&w)[3],
+                                         const float (&p)[3]) {
+  if (!own) return;
+  float* e = xw_area(stage, k) + XW * ((int)threadIdx.x & 63);
+  e[0] = z[0];
+  e[1] = z[1];
+  e[2] = z[2];
+  e[3] = w[0];
+  e[4] = w[1];
+  e[5] = w[2];
+  e[6] = p[0];
+  e[7] = p[1];
+  e[8] = p[2];
+}
+
+__device__ __forceinline__ void xw_flush(float* xw, int R0, int k, int nr, float* stage) {
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int first = RES_NT * k + 64 * wv;
+  const int valid = min(64, nr - first);
+  if (valid <= 0) return;
+  const float* sw = xw_area(stage, k);
+  const int lane = (int)threadIdx.x & 63;
+  const int nf = XW * valid;
+  float* dst = xw + XW * (size_t)(R0 + first);
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int f = 4 * lane + 256 * u;
+    if (f + 4 <= nf) st_wt(dst + f, *reinterpret_cast<const f4v*>(sw + f));
+  }
+  if ((nf & 3) && lane == ((nf >> 2) & 63))  // a partial wave's last 1-3 floats
+    for (int t = nf & ~3; t < nf; ++t) st_wt(dst + t, sw[t]);
+}
+constexpr int XW_STAGE = 64 * XW;  // LDS floats per wave and k for xw_stage
+
+// Exchange record of one row written in place (three write-through stores);
+// the streamed kernels use this (staging costs them registers that decide
+// how many workgroups fit on a CU).
 __device__ __forceinline__ void st_xw(float* xw, int i, const float (&z)[3], const float (&w)[3],
                                       const float (&p)[3]) {
   float* e = xw + XW * (size_t)i;
@@ -1446,6 +1500,11 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
   f4v* seg_l = ures_l + RES_UC * NT;
   float* mat_l = reinterpret_cast<float*>(seg_l + RES_SEG + 1);
   uint16_t* lidx_l = reinterpret_cast<uint16_t*>(mat_l + 9 * RES_JL * NT);
+  // Record staging of the resident kernel (xw_stage): the operand table,
+  // which nothing reads between the end of an SpMV and the next phase's
+  // barrier.
+  float* stage = reinterpret_cast<float*>(ures_l);
+  static_assert(!RES || XW_STAGE * (NT / 64) * K <= 4 * RES_UC * NT, "record staging fits");
   const int inst = a.chunk_inst[c];
   const int R0 = a.chunk_rows[c], nr = a.chunk_rows[c + 1] - R0;
   const int cb = a.inst_chunks[inst], ce = a.inst_chunks[inst + 1], nci = ce - cb;
@@ -1583,13 +1642,22 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
       x[k][0] = x0.x; x[k][1] = x0.y; x[k][2] = x0.z;
       r[k][0] = rr0.x; r[k][1] = rr0.y; r[k][2] = rr0.z;
       z[k][0] = zz.x; z[k][1] = zz.y; z[k][2] = zz.z;
-      st_xw(a.xw, i, z[k], w[k], p[k]);
       pb[0] += dot_cg(b, b);
       pb[1] += dot_cg(rr0, rr0);
       pb[2] += dot_cg(rr0, zz);
     }
+    if constexpr (RES)
+      xw_stage(stage, k, own[k], z[k], w[k], p[k]);
+    else if (own[k])
+      st_xw(a.xw, R0 + threadIdx.x + NT * k, z[k], w[k], p[k]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) qv[k][d] = 0.f;
+  }
+  if constexpr (RES) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = 0; k < K; ++k) xw_flush(a.xw, R0, k, nr, stage);
+    asm volatile("" ::: "memory");
   }
   {
     double wsum[3];
@@ -1684,10 +1752,12 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
+      if constexpr (RES)
+        xw_stage(stage, k, own[k], z[k], w[k], p[k]);
+      else if (own[k])
+        st_xw(xw_next, R0 + threadIdx.x + NT * k, z[k], w[k], p[k]);
       if (own[k]) {
-        const int i = R0 + threadIdx.x + NT * k;
         const v3 qq = {qv[k][0], qv[k][1], qv[k][2]}, ww = {w[k][0], w[k][1], w[k][2]};
-        st_xw(xw_next, i, z[k], w[k], p[k]);
         const v3 pv = {p[k][0], p[k][1], p[k][2]}, zv = {z[k][0], z[k][1], z[k][2]},
                  rv = {r[k][0], r[k][1], r[k][2]};
         pp[0] += dot_cg(pv, qq);
@@ -1698,6 +1768,12 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
         pp[5] += dot_cg(rv, zv);
         pp[6] += dot_cg(rv, rv);
       }
+    }
+    if constexpr (RES) {
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int k = 0; k < K; ++k) xw_flush(xw_next, R0, k, nr, stage);
+      asm volatile("" ::: "memory");
     }
     {
       double ws[7];
