@@ -12,7 +12,8 @@ residual-masked so the result equals the tol-1e-2 solve).
 Prints one JSON line: the substep rate, the CG kernel's HBM roofline
 (algorithmic bytes over its HIP-event duration, plus the measured HBM traffic
 per launch from the committed rocprofv3 summary), the per-phase times, and the
-batched, adjoint, showcase-sized, cut-scene and CPU-baseline measurements.
+Hex8 cantilever, batched, adjoint, showcase-sized, cut-scene and CPU-baseline
+measurements.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -459,6 +460,87 @@ def bench_cut(dev, frames: int = 60, warmup: int = 20):
     }
 
 
+def bench_cantilever(dev, steps: int = 100, warmup: int = 5):
+    """Hex8 Q1 cantilever: 64x16x16 hexes of 5 mm (18,785 nodes, 16,384 hexes,
+    2x2x2 Gauss), E 1e6, nu 0.4, rho 1100, x = 0 face fixed, gravity; each
+    substep refreshes the tangent, evaluates forces + predictor and runs
+    exactly 50 PCG iterations (as the large Tet4 substep).  `steps`
+    consecutive substeps from rest are timed back to back, with HIP events
+    around each phase on the launch stream."""
+    import numpy as np
+    import torch
+
+    from newton_solver_fem_amd import build_hex_model, meshes
+    from newton_solver_fem_amd._lib import check
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    sc = meshes.hex_cantilever_scene()
+    model = build_hex_model(sc, device=dev)
+    solver = SolverFEM(model, dt=DT, solver_type="cg", preconditioner_type="block_diag",
+                       linear_solver_maxiter=CG_ITERS, linear_solver_tolerance=1e-2, self_contact_ke=0.0,
+                       fixed_particle_indices=sc.fixed, linear_solver_use_cuda_graph=False)
+    lib, sysh = solver._lib, solver._sys
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+    st = [model.state(), model.state()]
+
+    def substep(ev=None):
+        a, b = st
+        if ev is not None:
+            ev[0].record(stream)
+        check(lib.fem_restore_fixed(sysh, a.particle_q.data_ptr(), None, sptr), "restore")
+        check(lib.fem_assemble(sysh, a.particle_q.data_ptr(), DT, sptr), "assemble")
+        if ev is not None:
+            ev[1].record(stream)
+        check(lib.fem_step_rhs(sysh, a.particle_q.data_ptr(), a.particle_qd.data_ptr(), a.particle_f.data_ptr(),
+                               None, None, DT, sptr), "step_rhs")
+        if ev is not None:
+            ev[2].record(stream)
+        check(lib.fem_step_solve(sysh, a.particle_q.data_ptr(), a.particle_qd.data_ptr(), a.particle_f.data_ptr(),
+                                 b.particle_q.data_ptr(), b.particle_qd.data_ptr(), DT, sptr), "step_solve")
+        if ev is not None:
+            ev[3].record(stream)
+        st.reverse()
+
+    for _ in range(warmup):
+        substep()
+    torch.cuda.synchronize()
+    st[:] = [model.state(), model.state()]  # restart from rest
+    solver.dv.zero_()
+    events =[[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        substep(events[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    asm_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    rhs_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    cg_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in events]))
+    if not torch.isfinite(st[0].particle_q).all():
+        raise RuntimeError("non-finite cantilever state")
+    P, H, B = model.particle_count, solver.hex_count, solver.A_bsr.nnz
+    tip = float(st[0].particle_q[:, 2].min() - model.particle_q[:, 2].min())
+    # Assembly: 364 B per hex (8 indices, 8 inverse Jacobians, 8 weights, materials), A and M^-1 written,
+    # positions read; forces + predictor: the hex data again, A read (values + columns), x, v, b.
+    asm_bytes = 364 * H + 36 * B + 48 * P
+    rhs_bytes = 364 * H + 40 * B + 48 * P
+    cg_bytes = cg_algorithmic_bytes(B, P, CG_ITERS)
+    return {
+        "workload": "Hex8 Q1 cantilever 64x16x16 hexes (2x2x2 Gauss), x=0 face fixed, gravity; per substep tangent "
+                    "refresh + forces/predictor + 50 PCG iterations, 100 substeps from rest",
+        "nodes": P,
+        "hexes": H,
+        "bsr_blocks": B,
+        "substeps_per_s": steps / elapsed,
+        "ms_per_substep": elapsed / steps * 1e3,
+        "phases_ms": {"assemble": asm_ms, "forces_rhs": rhs_ms, "cg": cg_ms},
+        "achieved_GBs": {"assemble": asm_bytes / (asm_ms * 1e-3) / 1e9, "forces_rhs": rhs_bytes / (rhs_ms * 1e-3) / 1e9,
+                         "cg": cg_bytes / (cg_ms * 1e-3) / 1e9},
+        "tip_drop_m": tip,
+    }
+
+
 def bench_batched(dev, replays: int):
     """256 uneven drop meshes: one batched solver vs 256 solvers stepped in turn.
 
@@ -659,6 +741,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-showcase", action="store_true", help="skip the showcase-sized scene")
     ap.add_argument("--no-cut", action="store_true", help="skip the streaming-topology cut scene")
+    ap.add_argument("--no-cantilever", action="store_true", help="skip the Hex8 cantilever substeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design only "
@@ -778,6 +861,8 @@ def main():
         "cg_iterations_at_tol": early_iters,
         "launch": solver.launch_config(),
     }
+    if not args.no_cantilever:
+        out["cantilever"] = bench_cantilever(dev)
     if not args.no_batched:
         out["batched"] = bench_batched(dev, args.batched_replays)
     adj_ctx = None

@@ -37,6 +37,8 @@ using namespace fem;
 namespace {
 
 constexpr int BLOCK = 256;
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
 // Per-instance words holding max|b| (reduced by the solve).
 constexpr int RHS_MAX_SLOTS = 64;
@@ -84,6 +86,11 @@ struct RowArgs {
   const float* hex_invJ;
   const float* hex_detw;
   const float* hex_mat;
+  // Per (hex, Gauss point) records of k_hex_gauss: the clamped F and the
+  // tangent coefficients (read by the assembly), the first Piola stress
+  // (read by the force evaluation).
+  const float* hex_gp_t;
+  const float* hex_gp_f;
   const float* mass;
   const float* inv_mass;
   const float* radius;
@@ -156,32 +163,65 @@ __device__ __forceinline__ void hex_gather(const int* __restrict__ idx, const fl
   }
 }
 
-__device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __restrict__ x,
-                                             const float* __restrict__ v, int e, int la) {
-  float px[8], py[8], pz[8], vx[8], vy[8], vz[8];
+// Per (hex, Gauss point) records, one thread each (eval_hexahedra
+// kernels.py:1481-1601 and build_system_matrix_hex_kernel :1609-1747 evaluate
+// F and the stretch clamp for every corner of every Gauss point; here each
+// is evaluated once and the element's eight corners read it).
+constexpr int HEX_GT = 12;  // tangent record: clamped F (9), k_mu s, coef, padding
+constexpr int HEX_GF = 9;   // force record: first Piola stress P
+
+template <bool FORCE>
+__global__ __launch_bounds__(BLOCK) void k_hex_gauss(RowArgs a, int H, const float* __restrict__ x,
+                                                     const float* __restrict__ v, float* __restrict__ out) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= 8 * H) return;
+  const int e = t >> 3, g = t & 7;
+  float px[8], py[8], pz[8];
   hex_gather(a.hex_idx + 8 * (size_t)e, x, px, py, pz);
-  hex_gather(a.hex_idx + 8 * (size_t)e, v, vx, vy, vz);
-  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1], k_damp = a.hex_mat[3 * e + 2];
-  const float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
-  v3 f = {0.f, 0.f, 0.f};
-  for (int g = 0; g < 8; ++g) {
-    float gx[8], gy[8], gz[8];
-    hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
-    float F[9], dF[9];
-    hex_F(px, py, pz, gx, gy, gz, F);
-    clamp_stretch(F, a.mn, a.mx);
-    hex_F(vx, vy, vz, gx, gy, gz, dF);
-    float ic = 0.f;
+  float gx[8], gy[8], gz[8];
+  hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
+  float F[9];
+  hex_F(px, py, pz, gx, gy, gz, F);
+  clamp_stretch(F, a.mn, a.mx);
+  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1];
+  float ic = 0.f;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
+  for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
+  if (!FORCE) {
+    const float ic1 = fmaxf(ic + 1.f, 1e-6f);
+    const float s = 1.f - 1.f / ic1;
+    const float coef = k_mu * 2.f / (ic1 * ic1);
+    f4v* o = reinterpret_cast<f4v*>(out + HEX_GT * (size_t)t);
+    o[0] = f4v{F[0], F[1], F[2], F[3]};
+    o[1] = f4v{F[4], F[5], F[6], F[7]};
+    o[2] = f4v{F[8], k_mu * s, coef, 0.f};
+  } else {
+    float vx[8], vy[8], vz[8];
+    hex_gather(a.hex_idx + 8 * (size_t)e, v, vx, vy, vz);
+    float dF[9];
+    hex_F(vx, vy, vz, gx, gy, gz, dF);
+    const float k_damp = a.hex_mat[3 * e + 2];
+    const float alpha = 1.f + k_mu / k_lambda - k_mu / (4.f * k_lambda);
     const float sdev = k_mu * (1.f - 1.f / (ic + 1.f));
     float cof[9];
     cofactor(F, cof);
     const float J = fmaxf(det3(F), 0.01f);
     const float vol = k_lambda * (J - alpha);
+    float* o = out + HEX_GF * (size_t)t;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = F[k] * sdev + dF[k] * k_damp + cof[k] * vol;
+  }
+}
+
+// Restoring force on corner la of hex e from its Gauss-point stresses.
+__device__ __forceinline__ v3 hex_node_force(const RowArgs& a, int e, int la) {
+  v3 f = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int g = 0; g < 8; ++g) {
+    const float* r = a.hex_gp_f + HEX_GF * (8 * (size_t)e + g);
     float P[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) P[k] = F[k] * sdev + dF[k] * k_damp + cof[k] * vol;
+    for (int k = 0; k < 9; ++k) P[k] = r[k];
     const float w = a.hex_detw[8 * (size_t)e + g];
     const v3 dNa = hex_grad_corner(a.hex_invJ + 72 * (size_t)e + 9 * g, g, la);
     f = f - mv(P, dNa) * w;
@@ -189,12 +229,10 @@ __device__ __forceinline__ v3 hex_node_force(const RowArgs& a, const float* __re
   return f;
 }
 
-// Row `la` of the Hex8 tangent: K_ab, b = 0..7, summed over Gauss points, x h^2.
-__device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* __restrict__ x, int e, int la,
-                                                float h2, float K[8][9]) {
-  float px[8], py[8], pz[8];
-  hex_gather(a.hex_idx + 8 * (size_t)e, x, px, py, pz);
-  const float k_mu = a.hex_mat[3 * e], k_lambda = a.hex_mat[3 * e + 1];
+// Row `la` of the Hex8 tangent: K_ab, b = 0..7, summed over Gauss points, x h^2,
+// from the Gauss-point records.
+__device__ __forceinline__ void hex_tangent_row(const RowArgs& a, int e, int la, float h2, float K[8][9]) {
+  const float k_lambda = a.hex_mat[3 * e + 1];
 #pragma unroll
   for (int b = 0; b < 8; ++b)
 #pragma unroll
@@ -202,15 +240,10 @@ __device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* _
   for (int g = 0; g < 8; ++g) {
     float gx[8], gy[8], gz[8];
     hex_grads(a.hex_invJ + 72 * (size_t)e + 9 * g, g, gx, gy, gz);
-    float F[9];
-    hex_F(px, py, pz, gx, gy, gz, F);
-    clamp_stretch(F, a.mn, a.mx);
-    float ic = 0.f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
-    const float ic1 = fmaxf(ic + 1.f, 1e-6f);
-    const float s = 1.f - 1.f / ic1;
-    const float coef = k_mu * 2.f / (ic1 * ic1);
+    const f4v* r = reinterpret_cast<const f4v*>(a.hex_gp_t + HEX_GT * (8 * (size_t)e + g));
+    const f4v r0 = r[0], r1 = r[1], r2 = r[2];
+    const float F[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+    const float c1 = r2.y, coef = r2.z;
     float cof[9];
     cofactor(F, cof);
     const float w = a.hex_detw[8 * (size_t)e + g];
@@ -220,7 +253,7 @@ __device__ __forceinline__ void hex_tangent_row(const RowArgs& a, const float* _
     for (int b = 0; b < 8; ++b) {
       const v3 dNb = {gx[b], gy[b], gz[b]};
       float blk[9];
-      tangent_block(blk, k_mu * s, coef, k_lambda, w, dNa, fa, ja, dNb, mv(F, dNb), mv(cof, dNb));
+      tangent_block(blk, c1, coef, k_lambda, w, dNa, fa, ja, dNb, mv(F, dNb), mv(cof, dNb));
 #pragma unroll
       for (int k = 0; k < 9; ++k) K[b][k] += blk[k];
     }
@@ -396,7 +429,7 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
   if (EL & 2) for (int t = threadIdx.x; t < nh; t += BLOCK) {
     const int code = a.hinc[hb + t];
     float K[8][9];
-    hex_tangent_row(a, x, code >> 3, code & 7, h2, K);
+    hex_tangent_row(a, code >> 3, code & 7, h2, K);
     float* o = st_h + ST_HEX * t;
 #pragma unroll
     for (int b = 0; b < 8; ++b)
@@ -549,7 +582,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
       const int h1 = a.hinc_ptr[i + 1];
       for (int t = a.hinc_ptr[i] + lane; t < h1; t += G) {
         const int code = a.hinc[t];
-        fh = fh + hex_node_force(a, x, v, code >> 3, code & 7);
+        fh = fh + hex_node_force(a, code >> 3, code & 7);
       }
     }
     if (mode == MODE_STEP) {
@@ -964,8 +997,6 @@ __device__ __forceinline__ bool spin_until(const unsigned* p, unsigned target, i
 // chunk of the instance has raised its flag.  No L2 write-back or invalidate
 // is needed, so read-only data (inverse diagonal, streamed blocks, column
 // lists) stays cached across iterations.
-typedef double d2v __attribute__((ext_vector_type(2)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f3v __attribute__((ext_vector_type(3), aligned(4)));
 // Four floats at 4-byte alignment (exchange records are packed at 36 bytes).
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
@@ -2135,6 +2166,7 @@ struct fem_system {
   int *d_tet_idx = nullptr, *d_hex_idx = nullptr;
   float *d_tet_pose = nullptr, *d_tet_mat = nullptr, *d_hex_invJ = nullptr, *d_hex_detw = nullptr,
         *d_hex_mat = nullptr;
+  float *d_hex_gp_t = nullptr, *d_hex_gp_f = nullptr;  // k_hex_gauss records (H > 0)
   float *d_mass = nullptr, *d_inv_mass = nullptr, *d_radius = nullptr, *d_anchor = nullptr;
   int* d_flags = nullptr;
   unsigned char* d_row_fixed = nullptr;
@@ -2324,6 +2356,8 @@ struct fem_system {
     a.hex_invJ = d_hex_invJ;
     a.hex_detw = d_hex_detw;
     a.hex_mat = d_hex_mat;
+    a.hex_gp_t = d_hex_gp_t;
+    a.hex_gp_f = d_hex_gp_f;
     a.mass = d_mass;
     a.inv_mass = d_inv_mass;
     a.radius = d_radius;
@@ -3090,6 +3124,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->upload(&s->d_hex_invJ, d->hex_inv_J0, 72 * (size_t)s->H)) ||
       (rc = s->upload(&s->d_hex_detw, d->hex_det_J0_w, 8 * (size_t)s->H)) ||
       (rc = s->upload(&s->d_hex_mat, d->hex_materials, 3 * (size_t)s->H)) ||
+      (s->H > 0 && ((rc = s->alloc(&s->d_hex_gp_t, HEX_GT * 8 * (size_t)s->H)) ||
+                    (rc = s->alloc(&s->d_hex_gp_f, HEX_GF * 8 * (size_t)s->H)))) ||
       (rc = s->upload(&s->d_mass, d->particle_mass, P)) ||
       (rc = s->upload(&s->d_inv_mass, d->particle_inv_mass, P)) ||
       (rc = s->upload(&s->d_radius, d->particle_radius, P)) ||
@@ -3244,6 +3280,24 @@ int fem_set_cg_variant(fem_system* s, int32_t variant) {
   return FEM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Gauss-point records of every hex at positions x (and velocities v for the
+// stresses), ahead of the kernels that read them on the same stream.
+template <bool FORCE>
+int hex_gauss(const fem_system* s, const RowArgs& ra, const float* x, const float* v, hipStream_t st) {
+  if (s->H == 0) return FEM_OK;
+  const int n = 8 * s->H;
+  hipLaunchKernelGGL(k_hex_gauss<FORCE>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, ra, s->H, x, v,
+                     FORCE ? s->d_hex_gp_f : s->d_hex_gp_t);
+  HIP_TRY(hipGetLastError());
+  return FEM_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int fem_restore_fixed(fem_system* s, float* q, float* qd, void* stream) {
   if (!s || !q) return fail(FEM_ERR_ARG, "null argument");
   if (s->nfixed == 0) return FEM_OK;
@@ -3260,6 +3314,7 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
   RowArgs ra = s->row_args();
+  if (int rc = hex_gauss<false>(s, ra, q, nullptr, (hipStream_t)stream)) return rc;
   int rc = dispatch_g<LaunchAssemble>(s->asm_rb, s->full(), ra, s->d_asm_groups, s->asm_ngroups, q, dt, s->b_vals,
                                       s->b_minv, s->asm_lds, (hipStream_t)stream);
   if (rc || !s->topo_count) return rc;
@@ -3278,7 +3333,9 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
                     void* stream) {
   if (!s || !q || !qd || !out) return fail(FEM_ERR_ARG, "null argument");
   if (s->P == 0) return FEM_OK;
+  int rc_hex = FEM_OK;
   RowArgs ra = s->row_args();
+  if ((terms & FEM_TERM_HEX) && (rc_hex = hex_gauss<true>(s, ra, q, qd, (hipStream_t)stream))) return rc_hex;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
@@ -3299,6 +3356,7 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
   RowArgs ra = s->row_args();
   const int terms = FEM_TERM_SPRING | FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_TRI | FEM_TERM_GROUND | FEM_TERM_GRAVITY |
                     FEM_TERM_DAMPING;
+  if (int rc = hex_gauss<true>(s, ra, q_in, qd_in, st)) return rc;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }
