@@ -42,7 +42,8 @@ enum {
   FEM_TERM_GRAVITY = 8,
   FEM_TERM_DAMPING = 16,
   FEM_TERM_SPRING = 32,
-  FEM_TERM_TRI = 64
+  FEM_TERM_TRI = 64,
+  FEM_TERM_PARTICLE = 128  /* hash-grid particle-particle contact (fem_set_particle_contact) */
 };
 
 /* Device-side solve status bits (fem_solve_stats.error_flags). */
@@ -132,6 +133,20 @@ int fem_bind(fem_system* sys, int32_t* offsets_dev, int32_t* columns_dev, float*
              float* increment_dev, int64_t block_capacity);
 
 int fem_set_params(fem_system* sys, const fem_params* params);
+
+/* Particle-particle contact, the "particle" term of _accumulate_forces
+ * (solver.py:570-589): SolverFEM.eval_particle_particle_forces
+ * (solver.py:1477-1510) builds a hash grid of cell width 2 max_radius +
+ * cohesion at the step's positions and eval_particle_forces
+ * (kernels.py:1069-1114, particle_force :1043-1066) sums penalty (ke),
+ * damping (kd) and Coulomb friction (kf, mu) over every active pair with
+ * distance - r_i - r_j <= cohesion.  Replaces the Model.particle_ke/kd/kf/mu/
+ * cohesion fields and Model.particle_grid: after this call every step and
+ * fem_eval_forces with FEM_TERM_PARTICLE evaluate the term, with the grid's
+ * storage allocated here (once).  ke <= 0, fewer than two particles or no
+ * positive radius switch the term off, as the reference skips it.  Radii
+ * and flags are the ones given to fem_create. */
+int fem_set_particle_contact(fem_system* sys, float ke, float kd, float kf, float mu, float cohesion);
 
 /* Gravity read from caller-owned device memory (float[3]) by every later
  * kernel instead of fem_params.gravity, so that a recorded graph follows

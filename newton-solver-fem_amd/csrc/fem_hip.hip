@@ -29,6 +29,7 @@
 
 #include "fem_hip.h"
 #include "fem_adjoint.hpp"
+#include "fem_contact.hpp"
 #include "fem_math.hpp"
 #include "fem_pattern.hpp"
 
@@ -91,6 +92,9 @@ struct RowArgs {
   // (read by the force evaluation).
   const float* hex_gp_t;
   const float* hex_gp_f;
+  // Particle-particle contact force per node (fem_contact), NULL when the
+  // system has no particle contact.
+  const float* fpart;
   const float* mass;
   const float* inv_mass;
   const float* radius;
@@ -609,6 +613,9 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     const bool on = (a.flags[i] & 1) != 0;
     const v3 xi = ld3(x, i), vi = ld3(v, i);
     v3 fg = {0.f, 0.f, 0.f}, fgr = {0.f, 0.f, 0.f}, fdm = {0.f, 0.f, 0.f};
+    // Particle-particle contact, evaluated over the hash grid before this
+    // kernel (zero for inactive particles).
+    const v3 fpc = (a.fpart && (terms & FEM_TERM_PARTICLE)) ? ld3(a.fpart, i) : v3{0.f, 0.f, 0.f};
     if ((terms & FEM_TERM_GROUND) && a.has_ground && on) {
       // eval_particle_ground_contacts kernels.py:1132-1195
       const v3 n = {a.gn[0], a.gn[1], a.gn[2]};
@@ -640,14 +647,15 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     v3 f;
     if (mode == MODE_STEP) {
       // _accumulate_forces order solver.py:570-589 (spring, tet, hex,
-      // triangle, ground, gravity, damping), then h*ext (:993-999), then
-      // subtract_elastic_predictor and restore_spring_damping_predictor
-      // implicit_predictor.py:9-41.
+      // triangle, particle, ground, gravity, damping), then h*ext
+      // (:993-999), then subtract_elastic_predictor and
+      // restore_spring_damping_predictor implicit_predictor.py:9-41.
       f = {0.f, 0.f, 0.f};
       if (EL & 4) f = f + fsp * h;
       f = f + ft * h;
       f = f + fh * h;
       if (EL & 4) f = f + ftri * h;
+      if (a.fpart) f = f + fpc * h;
       f = f + fg * h;
       f = f + fgr * h;
       f = f + fdm * h;
@@ -657,7 +665,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
       if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
       mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
     } else {
-      f = fsp + ft + fh + ftri + fg + fgr + fdm;
+      f = fsp + ft + fh + ftri + fpc + fg + fgr + fdm;
     }
     st3(out, i, f);
   }
@@ -2167,6 +2175,12 @@ struct fem_system {
   float *d_tet_pose = nullptr, *d_tet_mat = nullptr, *d_hex_invJ = nullptr, *d_hex_detw = nullptr,
         *d_hex_mat = nullptr;
   float *d_hex_gp_t = nullptr, *d_hex_gp_f = nullptr;  // k_hex_gauss records (H > 0)
+  // Particle-particle contact (fem_set_particle_contact): grid storage,
+  // coefficients, and whether the step evaluates the term.
+  ContactGrid contact;
+  ContactParams cparams{0.f, 0.f, 0.f, 0.f, 0.f};
+  bool contact_on = false;
+  float max_radius = 0.f;
   float *d_mass = nullptr, *d_inv_mass = nullptr, *d_radius = nullptr, *d_anchor = nullptr;
   int* d_flags = nullptr;
   unsigned char* d_row_fixed = nullptr;
@@ -2358,6 +2372,7 @@ struct fem_system {
     a.hex_mat = d_hex_mat;
     a.hex_gp_t = d_hex_gp_t;
     a.hex_gp_f = d_hex_gp_f;
+    a.fpart = contact_on ? contact.force : nullptr;
     a.mass = d_mass;
     a.inv_mass = d_inv_mass;
     a.radius = d_radius;
@@ -3138,6 +3153,8 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
       (rc = s->alloc(&s->d_stats, 4 * (size_t)s->ninst)) || (rc = s->alloc(&s->d_err, 1)) ||
       (rc = s->alloc(&s->d_norms, 2)) || (rc = s->alloc(&s->d_dstats, 4)))
     return cleanup(rc);
+  for (int i = 0; i < P; ++i)  // Model.particle_max_radius
+    if (std::isfinite(d->particle_radius[i])) s->max_radius = std::max(s->max_radius, d->particle_radius[i]);
   double avg_inc = P ? (double)(s->pt.tinc_ptr[P] + s->pt.hinc_ptr[P]) / P : 1.0;
   double avg_nnz = P ? (double)s->pt.offsets[P] / P : 1.0;
   (void)avg_nnz;
@@ -3170,6 +3187,39 @@ int fem_memory_stats(const fem_system* s, int64_t* allocations, int64_t* bytes) 
   }
   if (allocations) *allocations = (int64_t)s->allocs.size();
   if (bytes) *bytes = total;
+  return FEM_OK;
+}
+
+int fem_set_particle_contact(fem_system* s, float ke, float kd, float kf, float mu, float cohesion) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (!std::isfinite(ke) || !std::isfinite(kd) || !std::isfinite(kf) || !std::isfinite(mu) || !std::isfinite(cohesion) ||
+      cohesion < 0.f)
+    return fail(FEM_ERR_ARG, "particle contact coefficients must be finite and the cohesion distance nonnegative");
+  // The reference skips the term for ke <= 0, fewer than two particles or no
+  // positive radius (solver.py:1481-1486).
+  if (ke <= 0.f || s->P <= 1 || !(s->max_radius > 0.f)) {
+    s->contact_on = false;
+    return FEM_OK;
+  }
+  ContactGrid& g = s->contact;
+  if (!g.force) {
+    int bits = 6;
+    while (bits < 20 && (1 << bits) < 2 * s->P) ++bits;
+    const size_t tb = contact_temp_bytes(bits);
+    int rc;
+    unsigned char* temp = nullptr;
+    if ((rc = s->alloc(&g.keys, (size_t)s->P)) || (rc = s->alloc(&g.ids, (size_t)s->P)) ||
+        (rc = s->alloc(&temp, tb)) || (rc = s->alloc(&g.force, 3 * (size_t)s->P)))
+      return rc;
+    g.temp = temp;
+    g.P = s->P;
+    g.bits = bits;
+  }
+  // Cell width: the search radius 2 max_radius + cohesion (solver.py:1487).
+  g.max_radius = s->max_radius;
+  g.cell = 2.f * s->max_radius + cohesion;
+  s->cparams = ContactParams{ke, kd, kf, mu, cohesion};
+  s->contact_on = true;
   return FEM_OK;
 }
 
@@ -3294,6 +3344,14 @@ int hex_gauss(const fem_system* s, const RowArgs& ra, const float* x, const floa
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }
+
+// Particle-particle contact forces at (x, v) ahead of the force kernel.
+int particle_contact(const fem_system* s, const float* x, const float* v, hipStream_t st) {
+  if (!s->contact_on) return FEM_OK;
+  const hipError_t e = contact_forces(s->contact, s->cparams, x, v, s->d_radius, s->d_flags, st);
+  if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("particle contact: ") + hipGetErrorString(e));
+  return FEM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -3336,6 +3394,7 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
   int rc_hex = FEM_OK;
   RowArgs ra = s->row_args();
   if ((terms & FEM_TERM_HEX) && (rc_hex = hex_gauss<true>(s, ra, q, qd, (hipStream_t)stream))) return rc_hex;
+  if ((terms & FEM_TERM_PARTICLE) && (rc_hex = particle_contact(s, q, qd, (hipStream_t)stream))) return rc_hex;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
@@ -3354,9 +3413,10 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
     HIP_TRY(hipGetLastError());
   }
   RowArgs ra = s->row_args();
-  const int terms = FEM_TERM_SPRING | FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_TRI | FEM_TERM_GROUND | FEM_TERM_GRAVITY |
-                    FEM_TERM_DAMPING;
+  const int terms = FEM_TERM_SPRING | FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_TRI | FEM_TERM_PARTICLE |
+                    FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
   if (int rc = hex_gauss<true>(s, ra, q_in, qd_in, st)) return rc;
+  if (int rc = particle_contact(s, q_in, qd_in, st)) return rc;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }

@@ -10,6 +10,7 @@ from .model import (
     PARTICLE_FLAG_ACTIVE,
     Control,
     FEMBuilder,
+    HashGrid,
     LinearFEMData,
     Model,
     ModelBuilder,
@@ -24,6 +25,7 @@ __all__ = [
     "PARTICLE_FLAG_ACTIVE",
     "Control",
     "FEMBuilder",
+    "HashGrid",
     "LinearFEMData",
     "Model",
     "ModelBuilder",

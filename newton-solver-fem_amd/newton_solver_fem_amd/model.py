@@ -89,6 +89,12 @@ class Control:
         self.tet_activations = tet_activations
 
 
+class HashGrid:
+    """Marks a model whose particles collide with each other (Newton's
+    ``Model.particle_grid``, a Warp hash grid).  The device grid itself is
+    owned and rebuilt every step by the solver library."""
+
+
 class Model:
     """Finalized arrays on one torch device (``cuda`` for the HIP solver)."""
 
@@ -99,13 +105,22 @@ class Model:
         self.gravity_array = torch.zeros(3, dtype=torch.float32, device=self.device)
         self.gravity = gravity
         self.particle_count = int(len(host["particle_q"]))
-        # Hash grid of particle-particle contact.  Models built here are
-        # FEM meshes and carry none, so SolverFEM skips that force term, as
-        # the reference does when Model.particle_grid is None
-        # (solver.py:1477-1485, kernels.py:1032-1038).
+        # Hash grid of particle-particle contact.  None (the default of
+        # finalize) skips that force term, as the reference does when
+        # Model.particle_grid is None (solver.py:1477-1485,
+        # kernels.py:1032-1038); finalize(particle_grid=True) sets a
+        # HashGrid and SolverFEM then evaluates the term.  Contact
+        # coefficients follow Newton's Model fields; SolverFEM writes
+        # particle_ke / particle_kd from self_contact_ke / self_contact_kd.
         self.particle_grid = None
-        self.particle_ke = 0.0
-        self.particle_kd = 0.0
+        self.particle_ke = 1.0e3
+        self.particle_kd = 1.0e2
+        self.particle_kf = 1.0e2
+        self.particle_mu = 0.5
+        self.particle_cohesion = 0.0
+        r = np.asarray(host["particle_radius"], dtype=np.float64)
+        r = r[np.isfinite(r)]
+        self.particle_max_radius = float(r.max()) if r.size else 0.0
         self.tet_count = int(len(host["tet_indices"]))
         self.spring_count = int(len(host["spring_indices"]))
         self.tri_count = int(len(host["tri_indices"]))
@@ -350,8 +365,13 @@ class ModelBuilder:
             "hex_det_J0_w": np.zeros((0, 8), np.float32),
         }
 
-    def finalize(self, device="cuda", requires_grad: bool = False) -> Model:
-        return Model(device, self._host_arrays(), self.gravity)
+    def finalize(self, device="cuda", requires_grad: bool = False, particle_grid: bool = False) -> Model:
+        """``particle_grid=True`` gives the model a :class:`HashGrid`, so that
+        SolverFEM evaluates particle-particle contact."""
+        m = Model(device, self._host_arrays(), self.gravity)
+        if particle_grid:
+            m.particle_grid = HashGrid()
+        return m
 
 
 class FEMBuilder(ModelBuilder):
@@ -423,7 +443,7 @@ class FEMBuilder(ModelBuilder):
         self.fem_hex_indices.append(indices)
         self.fem_hex_materials.append(materials)
 
-    def finalize(self, device="cuda", requires_grad: bool = False) -> Model:
+    def finalize(self, device="cuda", requires_grad: bool = False, particle_grid: bool = False) -> Model:
         host = self._host_arrays()
         if self.fem_hex_indices:
             hexes = np.concatenate(self.fem_hex_indices).reshape(-1, 8)
@@ -432,7 +452,10 @@ class FEMBuilder(ModelBuilder):
             host["hex_materials"] = np.concatenate(self.fem_hex_materials).astype(np.float32).reshape(-1, 3)
             host["hex_inv_J0"] = inv.astype(np.float32)
             host["hex_det_J0_w"] = det.astype(np.float32)
-        return Model(device, host, self.gravity)
+        m = Model(device, host, self.gravity)
+        if particle_grid:
+            m.particle_grid = HashGrid()
+        return m
 
 
 def build_tet_model(scene, device="cuda", gravity=(0.0, 0.0, -9.81)) -> Model:

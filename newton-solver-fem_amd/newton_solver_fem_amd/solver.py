@@ -103,9 +103,8 @@ class SolverFEM:
     Constructor arguments follow the reference (solver.py:212-240).  The
     implemented linear solver is block-Jacobi preconditioned CG
     (``solver_type="cg"``, ``preconditioner_type="block_diag"``); other
-    Krylov methods / preconditioners, springs, triangles, particle
-    self-contact and soft contacts are outside this substep path and raise
-    ``NotImplementedError``.
+    Krylov methods / preconditioners and soft contacts are outside this
+    substep path and raise ``NotImplementedError``.
     """
 
     def __init__(
@@ -163,15 +162,17 @@ class SolverFEM:
             raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
         if torque_stiffness > 0.0 and spring_rest_direction is not None:
             raise NotImplementedError("spring bend torque is not part of this substep path")
-        # Particle-particle contact runs only for models with a hash grid
-        # (reference eval_particle_particle_forces solver.py:1477-1485); FEM
-        # meshes have none and skip the term whatever self_contact_ke is.
-        if (self_contact_ke > 0.0 and model.particle_count > 1 and getattr(model, "particle_grid", None) is not None
-                and float(np.max(model.host["particle_radius"], initial=0)) > 0):
-            raise NotImplementedError("hash-grid particle self-contact is not part of this substep path")
         # The reference stores the contact coefficients on the model (solver.py:266-267).
         model.particle_ke = float(self_contact_ke)
         model.particle_kd = float(self_contact_kd)
+        # Particle-particle contact runs only for models with a hash grid
+        # (reference eval_particle_particle_forces solver.py:1477-1486); FEM
+        # meshes built without one skip the term whatever self_contact_ke is.
+        self._particle_contact = (model.particle_ke > 0.0 and model.particle_count > 1
+                                  and getattr(model, "particle_grid", None) is not None
+                                  and getattr(model, "particle_max_radius", 0.0) > 0.0)
+        if self._particle_contact and differentiable:
+            raise NotImplementedError("differentiable steps with particle-particle contact are not supported")
         if model.device.type != "cuda":
             raise ValueError("SolverFEM runs on the HIP device: finalize the model with device='cuda'")
         if tet_volume_weights is not None:
@@ -280,6 +281,10 @@ class SolverFEM:
         self.last_update = {}
         self._topology_storage = self._topology_signature()
         _lib.check(self._lib.fem_bind_gravity(self._sys, model.gravity_array.data_ptr()), "fem_bind_gravity")
+        if self._particle_contact:
+            _lib.check(self._lib.fem_set_particle_contact(
+                self._sys, model.particle_ke, model.particle_kd, float(model.particle_kf), float(model.particle_mu),
+                float(model.particle_cohesion)), "fem_set_particle_contact")
         self._push_params()
         # Construction-time rest tangent (solver.py:324-341).
         _lib.check(self._lib.fem_assemble(self._sys, model.particle_q.data_ptr(), self.dt, _stream(dev)), "fem_assemble")
@@ -773,6 +778,13 @@ class SolverFEM:
 
     def eval_hexahedral_forces(self, model, control, state):
         return self._eval(_lib.FEM_TERM_HEX, state, control)
+
+    def eval_particle_particle_forces(self, model, control, state):
+        """Hash-grid particle-particle penalty + Coulomb friction [N] (zero
+        when the model has no particle grid, reference solver.py:1477-1510)."""
+        if not self._particle_contact:
+            return torch.zeros_like(state.particle_q)
+        return self._eval(_lib.FEM_TERM_PARTICLE, state)
 
     def eval_particle_ground_contact_forces(self, model, control, state):
         return self._eval(_lib.FEM_TERM_GROUND, state)

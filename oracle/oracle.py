@@ -99,6 +99,42 @@ def block_inverse(B, bits=64):
     return out.reshape(3, 3)
 
 
+def particle_contact_forces(q, qd, radius, flags, ke, kd, kf, mu, cohesion=0.0):
+    """Particle-particle contact force on every particle, FP64, all pairs.
+
+    Restates eval_particle_forces (kernels.py:1069-1114) with particle_force
+    (kernels.py:1043-1066): for active i and every other active j at distance
+    d >= 1e-10 with gap err = d - r_i - r_j <= cohesion, n = (x_i - x_j)/d,
+    relative velocity v = v_i - v_j: fn = err ke + min(n.v, 0) kd, tangential
+    vt = v - n (n.v), ft = min(|vt| kf, mu |fn|), f_i += -n fn - vt/|vt| ft.
+    The hash grid only selects candidate pairs (every pair that can satisfy
+    the gap test lies in neighbouring cells), so an all-pairs sum in ascending
+    j is the same set of terms.
+    """
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 3)
+    qd = np.asarray(qd, dtype=np.float64).reshape(-1, 3)
+    r = np.asarray(radius, dtype=np.float64)
+    on = (np.asarray(flags) & 1) != 0
+    out = np.zeros_like(q)
+    for i in np.nonzero(on)[0]:
+        n = q[i] - q
+        d = np.sqrt(np.sum(n * n, axis=1))
+        err = d - r[i] - r
+        sel = on & (np.arange(len(q)) != i) & (d >= 1e-10) & (err <= cohesion)
+        if not sel.any():
+            continue
+        n = n[sel] / d[sel, None]
+        v = qd[i] - qd[sel]
+        vn = np.sum(n * v, axis=1)
+        fn = err[sel] * ke + np.minimum(vn, 0.0) * kd
+        vt = v - n * vn[:, None]
+        vs = np.sqrt(np.sum(vt * vt, axis=1))
+        vt = np.where(vs[:, None] > 0, vt / np.where(vs > 0, vs, 1.0)[:, None], vt)
+        ft = np.minimum(vs * kf, mu * np.abs(fn))
+        out[i] = np.sum(-n * fn[:, None] - vt * ft[:, None], axis=0)
+    return out
+
+
 class OracleSolver:
     """CPU restatement of ``SolverFEM`` for one model.
 
