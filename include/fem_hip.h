@@ -148,6 +148,20 @@ int fem_set_params(fem_system* sys, const fem_params* params);
  * and flags are the ones given to fem_create. */
 int fem_set_particle_contact(fem_system* sys, float ke, float kd, float kf, float mu, float cohesion);
 
+/* Linear solver of the step, SolverFEM(solver_type=..., preconditioner_type=
+ * ...) (solver.py:212-218, _make_linear_solver :1219-1252, _build_
+ * preconditioner :1142-1173).  method: 0 cg (the persistent PCG, default),
+ * 1 bicgstab, 2 cr, 3 gmres (restarted after `restart` iterations, 1..64;
+ * warp.optim.linear's default is 31); preconditioner: 0 block_diag (default),
+ * 1 diag, 2 diag_abs (inverse of the diagonal entries of A's diagonal blocks,
+ * or of their magnitudes; 1 where zero or absent), 3 id (none).  The
+ * methods besides cg run a fixed sequence of launches per iteration with
+ * residual-masked updates (no host synchronisation, graph-capturable).
+ * Batched instances, reserved pools, component scaling and
+ * fem_step_solve_static need cg with block_diag.  Allocates the work
+ * storage; call between steps. */
+int fem_set_solver(fem_system* sys, int32_t method, int32_t preconditioner, int32_t restart);
+
 /* Gravity read from caller-owned device memory (float[3]) by every later
  * kernel instead of fem_params.gravity, so that a recorded graph follows
  * changes made between replays: the reference's Model.gravity is a device

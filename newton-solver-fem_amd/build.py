@@ -15,8 +15,9 @@ PKG_ROOT = Path(__file__).resolve().parent
 REPO = PKG_ROOT.parent
 CSRC = PKG_ROOT / "csrc"
 LIB = PKG_ROOT / "newton_solver_fem_amd" / "libfem_hip.so"
-SOURCES = [CSRC / "fem_hip.hip", CSRC / "fem_contact.hip", CSRC / "fem_pattern.cpp"]
-HEADERS = [CSRC / "fem_math.hpp", CSRC / "fem_pattern.hpp", CSRC / "fem_contact.hpp", REPO / "include" / "fem_hip.h"]
+SOURCES = [CSRC / "fem_hip.hip", CSRC / "fem_contact.hip", CSRC / "fem_krylov.hip", CSRC / "fem_pattern.cpp"]
+HEADERS = [CSRC / "fem_math.hpp", CSRC / "fem_pattern.hpp", CSRC / "fem_contact.hpp", CSRC / "fem_krylov.hpp",
+           REPO / "include" / "fem_hip.h"]
 ARCH = os.environ.get("FEM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -30,6 +30,7 @@
 #include "fem_hip.h"
 #include "fem_adjoint.hpp"
 #include "fem_contact.hpp"
+#include "fem_krylov.hpp"
 #include "fem_math.hpp"
 #include "fem_pattern.hpp"
 
@@ -2181,6 +2182,14 @@ struct fem_system {
   ContactParams cparams{0.f, 0.f, 0.f, 0.f, 0.f};
   bool contact_on = false;
   float max_radius = 0.f;
+  // Linear solver of the step (fem_set_solver): Krylov method, point-Jacobi
+  // preconditioner blocks and the work storage of the methods besides PCG.
+  int kr_method = KR_CG, kr_precond = KP_BLOCK, kr_restart = 31;
+  float* d_pjac = nullptr;  // [P][9] when kr_precond != KP_BLOCK
+  float* d_kr_work = nullptr;
+  double *d_kr_part = nullptr, *d_kr_sc = nullptr;
+  // Blocks of the configured preconditioner (M in z = M r).
+  const float* precond_blocks() const { return kr_precond == KP_BLOCK ? b_minv : d_pjac; }
   float *d_mass = nullptr, *d_inv_mass = nullptr, *d_radius = nullptr, *d_anchor = nullptr;
   int* d_flags = nullptr;
   unsigned char* d_row_fixed = nullptr;
@@ -3223,6 +3232,47 @@ int fem_set_particle_contact(fem_system* s, float ke, float kd, float kf, float 
   return FEM_OK;
 }
 
+int fem_set_solver(fem_system* s, int32_t method, int32_t preconditioner, int32_t restart) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (method < KR_CG || method > KR_GMRES) return fail(FEM_ERR_ARG, "unknown Krylov method");
+  if (preconditioner < KP_BLOCK || preconditioner > KP_ID) return fail(FEM_ERR_ARG, "unknown preconditioner");
+  if (restart < 1 || restart > KR_MAX_RESTART) return fail(FEM_ERR_ARG, "GMRES restart must be in [1, 64]");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if ((method != KR_CG || preconditioner != KP_BLOCK) && (s->ninst > 1 || s->topo_count || s->cs_scale))
+    return fail(FEM_ERR_UNSUPPORTED,
+                "batched instances, reserved pools and component scaling run the block-Jacobi PCG only");
+  int rc;
+  if (preconditioner != KP_BLOCK && !s->d_pjac && (rc = s->alloc(&s->d_pjac, 9 * (size_t)std::max(s->P, 1))))
+    return rc;
+  if (method != KR_CG) {
+    const size_t wf = krylov_work_floats(s->P, method, restart);
+    if (!s->holds(s->d_kr_work, sizeof(float) * wf)) {
+      float* w = nullptr;
+      if ((rc = s->alloc(&w, std::max<size_t>(wf, 1)))) return rc;  // zero-filled
+      s->drop(s->d_kr_work);
+      s->d_kr_work = w;
+    }
+    const size_t pd = krylov_part_doubles(s->P, restart);
+    if (!s->holds(s->d_kr_part, sizeof(double) * pd)) {
+      double* p = nullptr;
+      if ((rc = s->alloc(&p, pd))) return rc;
+      s->drop(s->d_kr_part);
+      s->d_kr_part = p;
+    }
+    if (!s->d_kr_sc && (rc = s->alloc(&s->d_kr_sc, (size_t)KR_NSC))) return rc;
+  }
+  s->kr_method = method;
+  s->kr_precond = preconditioner;
+  s->kr_restart = restart;
+  if (preconditioner != KP_BLOCK && s->P > 0) {
+    const hipError_t e = krylov_point_jacobi(s->P, s->d_offsets, s->d_diag, s->b_vals, preconditioner, s->d_pjac,
+                                             nullptr);
+    if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("point-Jacobi preconditioner: ") + hipGetErrorString(e));
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  return FEM_OK;
+}
+
 int fem_pattern_size(const fem_system* s, int64_t* nnz, int32_t* max_row_blocks) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   if (nnz) *nnz = s->pt.offsets[s->P];
@@ -3375,6 +3425,12 @@ int fem_assemble(fem_system* s, const float* q, float dt, void* stream) {
   if (int rc = hex_gauss<false>(s, ra, q, nullptr, (hipStream_t)stream)) return rc;
   int rc = dispatch_g<LaunchAssemble>(s->asm_rb, s->full(), ra, s->d_asm_groups, s->asm_ngroups, q, dt, s->b_vals,
                                       s->b_minv, s->asm_lds, (hipStream_t)stream);
+  if (!rc && s->kr_precond != KP_BLOCK) {
+    // preconditioner(A, "diag" / "diag_abs" / "id") of the refreshed matrix (solver.py:1183-1185)
+    const hipError_t e = krylov_point_jacobi(s->P, s->d_offsets, s->d_diag, s->b_vals, s->kr_precond, s->d_pjac,
+                                             (hipStream_t)stream);
+    if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("point-Jacobi preconditioner: ") + hipGetErrorString(e));
+  }
   if (rc || !s->topo_count) return rc;
   // Positions of the current tangent (streaming.py:603, :716), compared with
   // by the next topology update.
@@ -3431,7 +3487,7 @@ CgArgs cg_args(const fem_system* s) {
   ca.offsets = s->compact ? s->d_coffsets : s->d_offsets;
   ca.cols = s->compact ? s->d_ccols : s->d_cols;
   ca.vals = s->compact ? s->d_cvals : s->b_vals;
-  ca.minv = s->b_minv;
+  ca.minv = s->precond_blocks();
   ca.row_fixed = s->d_row_fixed;
   ca.dv = s->b_dv;
   ca.anchor = s->d_anchor;
@@ -3489,12 +3545,55 @@ size_t static_scratch_layout(const fem_system* s, size_t* flags_off, size_t* xw_
   return part + flags + sizeof(float) * 2 * XW * (size_t)s->P;
 }
 
+// The configured Krylov method besides PCG on the step's system with
+// right-hand side `rhs` and warm start / increment `dv` (no state update
+// unless the caller sets q_out).
+KrylovArgs krylov_args(const fem_system* s, const float* rhs, float* dv) {
+  KrylovArgs ka;
+  std::memset(&ka, 0, sizeof ka);
+  ka.P = s->P;
+  ka.offsets = s->d_offsets;
+  ka.cols = s->d_cols;
+  ka.vals = s->b_vals;
+  ka.row_fixed = s->d_row_fixed;
+  ka.M = s->kr_precond == KP_ID ? nullptr : s->precond_blocks();
+  ka.rhs = rhs;
+  ka.rhs_max = s->d_rhs_max;
+  ka.rhs_max_slots = RHS_MAX_SLOTS;
+  ka.dv = dv;
+  ka.anchor = s->d_anchor;
+  ka.tolerance = s->prm.tolerance;
+  ka.maxiter = s->prm.maxiter;
+  ka.restart = std::max(1, std::min(s->kr_restart, s->prm.maxiter));
+  ka.work = s->d_kr_work;
+  ka.part = s->d_kr_part;
+  ka.sc = s->d_kr_sc;
+  ka.stats = s->d_stats;
+  return ka;
+}
+
+int krylov_launch(const fem_system* s, const KrylovArgs& ka, hipStream_t st) {
+  const hipError_t e = krylov_solve(ka, s->kr_method, st);
+  if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("krylov solve: ") + hipGetErrorString(e));
+  return FEM_OK;
+}
+
 int step_solve(fem_system* s, const float* q_in, const float* qd_in, const float* f_in, float* q_out, float* qd_out,
                float dt, void* stream, bool use_static) {
   if (!s || !q_in || !qd_in || !f_in || !q_out || !qd_out) return fail(FEM_ERR_ARG, "null argument");
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (s->P == 0) return FEM_OK;
+  if (s->kr_method != KR_CG) {
+    if (use_static) return fail(FEM_ERR_UNSUPPORTED, "the static-scratch solve runs the block-Jacobi PCG only");
+    KrylovArgs ka = krylov_args(s, f_in, s->b_dv);
+    ka.q_in = q_in;
+    ka.qd_in = qd_in;
+    ka.q_out = q_out;
+    ka.qd_out = qd_out;
+    ka.h = dt;
+    return krylov_launch(s, ka, (hipStream_t)stream);
+  }
   CgArgs ca = cg_args(s);
   if (use_static) {
     size_t flags_off = 0, xw_off = 0;
@@ -3599,7 +3698,7 @@ int fem_apply_preconditioner(fem_system* s, const float* x, float* y, void* stre
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (s->P == 0) return FEM_OK;
   hipLaunchKernelGGL(k_apply_minv, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
-                     (const float*)s->b_minv, x, y, s->P);
+                     s->precond_blocks(), x, y, s->P);
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }
@@ -3617,7 +3716,7 @@ int fem_block_jacobi_mv(fem_system* s, const float* x, const float* y, float* z,
   if (!s->b_minv) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (s->P == 0) return FEM_OK;
   hipLaunchKernelGGL(k_block_jacobi_mv, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream,
-                     (const float*)s->b_minv, x, y, z, alpha, beta, s->P);
+                     s->precond_blocks(), x, y, z, alpha, beta, s->P);
   HIP_TRY(hipGetLastError());
   return FEM_OK;
 }
@@ -3654,6 +3753,7 @@ int fem_solve_increment(fem_system* s, const float* b, float* dv, void* stream) 
   hipLaunchKernelGGL(k_absmax, dim3((s->P + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, b, s->P,
                      s->ninst > 1 ? (const int*)s->d_row_inst : (const int*)nullptr, s->d_rhs_max);
   HIP_TRY(hipGetLastError());
+  if (s->kr_method != KR_CG) return krylov_launch(s, krylov_args(s, b, dv), st);
   CgArgs ca = cg_args(s);
   ca.rhs = b;
   ca.dv = dv;
@@ -3749,6 +3849,8 @@ int fem_prepare_topology(fem_system* s, int32_t original_node_count) {
   if (s->topo_count) return fail(FEM_ERR_STATE, "Tetrahedral updates are already prepared");
   if (s->cs_scale) return fail(FEM_ERR_STATE, "Prepare tetrahedral updates before component scaling");
   if (s->nfixed) return fail(FEM_ERR_ARG, "Streaming topology is not supported with fixed_particle_indices");
+  if (s->kr_precond != KP_BLOCK) return fail(FEM_ERR_ARG, "Persistent topology updates require the block_diag preconditioner");
+  if (s->kr_method != KR_CG) return fail(FEM_ERR_UNSUPPORTED, "Persistent topology updates run the block-Jacobi PCG only");
   if (s->T < 1 || s->H || s->S || s->R) return fail(FEM_ERR_ARG, "Streaming assembly requires tetrahedral elements only");
   if (s->ninst > 1) return fail(FEM_ERR_UNSUPPORTED, "topology updates of a batched system are not supported");
   const int T = s->T, P = s->P;
@@ -4052,6 +4154,8 @@ int fem_bind_component_scaling(fem_system* s, int32_t capacity, double* totals, 
   if (!s || !totals || !scale || !values || !inv_diag || !rhs) return fail(FEM_ERR_ARG, "null buffer");
   if (s->cs_scale) return fail(FEM_ERR_STATE, "Component scaling is already prepared");
   if (s->nfixed) return fail(FEM_ERR_ARG, "Component scaling is not supported with fixed_particle_indices");
+  if (s->kr_precond != KP_BLOCK) return fail(FEM_ERR_ARG, "Component scaling requires block_diag preconditioning");
+  if (s->kr_method != KR_CG) return fail(FEM_ERR_UNSUPPORTED, "Component scaling runs the block-Jacobi PCG only");
   if (capacity < 1) return fail(FEM_ERR_ARG, "component_capacity must be a positive integer");
   if (s->ninst > 1) return fail(FEM_ERR_UNSUPPORTED, "component scaling of a batched system is not supported");
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");

@@ -22,8 +22,10 @@ import torch
 from . import _lib
 from .model import Control, Model, State
 
-SUPPORTED_SOLVER = "cg"
-SUPPORTED_PRECONDITIONER = "block_diag"
+# solver_type / preconditioner_type -> fem_set_solver codes (solver.py:1142-1173, :1219-1252).
+SOLVER_TYPES = {"cg": 0, "bicgstab": 1, "cr": 2, "gmres": 3}
+PRECONDITIONER_TYPES = {"block_diag": 0, "diag": 1, "diag_abs": 2, "id": 3}
+GMRES_RESTART = 31  # warp.optim.linear.gmres default
 
 
 class BlockJacobiOperator:
@@ -101,10 +103,10 @@ class SolverFEM:
     """Linearly implicit backward-Euler FEM solver (Tet4 + Hex8, SNH).
 
     Constructor arguments follow the reference (solver.py:212-240).  The
-    implemented linear solver is block-Jacobi preconditioned CG
-    (``solver_type="cg"``, ``preconditioner_type="block_diag"``); other
-    Krylov methods / preconditioners and soft contacts are outside this
-    substep path and raise ``NotImplementedError``.
+    linear solve runs ``solver_type`` ``"cg"`` (the persistent PCG),
+    ``"bicgstab"``, ``"cr"`` or ``"gmres"`` with ``preconditioner_type``
+    ``"block_diag"``, ``"diag"``, ``"diag_abs"`` or ``"id"``; soft contacts
+    are outside this substep path and raise ``NotImplementedError``.
     """
 
     def __init__(
@@ -153,11 +155,12 @@ class SolverFEM:
             raise ValueError("fixed_particle_indices is not supported with differentiable=True")
         if not (isinstance(dt, (int, float)) and math.isfinite(dt) and dt > 0):
             raise ValueError("dt must be finite and positive")
-        if solver_type != SUPPORTED_SOLVER or preconditioner_type != SUPPORTED_PRECONDITIONER:
-            raise NotImplementedError(
-                "this substep implements solver_type='cg' with preconditioner_type='block_diag'; "
-                f"got solver_type={solver_type!r}, preconditioner_type={preconditioner_type!r}"
-            )
+        if solver_type not in SOLVER_TYPES:
+            raise ValueError(f"Invalid solver type: {solver_type}")
+        if preconditioner_type not in PRECONDITIONER_TYPES:
+            raise ValueError(f"Unsupported preconditioner type {preconditioner_type}")
+        if instance_offsets is not None and (solver_type, preconditioner_type) != ("cg", "block_diag"):
+            raise NotImplementedError("batched instances run solver_type='cg' with preconditioner_type='block_diag'")
         if differentiable and model.tri_count and np.any(model.host["tri_materials"][:, 3:5] != 0):
             raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
         if torque_stiffness > 0.0 and spring_rest_direction is not None:
@@ -281,6 +284,10 @@ class SolverFEM:
         self.last_update = {}
         self._topology_storage = self._topology_signature()
         _lib.check(self._lib.fem_bind_gravity(self._sys, model.gravity_array.data_ptr()), "fem_bind_gravity")
+        if (solver_type, preconditioner_type) != ("cg", "block_diag"):
+            _lib.check(self._lib.fem_set_solver(self._sys, SOLVER_TYPES[solver_type],
+                                                PRECONDITIONER_TYPES[preconditioner_type], GMRES_RESTART),
+                       "fem_set_solver")
         if self._particle_contact:
             _lib.check(self._lib.fem_set_particle_contact(
                 self._sys, model.particle_ke, model.particle_kd, float(model.particle_kf), float(model.particle_mu),
@@ -360,8 +367,10 @@ class SolverFEM:
             raise RuntimeError("Tetrahedral updates are already prepared")
         if self._component_scaling is not None:
             raise RuntimeError("Prepare tetrahedral updates before component scaling")
-        if self.preconditioner_type != SUPPORTED_PRECONDITIONER:
-            raise ValueError("Streaming assembly requires the block_diag preconditioner")
+        if self.preconditioner_type != "block_diag":
+            raise ValueError("Persistent topology updates require the block_diag preconditioner")
+        if self.solver_type != "cg":
+            raise NotImplementedError("persistent topology updates run solver_type='cg'")
         torch.cuda.synchronize(self.device)
         _lib.check(self._lib.fem_prepare_topology(self._sys, int(original_node_count)), "prepare_tetrahedral_updates")
         self._streaming = True
@@ -449,8 +458,10 @@ class SolverFEM:
             raise RuntimeError("Prepare component scaling before capturing or stepping the solver")
         if self._component_scaling is not None:
             raise RuntimeError("Component scaling is already prepared")
-        if self.preconditioner_type != SUPPORTED_PRECONDITIONER:
+        if self.preconditioner_type != "block_diag":
             raise ValueError("Component scaling requires block_diag preconditioning")
+        if self.solver_type != "cg":
+            raise NotImplementedError("component scaling runs solver_type='cg'")
         if not isinstance(component_capacity, int) or isinstance(component_capacity, bool) or component_capacity < 1:
             raise ValueError("component_capacity must be a positive integer")
         P = self.model.particle_count

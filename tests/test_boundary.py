@@ -85,11 +85,13 @@ def test_fixed_with_differentiable_rejected():
         _solver(cube(), fixed_particle_indices=(0,), differentiable=True)
 
 
-def test_unsupported_krylov_methods_raise():
-    with pytest.raises(NotImplementedError):
-        _solver(cube(), solver_type="bicgstab")
-    with pytest.raises(NotImplementedError):
-        _solver(cube(), preconditioner_type="diag")
+def test_unknown_krylov_methods_and_preconditioners_raise():
+    """Reference _make_linear_solver solver.py:1222-1224 ('Invalid solver type')
+    and warp's preconditioner() for an unknown ptype."""
+    with pytest.raises(ValueError, match="Invalid solver type"):
+        _solver(cube(), solver_type="minres")
+    with pytest.raises(ValueError, match="preconditioner type"):
+        _solver(cube(), preconditioner_type="ilu")
 
 
 def test_host_model_is_rejected_loudly():
