@@ -148,6 +148,15 @@ int fem_set_params(fem_system* sys, const fem_params* params);
  * and flags are the ones given to fem_create. */
 int fem_set_particle_contact(fem_system* sys, float ke, float kd, float kf, float mu, float cohesion);
 
+/* Spring bend torque, SolverFEM(torque_stiffness, torque_damping,
+ * spring_rest_direction) (solver.py:227-229, :390-401; the torque part of
+ * eval_springs_linear_and_torque kernels.py:222-292): rest_direction is a
+ * HOST float[spring_count][3] copied here (springs whose direction has
+ * length < 0.5 get no torque); stiffness <= 0 (or a NULL direction array)
+ * switches the torque off.  The spring term of every later force evaluation
+ * adds the torque force; it has no tangent block, as in the reference. */
+int fem_set_spring_torque(fem_system* sys, const float* rest_direction, float stiffness, float damping);
+
 /* Linear solver of the step, SolverFEM(solver_type=..., preconditioner_type=
  * ...) (solver.py:212-218, _make_linear_solver :1219-1252, _build_
  * preconditioner :1142-1173).  method: 0 cg (the persistent PCG, default),

@@ -109,6 +109,8 @@ struct RowArgs {
   const float* sp_rest;
   const float* sp_ke;
   const float* sp_kd;
+  const float* sp_dir;   // [S][3] bend-torque rest directions (NULL: no bend torque)
+  float sp_kt, sp_kw;    // bend-torque stiffness and damping
   const int* rinc_ptr;
   const int* rinc;       // (tri << 2) | corner
   const int* tri_idx;    // [R][3]
@@ -554,8 +556,12 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
         const int p = a.sp_idx[2 * sp], q = a.sp_idx[2 * sp + 1];
         const v3 vp = ld3(v, p), vq = ld3(v, q);
         const float kd = a.sp_kd[sp];
-        if (terms & FEM_TERM_SPRING)
-          fsp = fsp + spring_end_force(ld3(x, p), ld3(x, q), vp, vq, a.sp_rest[sp], a.sp_ke[sp], kd, end);
+        if (terms & FEM_TERM_SPRING) {
+          const v3 xp = ld3(x, p), xq = ld3(x, q);
+          fsp = fsp + spring_end_force(xp, xq, vp, vq, a.sp_rest[sp], a.sp_ke[sp], kd, end);
+          if (a.sp_dir)
+            fsp = fsp + spring_torque_end_force(xp, xq, vp, vq, ld3(a.sp_dir, sp), a.sp_kt, a.sp_kw, end);
+        }
         srest = srest + (end == 0 ? vp - vq : vq - vp) * (h * kd);
       }
     }
@@ -2182,6 +2188,10 @@ struct fem_system {
   ContactParams cparams{0.f, 0.f, 0.f, 0.f, 0.f};
   bool contact_on = false;
   float max_radius = 0.f;
+  // Spring bend torque (fem_set_spring_torque).
+  float* d_sp_dir = nullptr;
+  bool torque_on = false;
+  float sp_kt = 0.f, sp_kw = 0.f;
   // Linear solver of the step (fem_set_solver): Krylov method, point-Jacobi
   // preconditioner blocks and the work storage of the methods besides PCG.
   int kr_method = KR_CG, kr_precond = KP_BLOCK, kr_restart = 31;
@@ -2394,6 +2404,9 @@ struct fem_system {
     a.sp_rest = d_sp_rest;
     a.sp_ke = d_sp_ke;
     a.sp_kd = d_sp_kd;
+    a.sp_dir = torque_on ? d_sp_dir : nullptr;
+    a.sp_kt = sp_kt;
+    a.sp_kw = sp_kw;
     a.rinc_ptr = R > 0 ? d_rinc_ptr : nullptr;
     a.rinc = d_rinc;
     a.tri_idx = d_tri_idx;
@@ -3229,6 +3242,29 @@ int fem_set_particle_contact(fem_system* s, float ke, float kd, float kf, float 
   g.cell = 2.f * s->max_radius + cohesion;
   s->cparams = ContactParams{ke, kd, kf, mu, cohesion};
   s->contact_on = true;
+  return FEM_OK;
+}
+
+int fem_set_spring_torque(fem_system* s, const float* rest_direction, float stiffness, float damping) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (!std::isfinite(stiffness) || !std::isfinite(damping))
+    return fail(FEM_ERR_ARG, "torque stiffness and damping must be finite");
+  // The reference applies the torque only for stiffness > 0 (solver.py:1442).
+  if (!(stiffness > 0.f) || !rest_direction || s->S == 0) {
+    s->torque_on = false;
+    return FEM_OK;
+  }
+  for (int k = 0; k < 3 * s->S; ++k)
+    if (!std::isfinite(rest_direction[k])) return fail(FEM_ERR_ARG, "spring_rest_direction must be finite");
+  int rc;
+  if (!s->d_sp_dir) {
+    if ((rc = s->upload(&s->d_sp_dir, rest_direction, 3 * (size_t)s->S))) return rc;
+  } else if ((rc = s->write(s->d_sp_dir, rest_direction, 3 * (size_t)s->S))) {
+    return rc;
+  }
+  s->sp_kt = stiffness;
+  s->sp_kw = damping;
+  s->torque_on = true;
   return FEM_OK;
 }
 

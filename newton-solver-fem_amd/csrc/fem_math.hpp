@@ -295,6 +295,33 @@ FEM_DI v3 spring_end_force(v3 xp, v3 xq, v3 vp, v3 vq, float rest, float ke, flo
   return end == 0 ? fs * -1.f : fs;
 }
 
+// Bend torque of a spring toward its rest direction on end `end` (0: p, 1: q)
+// (eval_springs_linear_and_torque kernels.py:222-292, torque part): with
+// d = (x_p - x_q)/len and d0 = -rest_dir (|d0| < 0.5: no torque), theta =
+// acos(d.d0), axis = d0 x d / |d0 x d|, omega = (d x d_dot).axis with
+// d_dot = (v_pq - d (d.v_pq))/len, tau = -(k_t theta + k_w omega) len,
+// F = (axis tau) x d / len, applied as -F at p and +F at q.
+FEM_DI v3 spring_torque_end_force(v3 xp, v3 xq, v3 vp, v3 vq, v3 rest_dir, float kt, float kw, int end) {
+  const v3 xpq = xp - xq, vpq = vp - vq;
+  const float len = sqrtf(dot(xpq, xpq));
+  if (len < 1.0e-6f) return {0.f, 0.f, 0.f};
+  const v3 d = {xpq.x / len, xpq.y / len, xpq.z / len};
+  const v3 d0 = rest_dir * -1.f;
+  if (sqrtf(dot(d0, d0)) < 0.5f) return {0.f, 0.f, 0.f};
+  const float theta = acosf(fminf(fmaxf(dot(d, d0), -1.f), 1.f));
+  v3 axis = cross(d0, d);
+  const float rl = sqrtf(dot(axis, axis));
+  if (rl < 1.0e-6f) return {0.f, 0.f, 0.f};
+  axis = {axis.x / rl, axis.y / rl, axis.z / rl};
+  const v3 dd = vpq - d * dot(d, vpq);
+  const v3 d_dot = {dd.x / len, dd.y / len, dd.z / len};
+  const float omega = dot(cross(d, d_dot), axis);
+  const float tau = -(kt * theta + kw * omega) * len;
+  const v3 t = cross(axis * tau, d);
+  const v3 F = {t.x / len, t.y / len, t.z / len};
+  return end == 0 ? F * -1.f : F;
+}
+
 FEM_DI v3 normalized(v3 a) {
   const float l = sqrtf(dot(a, a));
   return l > 0.f ? a * (1.f / l) : v3{0.f, 0.f, 0.f};

@@ -83,6 +83,7 @@ SIGNATURES = {
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "fem_set_particle_contact": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 5),
     "fem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "fem_set_spring_torque": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_float, ctypes.c_float]),
     "fem_bind_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "fem_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fem_get_launch": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip]),

@@ -163,8 +163,20 @@ class SolverFEM:
             raise NotImplementedError("batched instances run solver_type='cg' with preconditioner_type='block_diag'")
         if differentiable and model.tri_count and np.any(model.host["tri_materials"][:, 3:5] != 0):
             raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
-        if torque_stiffness > 0.0 and spring_rest_direction is not None:
-            raise NotImplementedError("spring bend torque is not part of this substep path")
+        # Spring bend torque (reference solver.py:390-401, eval_spring_forces :1436-1476).
+        rest_dir = None
+        if spring_rest_direction is not None:
+            src = spring_rest_direction
+            if isinstance(src, torch.Tensor):
+                src = src.detach().cpu().numpy()
+            rest_dir = np.ascontiguousarray(src, dtype=np.float32)
+            if rest_dir.shape != (model.spring_count, 3):
+                raise ValueError(f"spring_rest_direction shape {rest_dir.shape} != "
+                                 f"(spring_count={model.spring_count}, 3)")
+        torque_active = (float(torque_stiffness) > 0.0 and rest_dir is not None
+                         and bool(np.any(np.linalg.norm(rest_dir, axis=1) >= 0.5)))
+        if torque_active and differentiable:
+            raise NotImplementedError("differentiable steps with spring bend torque are not supported")
         # The reference stores the contact coefficients on the model (solver.py:266-267).
         model.particle_ke = float(self_contact_ke)
         model.particle_kd = float(self_contact_kd)
@@ -202,6 +214,10 @@ class SolverFEM:
         self.last_cg_iters: int | None = None
         self.last_differentiation_residuals = None
         self.dt = float(dt)
+        self.torque_stiffness = float(torque_stiffness)
+        self.torque_damping = float(torque_damping)
+        self.spring_rest_direction = torch.as_tensor(
+            rest_dir if rest_dir is not None else np.zeros((model.spring_count, 3), np.float32), device=model.device)
         self._ground_plane = None if ground_plane is None else tuple(float(g) for g in ground_plane)
         self._ground = (float(ground_ke), float(ground_kd), float(ground_kf), float(ground_mu))
         self._tolerance = float(linear_solver_tolerance)
@@ -288,6 +304,9 @@ class SolverFEM:
             _lib.check(self._lib.fem_set_solver(self._sys, SOLVER_TYPES[solver_type],
                                                 PRECONDITIONER_TYPES[preconditioner_type], GMRES_RESTART),
                        "fem_set_solver")
+        if torque_active:
+            _lib.check(self._lib.fem_set_spring_torque(self._sys, fptr("sdir", rest_dir), self.torque_stiffness,
+                                                       self.torque_damping), "fem_set_spring_torque")
         if self._particle_contact:
             _lib.check(self._lib.fem_set_particle_contact(
                 self._sys, model.particle_ke, model.particle_kd, float(model.particle_kf), float(model.particle_mu),

@@ -135,6 +135,46 @@ def particle_contact_forces(q, qd, radius, flags, ke, kd, kf, mu, cohesion=0.0):
     return out
 
 
+def spring_torque_forces(q, qd, spring_indices, rest_direction, kt, kw):
+    """Bend-torque part of eval_springs_linear_and_torque (kernels.py:222-292), FP64.
+
+    Per spring (i, j) with d = (x_i - x_j)/L and d0 = -rest_direction (no
+    torque when |d0| < 0.5, L < 1e-6 or |d0 x d| < 1e-6): theta = acos(d.d0),
+    axis = (d0 x d)/|d0 x d|, omega = (d x d_dot).axis with
+    d_dot = (v_ij - d (d.v_ij))/L, tau = -(kt theta + kw omega) L and
+    F = (axis tau) x d / L, applied as -F on i and +F on j.
+    """
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 3)
+    qd = np.asarray(qd, dtype=np.float64).reshape(-1, 3)
+    idx = np.asarray(spring_indices).reshape(-1, 2)
+    rd = np.asarray(rest_direction, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(q)
+    for s, (i, j) in enumerate(idx):
+        if i < 0 or j < 0:
+            continue
+        xij, vij = q[i] - q[j], qd[i] - qd[j]
+        L = np.linalg.norm(xij)
+        if L < 1e-6:
+            continue
+        d = xij / L
+        d0 = -rd[s]
+        if np.linalg.norm(d0) < 0.5:
+            continue
+        theta = np.arccos(np.clip(d @ d0, -1.0, 1.0))
+        axis = np.cross(d0, d)
+        rl = np.linalg.norm(axis)
+        if rl < 1e-6:
+            continue
+        axis /= rl
+        d_dot = (vij - d * (d @ vij)) / L
+        omega = np.cross(d, d_dot) @ axis
+        tau = -(kt * theta + kw * omega) * L
+        F = np.cross(axis * tau, d) / L
+        out[i] -= F
+        out[j] += F
+    return out
+
+
 class OracleSolver:
     """CPU restatement of ``SolverFEM`` for one model.
 
