@@ -46,7 +46,8 @@ constexpr long BARRIER_SPIN_LIMIT = 1L << 22;
 constexpr int RHS_MAX_SLOTS = 64;
 // CG partial sums per chunk and phase.
 constexpr int NPART = 8;
-enum { PT_PAP = 0, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR, PT_BB };
+// Slot 7 holds b.b after the prologue and r.p in every phase.
+enum { PT_PAP = 0, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR, PT_BB, PT_RP = PT_BB };
 
 thread_local std::string g_error;
 
@@ -777,9 +778,12 @@ __global__ __launch_bounds__(BLOCK) void k_vjp_rows(RowArgs a, const float* __re
         const int e = code >> 2, c = code & 3;
         const int n0 = a.tri_idx[3 * e], n1 = a.tri_idx[3 * e + 1], n2 = a.tri_idx[3 * e + 2];
         v3 gx10, gx20, gv10, gv20;
-        tri_edge_vjp(ld3(x, n0), ld3(x, n1), ld3(x, n2), ld3(v, n0), ld3(v, n1), ld3(v, n2), ld3(z, n0) * h,
-                     ld3(z, n1) * h, ld3(z, n2) * h, a.tri_pose + 4 * (size_t)e, a.tri_mat + 5 * (size_t)e, gx10, gx20,
-                     gv10, gv20);
+        const v3 x0 = ld3(x, n0), x1 = ld3(x, n1), x2 = ld3(x, n2), v0 = ld3(v, n0), v1 = ld3(v, n1),
+                 v2 = ld3(v, n2), w0 = ld3(z, n0) * h, w1 = ld3(z, n1) * h, w2 = ld3(z, n2) * h;
+        const float* mat = a.tri_mat + 5 * (size_t)e;
+        tri_edge_vjp(x0, x1, x2, v0, v1, v2, w0, w1, w2, a.tri_pose + 4 * (size_t)e, mat, gx10, gx20, gv10, gv20);
+        // drag and lift: the same velocity gradient for every corner
+        vb = vb + tri_aero_vjp(x0, x1, x2, v0, v1, v2, w0 + w1 + w2, mat, gx10, gx20);
         if (c == 0) {
           xb = xb - gx10 - gx20;
           vb = vb - gv10 - gv20;
@@ -1512,13 +1516,14 @@ __device__ __forceinline__ double residual_floor() { return 2.6442191937624087e-
 // (alpha_k = rz_k / p_k.Ap_k, beta_k = rz_{k+1} / rz_k, residual-masked
 // updates), reorganised so that the reduction and the exchange of the SpMV
 // operand share one synchronisation point.  Phase k (after the barrier):
-//   alpha_{k-1} from rz_{k-1}, p_{k-1}.q_{k-1};
+//   alpha_{k-1} = r_{k-1}.p_{k-1} / p_{k-1}.q_{k-1} (the exact line search;
+//   r.p = r.z in exact arithmetic);
 //   rz_k = rz_{k-1} - 2 alpha q.z + alpha^2 q.w      (w = M^-1 q; M^-1 is
 //   rr_k = rr_{k-1} - 2 alpha q.r + alpha^2 q.q       symmetric) give beta_{k-1};
 //   x += alpha p, r -= alpha q, z -= alpha w, p = z + beta p   (own rows);
 //   q = A p, gathering neighbour p from the exchange records (z, w, p) of
 //   phase k-1; w = M^-1 q and z = M^-1 r afresh; publish (z, w, p) and the partial sums
-//   p.q, q.z, q.w, q.r, q.q, r.z, r.r; barrier.
+//   p.q, q.z, q.w, q.r, q.q, r.z, r.r, r.p; barrier.
 // The direct r.z and r.r of phase k replace the recurrence values at phase
 // k+1, so recurrence error does not accumulate.  Exchange records and
 // partial sums are double buffered: phase k writes buffer (k+1)&1, which no
@@ -1731,10 +1736,10 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
     const int cur = ph & 1;
     float alpha = 0.f, beta = 0.f;
     if (ph > 0) {
-      double g[7];
+      double g[8];
       {
-        const int sl[7] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR};
-        if (!wait_sum<NT, 7>(a.flags, a.part + NPART * nch * cur, cb, ce, epoch, sl, g, red, a.err, a.spin_limit)) {
+        const int sl[8] = {PT_PAP, PT_QZ, PT_QW, PT_QR, PT_QQ, PT_RZ, PT_RR, PT_RP};
+        if (!wait_sum<NT, 8>(a.flags, a.part + NPART * nch * cur, cb, ce, epoch, sl, g, red, a.err, a.spin_limit)) {
           failed = true;
           break;
         }
@@ -1744,7 +1749,10 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
       const bool live = rr > atol_sq;
       if (!live && a.early_exit) break;
       if (live) ++iters;
-      alpha = (live && g[0] != 0.0) ? (float)(rz / g[0]) : 0.f;
+      // Exact line search along p: alpha = r.p / p.Ap (= r.z / p.Ap in exact
+      // arithmetic).  With beta from the r.z recurrence, r.z / p.Ap lets a
+      // solve that reached the fp32 noise level drift away again.
+      alpha = (live && g[0] != 0.0) ? (float)(g[7] / g[0]) : 0.f;
       const double al = (double)alpha;
       const double rr_next = rr - 2.0 * al * g[3] + al * al * g[4];
       const double rz_next = rz - 2.0 * al * g[1] + al * al * g[2];
@@ -1765,7 +1773,7 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
     }
     if (ph == a.maxiter) break;
     spmv(std::false_type{}, alpha, beta, a.xw + XW * (size_t)a.P * cur);
-    float pp[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float pp[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float* xw_next = a.xw + XW * (size_t)a.P * (cur ^ 1);
     // Inverse diagonal blocks not held in registers: all rows' loads are
     // issued together (rows past the chunk load its first row, unused).
@@ -1813,6 +1821,7 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
         pp[4] += dot_cg(qq, qq);
         pp[5] += dot_cg(rv, zv);
         pp[6] += dot_cg(rv, rv);
+        pp[7] += dot_cg(rv, pv);
       }
     }
     if constexpr (RES) {
@@ -1822,15 +1831,15 @@ __device__ __forceinline__ void cg_chunk(const CgArgs& a, const int c, unsigned 
       asm volatile("" ::: "memory");
     }
     {
-      double ws[7];
-      wg_sum<NT, 7>(pp, ws, red);
+      double ws[8];
+      wg_sum<NT, 8>(pp, ws, red);
       static_assert(PT_PAP == 0 && PT_QZ == 1 && PT_QW == 2 && PT_QR == 3 && PT_RR == 6, "partial-sum slots");
       if (threadIdx.x == 0) {
         double* dst = a.part + NPART * (nch * (cur ^ 1) + c);
         st_wt(dst, d2v{ws[0], ws[1]});
         st_wt(dst + 2, d2v{ws[2], ws[3]});
         st_wt(dst + 4, d2v{ws[4], ws[5]});
-        st_wt(dst + 6, d2v{ws[6], 0.0});
+        st_wt(dst + 6, d2v{ws[6], ws[7]});
       }
     }
     publish(a.flags + c, ++epoch);
@@ -2213,7 +2222,6 @@ struct fem_system {
   const float* d_gravity = nullptr;  // caller-owned device gravity (fem_bind_gravity)
   // Kernels with Hex8, spring or triangle code (EL_ANY) are needed.
   bool full() const { return H > 0 || S > 0 || R > 0; }
-  bool tri_aero = false;  // a triangle with drag or lift (no reverse products for those)
   // solve scratch
   float* d_xw = nullptr;
   unsigned *d_rhs_max = nullptr, *d_cgflags = nullptr;
@@ -3119,7 +3127,6 @@ int fem_create(const fem_model_desc* d, fem_system** out) {
     // kernels.py:994-1008).
     std::vector<float> row_tri(s->R > 0 ? P : 0, 0.f);
     for (int t = 0; t < s->R; ++t) {
-      s->tri_aero = s->tri_aero || d->tri_materials[5 * (size_t)t + 3] != 0.f || d->tri_materials[5 * (size_t)t + 4] != 0.f;
       const float* D = d->tri_poses + 4 * (size_t)t;
       const float inv_area = (D[0] * D[3] - D[1] * D[2]) * 2.f;
       if (!(inv_area > 0.f)) continue;
@@ -3816,8 +3823,6 @@ int fem_vjp(fem_system* s, const float* q, const float* qd, const float* act, co
   if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
   if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
   if (mat_bar && (!dv || !q_asm)) return fail(FEM_ERR_ARG, "material gradients need dv and the assembly positions");
-  if (s->tri_aero)
-    return fail(FEM_ERR_UNSUPPORTED, "reverse products of triangle drag and lift are not provided");
   if (s->P == 0) return FEM_OK;
   hipStream_t st = (hipStream_t)stream;
   RowArgs ra = s->row_args();

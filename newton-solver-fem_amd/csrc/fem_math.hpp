@@ -413,6 +413,49 @@ FEM_DI void tri_edge_vjp(v3 x0, v3 x1, v3 x2, v3 v0, v3 v1, v3 v2, v3 w0, v3 w1,
   gv20 = gv20 + cr * (kd * hA);
 }
 
+// Reverse of the drag and lift terms of tri_corner_force: every corner
+// receives -(f_drag + f_lift) with f_drag = k_drag A |n.u| u and f_lift =
+// k_lift A asin(clamp(n.u/|u|)) (u.u) n, u = 0.3333 (v0 + v1 + v2), so for
+// corner weights w_c the term is L = -W.(f_drag + f_lift), W = w0 + w1 + w2.
+// Adds the gradients with respect to the edges x10, x20 (node 0 receives
+// minus their sum) and returns the gradient with respect to each corner's
+// velocity (the same for all three).  The clamp passes no gradient where it
+// binds, like the acos adjoint at |c| = 1.
+FEM_DI v3 tri_aero_vjp(v3 x0, v3 x1, v3 x2, v3 v0, v3 v1, v3 v2, v3 W, const float* __restrict__ mat, v3& gx10,
+                       v3& gx20) {
+  const float k_drag = mat[3], k_lift = mat[4];
+  const v3 zero = {0.f, 0.f, 0.f};
+  if (k_drag == 0.f && k_lift == 0.f) return zero;
+  const v3 x10 = x1 - x0, x20 = x2 - x0;
+  const v3 N = cross(x10, x20);
+  const float ln = sqrtf(dot(N, N));
+  if (!(ln > 0.f)) return zero;  // degenerate triangle: no aerodynamic force
+  const v3 n = N * (1.f / ln);
+  const float area = ln * 0.5f;
+  const v3 u = (v0 + v1 + v2) * 0.3333f;
+  const float sn = dot(n, u), Wu = dot(W, u), Wn = dot(W, n);
+  const float sg = sn > 0.f ? 1.f : (sn < 0.f ? -1.f : 0.f);
+  v3 gu = (n * (sg * Wu) + W * fabsf(sn)) * (-k_drag * area);
+  v3 gn = u * (sg * Wu * -k_drag * area);
+  float garea = -k_drag * fabsf(sn) * Wu;
+  const float uu = dot(u, u), ul = sqrtf(uu);
+  if (k_lift != 0.f && ul > 0.f) {
+    const v3 uh = u * (1.f / ul);
+    const float craw = dot(n, uh);
+    const float c = fminf(fmaxf(craw, -1.f), 1.f);
+    const float th = 1.5707963267948966f - acosf(c);
+    const float dd = 1.f - c * c;
+    const float dth = (dd > 0.f && fabsf(craw) < 1.f) ? 1.f / sqrtf(dd) : 0.f;
+    gu = gu + ((n - uh * c) * (uu * dth / ul) + u * (2.f * th)) * (-k_lift * area * Wn);
+    gn = gn + (uh * (dth * Wn) + W * th) * (-k_lift * area * uu);
+    garea += -k_lift * th * uu * Wn;
+  }
+  const v3 gN = n * (garea * 0.5f) + (gn - n * dot(n, gn)) * (1.f / ln);
+  gx10 = gx10 + cross(x20, gN);
+  gx20 = gx20 + cross(gN, x10);
+  return gu * 0.3333f;
+}
+
 // Volumetric activation force on node a (eval_tet_activations kernels.py:484-508).
 FEM_DI v3 tet_activation_force(v3 x0, v3 x1, v3 x2, v3 x3, float lam, float act, int a) {
   const float scale = lam * act / 6.f;

@@ -161,8 +161,6 @@ class SolverFEM:
             raise ValueError(f"Unsupported preconditioner type {preconditioner_type}")
         if instance_offsets is not None and (solver_type, preconditioner_type) != ("cg", "block_diag"):
             raise NotImplementedError("batched instances run solver_type='cg' with preconditioner_type='block_diag'")
-        if differentiable and model.tri_count and np.any(model.host["tri_materials"][:, 3:5] != 0):
-            raise NotImplementedError("differentiable steps with triangle drag or lift are not supported")
         # Spring bend torque (reference solver.py:390-401, eval_spring_forces :1436-1476).
         rest_dir = None
         if spring_rest_direction is not None:

@@ -356,3 +356,59 @@ def test_forward_is_independent_of_interleaved_large_adjoint_solves():
         loss, _ = _two_step_loss(model, s, q0, v0, act)
         (loss * seed).backward()
         assert forward() == pytest.approx(baseline, rel=1e-7, abs=0)
+
+
+def test_step_gradients_with_triangle_drag_and_lift_match_finite_differences():
+    """Triangle membranes with drag and lift (eval_triangles kernels.py:876-970): the
+    gradient of a one-step loss w.r.t. the input positions and velocities vs FP64
+    central differences of the oracle's differentiable step (fixed matrix), along
+    random directions; the oracle's forward includes the aerodynamic terms."""
+    print("case diff drag lift", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    b = ModelBuilder(gravity=(0.0, 0.0, -9.81))
+    n = 5
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 2) * 0.2
+    pts = np.column_stack([g, np.zeros(len(g)) + 0.3])
+    b.add_particles(pts, np.zeros_like(pts), 0.002, radius=0.0)
+    tris = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a0, a1, a2, a3 = i * n + j, (i + 1) * n + j, (i + 1) * n + j + 1, i * n + j + 1
+            tris += [(a0, a1, a2), (a0, a2, a3)]
+    # soft, large triangles: drag and lift carry 12-80% of the force terms' gradient here
+    b.add_triangles(tris, tri_ke=2.0, tri_ka=2.0, tri_kd=0.05, tri_drag=3.0, tri_lift=5.0)
+    model = b.finalize(DEV)
+    kw = dict(dt=DT, linear_damping=0.2)
+    s = SolverFEM(model, solver_type="cg", preconditioner_type="block_diag", self_contact_ke=0.0, differentiable=True,
+                  linear_solver_maxiter=128, **kw)
+    o = oracle.OracleSolver(model.host, bits=64, gravity=model.gravity, **kw)
+    rng = np.random.default_rng(17)
+    q0 = model.host["particle_q"].astype(np.float64) + rng.normal(0, 0.002, pts.shape)
+    v0 = rng.normal(0, 0.5, pts.shape) + (0.3, -0.2, 0.6)  # wind across and through the sheet
+    gq_out, gv_out = rng.normal(size=pts.shape), rng.normal(size=pts.shape)
+    tq = torch.tensor(q0, dtype=torch.float32, device=DEV, requires_grad=True)
+    tv = torch.tensor(v0, dtype=torch.float32, device=DEV, requires_grad=True)
+    st_in, st_out = model.state(), model.state()
+    st_in.particle_q, st_in.particle_qd = tq, tv
+    s.step(st_in, st_out, None, None, DT)
+    qo, vo, _ = o.diff_step(q0, v0)
+    assert rel(np_(st_out.particle_qd), vo) < 1e-4
+    loss = (st_out.particle_q * torch.tensor(gq_out, dtype=torch.float32, device=DEV)).sum() + (
+        st_out.particle_qd * torch.tensor(gv_out, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    gq, gv = np_(tq.grad), np_(tv.grad)
+
+    def L(q, v):
+        a, c, _ = o.diff_step(q, v)
+        return float(np.sum(a * gq_out) + np.sum(c * gv_out))
+
+    eps = 1e-6
+    for k in range(3):
+        dq, dvd = rng.normal(size=pts.shape), rng.normal(size=pts.shape)
+        fd_q = (L(q0 + eps * dq, v0) - L(q0 - eps * dq, v0)) / (2 * eps)
+        fd_v = (L(q0, v0 + eps * dvd) - L(q0, v0 - eps * dvd)) / (2 * eps)
+        # subtract the identity pass-through (q_out = q_in + h v_out) so the force terms are compared
+        base_q, base_v = float(np.sum(gq_out * dq)), float(np.sum((gv_out + DT * gq_out) * dvd))
+        assert abs(float(np.sum(gq * dq)) - fd_q) <= 2e-3 * max(abs(fd_q - base_q), 1e-12) + 1e-6 * abs(fd_q), k
+        assert abs(float(np.sum(gv * dvd)) - fd_v) <= 2e-3 * max(abs(fd_v - base_v), 1e-12) + 1e-6 * abs(fd_v), k
