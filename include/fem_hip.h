@@ -43,7 +43,8 @@ enum {
   FEM_TERM_DAMPING = 16,
   FEM_TERM_SPRING = 32,
   FEM_TERM_TRI = 64,
-  FEM_TERM_PARTICLE = 128  /* hash-grid particle-particle contact (fem_set_particle_contact) */
+  FEM_TERM_PARTICLE = 128, /* hash-grid particle-particle contact (fem_set_particle_contact) */
+  FEM_TERM_SOFT = 256      /* particle-shape soft contacts of a contact list (fem_set_soft_contacts) */
 };
 
 /* Device-side solve status bits (fem_solve_stats.error_flags). */
@@ -147,6 +148,23 @@ int fem_set_params(fem_system* sys, const fem_params* params);
  * positive radius switch the term off, as the reference skips it.  Radii
  * and flags are the ones given to fem_create. */
 int fem_set_particle_contact(fem_system* sys, float ke, float kd, float kf, float mu, float cohesion);
+
+/* Soft contacts, the "soft_contact" term of _accumulate_forces
+ * (solver.py:570-589): SolverFEM.eval_soft_contact_forces (solver.py:1540-1566)
+ * launches eval_soft_contacts (kernels.py:1199-1252) over the collision
+ * pipeline's contacts.soft_contact_* arrays with Model.soft_contact_ke/kd/kf/mu
+ * and the particle radii.  All arrays are DEVICE pointers that stay bound
+ * until the next call: count (int[1], read on the device at every
+ * evaluation, so a recorded graph follows it), particle (int[max_contacts];
+ * negative entries are skipped), body_pos, body_vel and normal
+ * (float[max_contacts][3], normal pointing from the body to the particle).
+ * Every later step and fem_eval_forces with FEM_TERM_SOFT add the term;
+ * scratch for the largest max_contacts seen is allocated here.  A NULL count
+ * or max_contacts <= 0 switches the term off.  Each particle sums its
+ * contacts in contact order (the reference adds them with atomics). */
+int fem_set_soft_contacts(fem_system* sys, const int* count, const int* particle, const float* body_pos,
+                          const float* body_vel, const float* normal, int max_contacts, float ke, float kd, float kf,
+                          float mu);
 
 /* Spring bend torque, SolverFEM(torque_stiffness, torque_damping,
  * spring_rest_direction) (solver.py:227-229, :390-401; the torque part of

@@ -185,6 +185,103 @@ size_t contact_temp_bytes(int bits) {
   return sizeof(int) * (3 * n + 1 + SCAN_NT);
 }
 
+namespace {
+// Contacts per particle (entries past the device count or with a particle
+// outside 0 .. P-1 take no part).
+__global__ __launch_bounds__(NT) void k_sc_count(const int* __restrict__ count, int max_contacts,
+                                                 const int* __restrict__ particle, int P, int* per) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  if (k >= max_contacts || k >= *count) return;
+  const int p = particle[k];
+  if (p >= 0 && p < P) atomicAdd(per + p, 1);
+}
+
+__global__ __launch_bounds__(NT) void k_sc_scatter(const int* __restrict__ count, int max_contacts,
+                                                   const int* __restrict__ particle, int P, int* cursor,
+                                                   int* __restrict__ ids) {
+  const int k = blockIdx.x * NT + threadIdx.x;
+  if (k >= max_contacts || k >= *count) return;
+  const int p = particle[k];
+  if (p >= 0 && p < P) ids[atomicAdd(cursor + p, 1)] = k;
+}
+
+// One thread per particle: its contacts put in ascending order, then their
+// forces summed in that order (eval_soft_contacts kernels.py:1222-1252).
+__global__ __launch_bounds__(NT) void k_sc_forces(const float* __restrict__ x, const float* __restrict__ v,
+                                                  const float* __restrict__ radius, int P,
+                                                  const int* __restrict__ start, const int* __restrict__ per,
+                                                  int* __restrict__ ids, const float* __restrict__ body_pos,
+                                                  const float* __restrict__ body_vel,
+                                                  const float* __restrict__ normal, float ke, float kd, float kf,
+                                                  float mu, float* __restrict__ force) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= P) return;
+  const int b = start[i], e = b + per[i];
+  for (int k = b + 1; k < e; ++k) {
+    const int c = ids[k];
+    int m = k - 1;
+    while (m >= b && ids[m] > c) {
+      ids[m + 1] = ids[m];
+      --m;
+    }
+    ids[m + 1] = c;
+  }
+  v3 f = {0.f, 0.f, 0.f};
+  if (e > b) {
+    const v3 xi = ld3(x, i), vi = ld3(v, i);
+    const float r = radius[i];
+    for (int k = b; k < e; ++k) {
+      const int c = ids[k];
+      const v3 n = ld3(normal, c);
+      // Penetration along the normal (body -> particle).
+      const float pen = r - dot(xi - ld3(body_pos, c), n);
+      if (pen > 0.f) {
+        const v3 rel = vi - ld3(body_vel, c);
+        const float vn = dot(rel, n);
+        const v3 vt = rel - n * vn;
+        const float fn = ke * pen - kd * fminf(vn, 0.f);
+        const float vt_mag = sqrtf(dot(vt, vt));
+        if (vt_mag > 1.0e-6f) {
+          const float ft = fminf(kf * vt_mag, mu * fmaxf(0.f, fn));
+          f = f + (n * fn - vt * (ft / vt_mag));
+        } else {
+          f = f + n * fn;
+        }
+      }
+    }
+  }
+  st3(force, i, f);
+}
+}  // namespace
+
+size_t soft_contact_temp_bytes(int P, int max_contacts) {
+  return sizeof(int) * (3 * (size_t)P + 1 + SCAN_NT + (size_t)std::max(max_contacts, 1));
+}
+
+hipError_t soft_contact_forces(const SoftContacts& c, const float* x, const float* v, const float* radius,
+                               hipStream_t st) {
+  if (c.P == 0) return hipSuccess;
+  const int P = c.P;
+  int* per = static_cast<int*>(c.temp);
+  int* start = per + P;        // [P + 1]
+  int* cursor = start + P + 1; // [P]
+  int* bsum = cursor + P;      // [SCAN_NT]
+  int* ids = bsum + SCAN_NT;   // [max_contacts]
+  const int nbk = (P + SCAN_NT - 1) / SCAN_NT;
+  const int gp = (P + NT - 1) / NT, gc = (std::max(c.max_contacts, 1) + NT - 1) / NT;
+  hipLaunchKernelGGL(k_pc_clear, dim3(gp), dim3(NT), 0, st, per, P);
+  if (c.max_contacts > 0)
+    hipLaunchKernelGGL(k_sc_count, dim3(gc), dim3(NT), 0, st, c.count, c.max_contacts, c.particle, P, per);
+  hipLaunchKernelGGL(k_pc_scan_blocks, dim3(nbk), dim3(SCAN_NT), 0, st, (const int*)per, P, start, bsum);
+  hipLaunchKernelGGL(k_pc_scan_sums, dim3(1), dim3(SCAN_NT), 0, st, bsum, nbk);
+  hipLaunchKernelGGL(k_pc_scan_add, dim3(nbk), dim3(SCAN_NT), 0, st, start, P, (const int*)bsum, cursor, 0);
+  if (c.max_contacts > 0)
+    hipLaunchKernelGGL(k_sc_scatter, dim3(gc), dim3(NT), 0, st, c.count, c.max_contacts, c.particle, P, cursor, ids);
+  hipLaunchKernelGGL(k_sc_forces, dim3(gp), dim3(NT), 0, st, x, v, radius, P, (const int*)start, (const int*)per, ids,
+                     c.body_pos, c.body_vel, c.normal, c.ke, c.kd, c.kf, c.mu, c.force);
+  return hipGetLastError();
+}
+
 hipError_t contact_forces(const ContactGrid& g, const ContactParams& p, const float* x, const float* v,
                           const float* radius, const int* flags, hipStream_t st) {
   if (g.P == 0) return hipSuccess;

@@ -97,6 +97,8 @@ struct RowArgs {
   // Particle-particle contact force per node (fem_contact), NULL when the
   // system has no particle contact.
   const float* fpart;
+  // Soft (particle-shape) contact force per node, NULL without a contact list.
+  const float* fsoft;
   const float* mass;
   const float* inv_mass;
   const float* radius;
@@ -624,6 +626,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     // Particle-particle contact, evaluated over the hash grid before this
     // kernel (zero for inactive particles).
     const v3 fpc = (a.fpart && (terms & FEM_TERM_PARTICLE)) ? ld3(a.fpart, i) : v3{0.f, 0.f, 0.f};
+    const v3 fsc = (a.fsoft && (terms & FEM_TERM_SOFT)) ? ld3(a.fsoft, i) : v3{0.f, 0.f, 0.f};
     if ((terms & FEM_TERM_GROUND) && a.has_ground && on) {
       // eval_particle_ground_contacts kernels.py:1132-1195
       const v3 n = {a.gn[0], a.gn[1], a.gn[2]};
@@ -655,7 +658,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     v3 f;
     if (mode == MODE_STEP) {
       // _accumulate_forces order solver.py:570-589 (spring, tet, hex,
-      // triangle, particle, ground, gravity, damping), then h*ext
+      // triangle, particle, ground, soft contact, gravity, damping), then h*ext
       // (:993-999), then subtract_elastic_predictor and
       // restore_spring_damping_predictor implicit_predictor.py:9-41.
       f = {0.f, 0.f, 0.f};
@@ -665,6 +668,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
       if (EL & 4) f = f + ftri * h;
       if (a.fpart) f = f + fpc * h;
       f = f + fg * h;
+      if (a.fsoft) f = f + fsc * h;
       f = f + fgr * h;
       f = f + fdm * h;
       if (ext) f = f + ld3(ext, i) * h;
@@ -673,7 +677,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
       if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
       mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
     } else {
-      f = fsp + ft + fh + ftri + fpc + fg + fgr + fdm;
+      f = fsp + ft + fh + ftri + fpc + fg + fsc + fgr + fdm;
     }
     st3(out, i, f);
   }
@@ -2196,6 +2200,9 @@ struct fem_system {
   ContactGrid contact;
   ContactParams cparams{0.f, 0.f, 0.f, 0.f, 0.f};
   bool contact_on = false;
+  SoftContacts soft;  // particle-shape contact list (fem_set_soft_contacts)
+  bool soft_on = false;
+  size_t soft_temp_cap = 0;
   float max_radius = 0.f;
   // Spring bend torque (fem_set_spring_torque).
   float* d_sp_dir = nullptr;
@@ -2400,6 +2407,7 @@ struct fem_system {
     a.hex_gp_t = d_hex_gp_t;
     a.hex_gp_f = d_hex_gp_f;
     a.fpart = contact_on ? contact.force : nullptr;
+    a.fsoft = soft_on ? soft.force : nullptr;
     a.mass = d_mass;
     a.inv_mass = d_inv_mass;
     a.radius = d_radius;
@@ -3252,6 +3260,48 @@ int fem_set_particle_contact(fem_system* s, float ke, float kd, float kf, float 
   return FEM_OK;
 }
 
+int fem_set_soft_contacts(fem_system* s, const int* count, const int* particle, const float* body_pos,
+                          const float* body_vel, const float* normal, int max_contacts, float ke, float kd, float kf,
+                          float mu) {
+  if (!s) return fail(FEM_ERR_ARG, "null system");
+  if (!count || max_contacts <= 0) {  // no contact list: the term is zero
+    s->soft_on = false;
+    return FEM_OK;
+  }
+  if (!particle || !body_pos || !body_vel || !normal) return fail(FEM_ERR_ARG, "null soft-contact array");
+  if (!std::isfinite(ke) || !std::isfinite(kd) || !std::isfinite(kf) || !std::isfinite(mu))
+    return fail(FEM_ERR_ARG, "soft contact coefficients must be finite");
+  if (s->P > 1024 * 1024) return fail(FEM_ERR_UNSUPPORTED, "soft contacts support at most 1048576 particles");
+  if (s->P == 0) {
+    s->soft_on = false;
+    return FEM_OK;
+  }
+  SoftContacts& c = s->soft;
+  const size_t tb = soft_contact_temp_bytes(s->P, max_contacts);
+  if (!c.force || tb > s->soft_temp_cap) {  // grows with the largest contact list seen
+    unsigned char* temp = nullptr;
+    int rc;
+    if ((rc = s->alloc(&temp, tb))) return rc;
+    if (!c.force && (rc = s->alloc(&c.force, 3 * (size_t)s->P))) return rc;
+    if (c.temp) s->drop(c.temp);
+    c.temp = temp;
+    s->soft_temp_cap = tb;
+  }
+  c.P = s->P;
+  c.max_contacts = max_contacts;
+  c.count = count;
+  c.particle = particle;
+  c.body_pos = body_pos;
+  c.body_vel = body_vel;
+  c.normal = normal;
+  c.ke = ke;
+  c.kd = kd;
+  c.kf = kf;
+  c.mu = mu;
+  s->soft_on = true;
+  return FEM_OK;
+}
+
 int fem_set_spring_torque(fem_system* s, const float* rest_direction, float stiffness, float damping) {
   if (!s) return fail(FEM_ERR_ARG, "null system");
   if (!std::isfinite(stiffness) || !std::isfinite(damping))
@@ -3445,6 +3495,14 @@ int particle_contact(const fem_system* s, const float* x, const float* v, hipStr
   if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("particle contact: ") + hipGetErrorString(e));
   return FEM_OK;
 }
+
+// Soft-contact forces of the bound contact list at (x, v) ahead of the force kernel.
+int soft_contact(const fem_system* s, const float* x, const float* v, hipStream_t st) {
+  if (!s->soft_on) return FEM_OK;
+  const hipError_t e = soft_contact_forces(s->soft, x, v, s->d_radius, st);
+  if (e != hipSuccess) return fail(FEM_ERR_HIP, std::string("soft contact: ") + hipGetErrorString(e));
+  return FEM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -3494,6 +3552,7 @@ int fem_eval_forces(fem_system* s, int32_t terms, const float* q, const float* q
   RowArgs ra = s->row_args();
   if ((terms & FEM_TERM_HEX) && (rc_hex = hex_gauss<true>(s, ra, q, qd, (hipStream_t)stream))) return rc_hex;
   if ((terms & FEM_TERM_PARTICLE) && (rc_hex = particle_contact(s, q, qd, (hipStream_t)stream))) return rc_hex;
+  if ((terms & FEM_TERM_SOFT) && (rc_hex = soft_contact(s, q, qd, (hipStream_t)stream))) return rc_hex;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, q, qd, act, (const float*)nullptr, (const float*)s->b_vals, out,
                                s->d_rhs_max, (int)terms, (int)MODE_EVAL, 1.f, (hipStream_t)stream);
 }
@@ -3513,9 +3572,10 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
   }
   RowArgs ra = s->row_args();
   const int terms = FEM_TERM_SPRING | FEM_TERM_TET | FEM_TERM_HEX | FEM_TERM_TRI | FEM_TERM_PARTICLE |
-                    FEM_TERM_GROUND | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
+                    FEM_TERM_GROUND | FEM_TERM_SOFT | FEM_TERM_GRAVITY | FEM_TERM_DAMPING;
   if (int rc = hex_gauss<true>(s, ra, q_in, qd_in, st)) return rc;
   if (int rc = particle_contact(s, q_in, qd_in, st)) return rc;
+  if (int rc = soft_contact(s, q_in, qd_in, st)) return rc;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
 }

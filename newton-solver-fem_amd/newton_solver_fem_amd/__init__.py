@@ -8,6 +8,7 @@ Public names mirror the reference package (``newton_solver_fem``):
 
 from .model import (
     PARTICLE_FLAG_ACTIVE,
+    Contacts,
     Control,
     FEMBuilder,
     HashGrid,
@@ -23,6 +24,7 @@ from .model import (
 
 __all__ = [
     "PARTICLE_FLAG_ACTIVE",
+    "Contacts",
     "Control",
     "FEMBuilder",
     "HashGrid",

@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfem_hip.so"
 FEM_OK = 0
 FEM_ERR_ARG, FEM_ERR_HIP, FEM_ERR_UNSUPPORTED, FEM_ERR_STATE = 1, 2, 3, 4
 FEM_TERM_TET, FEM_TERM_HEX, FEM_TERM_GROUND, FEM_TERM_GRAVITY, FEM_TERM_DAMPING = 1, 2, 4, 8, 16
-FEM_TERM_SPRING, FEM_TERM_TRI, FEM_TERM_PARTICLE = 32, 64, 128
+FEM_TERM_SPRING, FEM_TERM_TRI, FEM_TERM_PARTICLE, FEM_TERM_SOFT = 32, 64, 128, 256
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -82,6 +82,7 @@ SIGNATURES = {
     "fem_bind": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_int64]),
     "fem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "fem_set_particle_contact": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 5),
+    "fem_set_soft_contacts": (ctypes.c_int, [ctypes.c_void_p] * 6 + [ctypes.c_int] + [ctypes.c_float] * 4),
     "fem_set_solver": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "fem_set_spring_torque": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_float, ctypes.c_float]),
     "fem_bind_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

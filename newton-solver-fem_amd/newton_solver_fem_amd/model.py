@@ -89,6 +89,29 @@ class Control:
         self.tet_activations = tet_activations
 
 
+class Contacts:
+    """The soft-contact part of Newton's ``Contacts`` (the collision
+    pipeline's particle-shape contact list, read by the reference's
+    eval_soft_contact_forces solver.py:1540-1566): ``soft_contact_count`` an
+    int32 (1,) device tensor with the number of valid entries,
+    ``soft_contact_particle`` int32 (max,), ``soft_contact_body_pos``,
+    ``soft_contact_body_vel`` and ``soft_contact_normal`` float32 (max, 3)
+    (the normal points from the body to the particle), and
+    ``soft_contact_max`` = max.  Any object with these attributes (torch
+    tensors, or arrays torch.as_tensor takes without a copy) works as well."""
+
+    def __init__(self, max_contacts: int, device):
+        m = int(max_contacts)
+        if m < 0:
+            raise ValueError("max_contacts must be nonnegative")
+        self.soft_contact_max = m
+        self.soft_contact_count = torch.zeros(1, dtype=torch.int32, device=device)
+        self.soft_contact_particle = torch.full((m,), -1, dtype=torch.int32, device=device)
+        self.soft_contact_body_pos = torch.zeros((m, 3), dtype=torch.float32, device=device)
+        self.soft_contact_body_vel = torch.zeros((m, 3), dtype=torch.float32, device=device)
+        self.soft_contact_normal = torch.zeros((m, 3), dtype=torch.float32, device=device)
+
+
 class HashGrid:
     """Marks a model whose particles collide with each other (Newton's
     ``Model.particle_grid``, a Warp hash grid).  The device grid itself is
@@ -118,6 +141,14 @@ class Model:
         self.particle_kf = 1.0e2
         self.particle_mu = 0.5
         self.particle_cohesion = 0.0
+        # Particle-shape soft contact coefficients (Newton's Model
+        # soft_contact_ke / kd / kf / mu, read by the reference's
+        # eval_soft_contact_forces solver.py:1540-1566; the values are
+        # Newton's ModelBuilder defaults).
+        self.soft_contact_ke = 1.0e3
+        self.soft_contact_kd = 10.0
+        self.soft_contact_kf = 1.0e3
+        self.soft_contact_mu = 0.5
         r = np.asarray(host["particle_radius"], dtype=np.float64)
         r = r[np.isfinite(r)]
         self.particle_max_radius = float(r.max()) if r.size else 0.0

@@ -105,8 +105,9 @@ class SolverFEM:
     Constructor arguments follow the reference (solver.py:212-240).  The
     linear solve runs ``solver_type`` ``"cg"`` (the persistent PCG),
     ``"bicgstab"``, ``"cr"`` or ``"gmres"`` with ``preconditioner_type``
-    ``"block_diag"``, ``"diag"``, ``"diag_abs"`` or ``"id"``; soft contacts
-    are outside this substep path and raise ``NotImplementedError``.
+    ``"block_diag"``, ``"diag"``, ``"diag_abs"`` or ``"id"``.  A ``contacts``
+    argument with soft contacts (:class:`Contacts`, Newton's contact list)
+    adds the particle-shape soft-contact term to the step.
     """
 
     def __init__(
@@ -181,6 +182,7 @@ class SolverFEM:
         # Particle-particle contact runs only for models with a hash grid
         # (reference eval_particle_particle_forces solver.py:1477-1486); FEM
         # meshes built without one skip the term whatever self_contact_ke is.
+        self._soft_key = None  # bound soft-contact arrays (none at creation)
         self._particle_contact = (model.particle_ke > 0.0 and model.particle_count > 1
                                   and getattr(model, "particle_grid", None) is not None
                                   and getattr(model, "particle_max_radius", 0.0) > 0.0)
@@ -713,8 +715,7 @@ class SolverFEM:
             from .differentiation import differentiable_step
 
             return differentiable_step(self, state_in, state_out, control, contacts, dt, external_forces)
-        if contacts is not None and hasattr(contacts, "soft_contact_count"):
-            raise NotImplementedError("soft-rigid contacts are not part of this substep path")
+        self._bind_soft_contacts(contacts)
         for name, t in (("state_in.particle_q", state_in.particle_q), ("state_in.particle_qd", state_in.particle_qd),
                         ("state_in.particle_f", state_in.particle_f), ("state_out.particle_q", state_out.particle_q),
                         ("state_out.particle_qd", state_out.particle_qd)):
@@ -756,11 +757,64 @@ class SolverFEM:
         PreparedFEMStep.check_supported(self)
         if torch.cuda.is_current_stream_capturing():
             raise RuntimeError("Prepare FEM mechanics before CUDA graph capture")
-        if contacts is not None and hasattr(contacts, "soft_contact_count"):
-            raise NotImplementedError("soft-rigid contacts are not part of this substep path")
+        self._bind_soft_contacts(contacts)
         if not hasattr(self, "_prepared_step"):
             self._prepared_step = PreparedFEMStep(self)
         return self._prepared_step.prepare(state_in, control, contacts, dt, external_forces)
+
+    def _bind_soft_contacts(self, contacts) -> None:
+        """Bind the contact list's soft contacts (device arrays, read at run
+        time) for the next force evaluations, or unbind them when `contacts`
+        has none (the reference's eval_soft_contact_forces returns zeros then,
+        solver.py:1543-1544).  Rebinding different arrays inside a CUDA graph
+        capture is refused: a recorded graph keeps the arrays of its capture."""
+        if contacts is None or not hasattr(contacts, "soft_contact_count"):
+            key = None
+            arrays = None
+        else:
+            dev = self.device
+
+            def arr(name, dtype, shape):
+                x = torch.as_tensor(getattr(contacts, name), device=dev)
+                if x.dtype != dtype or tuple(x.shape) != shape or not x.is_contiguous():
+                    raise ValueError(f"contacts.{name} must be a contiguous {dtype} {shape} array on the model device")
+                return x
+
+            count = torch.as_tensor(contacts.soft_contact_count, device=dev)
+            if count.dtype != torch.int32 or count.numel() < 1:
+                raise ValueError("contacts.soft_contact_count must be an int32 device array holding the count")
+            part = torch.as_tensor(contacts.soft_contact_particle, device=dev)
+            m = int(getattr(contacts, "soft_contact_max", part.numel()))
+            if m < 0 or m > part.numel():
+                raise ValueError("contacts.soft_contact_max exceeds the contact arrays")
+            arrays = (count, arr("soft_contact_particle", torch.int32, (part.numel(),)),
+                      arr("soft_contact_body_pos", torch.float32, (part.numel(), 3)),
+                      arr("soft_contact_body_vel", torch.float32, (part.numel(), 3)),
+                      arr("soft_contact_normal", torch.float32, (part.numel(), 3)))
+            mdl = self.model
+            coef = (float(mdl.soft_contact_ke), float(mdl.soft_contact_kd), float(mdl.soft_contact_kf),
+                    float(mdl.soft_contact_mu))
+            key = tuple(a.data_ptr() for a in arrays) + (m,) + coef
+        if key == getattr(self, "_soft_key", None):
+            return
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("Step once with these contacts before CUDA graph capture")
+        if key is None:
+            _lib.check(self._lib.fem_set_soft_contacts(self._sys, None, None, None, None, None, 0, 0.0, 0.0, 0.0, 0.0),
+                       "fem_set_soft_contacts")
+        else:
+            _lib.check(self._lib.fem_set_soft_contacts(self._sys, *(a.data_ptr() for a in arrays), m, *coef),
+                       "fem_set_soft_contacts")
+        self._soft_key = key
+        self._soft_arrays = arrays  # the library reads them until the next binding
+
+    def eval_soft_contact_forces(self, model, state, contacts):
+        """Particle-shape penalty + damping + Coulomb friction [N] of the
+        contact list (reference solver.py:1540-1566; zero without one)."""
+        self._bind_soft_contacts(contacts)
+        if self._soft_key is None:
+            return torch.zeros_like(state.particle_q)
+        return self._eval(_lib.FEM_TERM_SOFT, state)
 
     def set_barrier_spin_limit(self, spins: int = 0) -> None:
         """Bound the persistent CG's cross-workgroup waits (flag polls; 0 =

@@ -135,6 +135,46 @@ def particle_contact_forces(q, qd, radius, flags, ke, kd, kf, mu, cohesion=0.0):
     return out
 
 
+def soft_contact_forces(q, qd, radius, count, particle, body_pos, body_vel, normal, ke, kd, kf, mu):
+    """Particle-shape soft-contact force on every particle, FP64.
+
+    Restates eval_soft_contacts (kernels.py:1199-1252): for every contact
+    k < count with particle p = particle[k] >= 0, penetration
+    pen = r_p - (x_p - body_pos[k]).n (n = normal[k], body -> particle); when
+    pen > 0: relative velocity rel = v_p - body_vel[k], vn = rel.n,
+    vt = rel - vn n, fn = ke pen - kd min(vn, 0); if |vt| > 1e-6 the force is
+    fn n - min(kf |vt|, mu max(0, fn)) vt/|vt|, else fn n; forces of one
+    particle are summed in contact order.
+    """
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 3)
+    qd = np.asarray(qd, dtype=np.float64).reshape(-1, 3)
+    r = np.asarray(radius, dtype=np.float64)
+    particle = np.asarray(particle)
+    bp = np.asarray(body_pos, dtype=np.float64).reshape(-1, 3)
+    bv = np.asarray(body_vel, dtype=np.float64).reshape(-1, 3)
+    nrm = np.asarray(normal, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(q)
+    for k in range(min(int(count), len(particle))):
+        p = int(particle[k])
+        if p < 0 or p >= len(q):
+            continue
+        n = nrm[k]
+        pen = r[p] - np.dot(q[p] - bp[k], n)
+        if pen <= 0.0:
+            continue
+        rel = qd[p] - bv[k]
+        vn = np.dot(rel, n)
+        vt = rel - vn * n
+        fn = ke * pen - kd * min(vn, 0.0)
+        vt_mag = np.sqrt(np.dot(vt, vt))
+        if vt_mag > 1.0e-6:
+            ft = min(kf * vt_mag, mu * max(0.0, fn))
+            out[p] += fn * n - ft * (vt / vt_mag)
+        else:
+            out[p] += fn * n
+    return out
+
+
 def spring_torque_forces(q, qd, spring_indices, rest_direction, kt, kw):
     """Bend-torque part of eval_springs_linear_and_torque (kernels.py:222-292), FP64.
 
