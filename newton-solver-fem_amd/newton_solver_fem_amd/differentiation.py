@@ -62,6 +62,8 @@ def prepare(solver):
     Fixed particles (allowed only with allow_fixed_in_differentiation) use
     the projected operator in both solves.
     """
+    if solver.model.particle_ke > 0.0:
+        raise NotImplementedError("Differentiation supports analytic ground contact; disable particle/surface contacts")
     if torch.cuda.is_current_stream_capturing():
         raise RuntimeError("Call prepare_differentiation() before CUDA graph capture")
     if getattr(solver, "_q_asm", None) is None:

@@ -412,3 +412,19 @@ def test_step_gradients_with_triangle_drag_and_lift_match_finite_differences():
         base_q, base_v = float(np.sum(gq_out * dq)), float(np.sum((gv_out + DT * gq_out) * dvd))
         assert abs(float(np.sum(gq * dq)) - fd_q) <= 2e-3 * max(abs(fd_q - base_q), 1e-12) + 1e-6 * abs(fd_q), k
         assert abs(float(np.sum(gv * dvd)) - fd_v) <= 2e-3 * max(abs(fd_v - base_v), 1e-12) + 1e-6 * abs(fd_v), k
+
+
+def test_differentiation_refuses_particle_contact_stiffness():
+    """Reference prepare (differentiation.py:238-242): a model whose
+    particle_ke (SolverFEM's self_contact_ke) is positive cannot be
+    differentiated, grid or not."""
+    print("case diff self_contact_ke", flush=True)
+    from newton_solver_fem_amd.solver import SolverFEM
+
+    model = cube("tet", density=100.0, gravity=(0.0, 0.0, -9.81), device=DEV)
+    s = SolverFEM(model, solver_type="cg", preconditioner_type="block_diag", differentiable=True, dt=DT)
+    a, b = model.state(), model.state()
+    with pytest.raises(NotImplementedError, match="analytic ground contact"):
+        s.step(a, b, None, None, DT)
+    with pytest.raises(NotImplementedError, match="analytic ground contact"):
+        s.prepare_differentiation()
