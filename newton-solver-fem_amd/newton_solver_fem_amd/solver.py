@@ -672,6 +672,8 @@ class SolverFEM:
         if dt != self.dt:
             if self.differentiable:
                 raise ValueError("Differentiable tangent refresh requires the construction timestep")
+            if torch.cuda.is_current_stream_capturing():
+                raise RuntimeError("Change the tangent timestep outside tapes and CUDA graph capture")
             if self._streaming:
                 raise ValueError("Streaming tangent refresh requires the construction timestep")
             self.dv.zero_()
@@ -715,6 +717,13 @@ class SolverFEM:
             from .differentiation import differentiable_step
 
             return differentiable_step(self, state_in, state_out, control, contacts, dt, external_forces)
+        if torch.is_grad_enabled() and any(
+                x is not None and x.requires_grad
+                for x in (state_in.particle_q, state_in.particle_qd, external_forces)):
+            # The reference refuses a step of a non-differentiable solver on a
+            # recording tape (solver.py:985-986); inputs that require
+            # gradients are the autograd counterpart.
+            raise RuntimeError("Construct the solver with differentiable=True before recording a step")
         self._bind_soft_contacts(contacts)
         for name, t in (("state_in.particle_q", state_in.particle_q), ("state_in.particle_qd", state_in.particle_qd),
                         ("state_in.particle_f", state_in.particle_f), ("state_out.particle_q", state_out.particle_q),

@@ -12,6 +12,10 @@
   one-poll bound on a four-workgroup solve; every kernel still exits
   normally) leaves NaN in the output state instead of a partial solve, and
   solve_stats() raises; the next solve after a reset is exact again.
+* Error behaviour of the reference's step and tangent refresh: a step of a
+  non-differentiable solver on inputs that record gradients is refused
+  (solver.py:985-986), and so is a timestep change while a CUDA graph is
+  being captured (solver.py:801-802); nothing is recorded by either.
 """
 
 import numpy as np
@@ -120,3 +124,21 @@ def test_barrier_timeout_leaves_nan_not_a_plausible_state(variant):
     assert s.solve_stats()["iterations"][0] > 0
     np.testing.assert_array_equal(np_(s1.particle_q), np_(f1.particle_q))
     np.testing.assert_array_equal(np_(s1.particle_qd), np_(f1.particle_qd))
+
+
+def test_step_refuses_gradient_inputs_and_capture_refuses_a_timestep_change(monkeypatch):
+    print("case boundary tape and capture guards", flush=True)
+    model = cube("tet", density=100.0, gravity=(0.0, 0.0, -9.81), device=DEV)
+    s = solver_for(model, dt=0.002, self_contact_ke=0.0)
+    a, b = model.state(), model.state()
+    s.refresh_tangent(a.particle_q)
+    a.particle_q.requires_grad_(True)
+    with pytest.raises(RuntimeError, match="differentiable=True before recording"):
+        s.step(a, b, None, None, 0.002)
+    with torch.no_grad():  # outside autograd recording the step runs
+        s.step(a, b, None, None, 0.002)
+    a.particle_q.requires_grad_(False)
+    # the capture state is simulated: nothing is recorded
+    monkeypatch.setattr(torch.cuda, "is_current_stream_capturing", lambda: True)
+    with pytest.raises(RuntimeError, match="outside tapes and CUDA graph capture"):
+        s.refresh_tangent(a.particle_q, 0.001)
