@@ -254,6 +254,15 @@ int fem_step_rhs(fem_system* sys, float* q_in_dev, float* qd_in_dev, float* f_ou
 int fem_step_solve(fem_system* sys, const float* q_in_dev, const float* qd_in_dev, const float* f_in_dev,
                    float* q_out_dev, float* qd_out_dev, float dt, void* stream);
 
+/* SolverFEM.implicit_integration's right-hand side (solver.py:1254-1307):
+ * f_dev holds the caller's h-scaled force sum (state_in.particle_f, forces
+ * and external loads already multiplied by dt); it is overwritten, in place,
+ * with that sum minus the elastic predictor (subtract_elastic_predictor
+ * implicit_predictor.py:9-23), plus the spring-damping restore (:26-41),
+ * fixed rows zeroed.  fem_step_solve(f_in = f_dev) then finishes the
+ * integration.  The input state is not touched. */
+int fem_integrate_rhs(fem_system* sys, const float* qd_in_dev, float* f_dev, float dt, void* stream);
+
 /* fem_step_solve without library-allocated scratch, for callers that cannot
  * allocate device memory (code under HIP graph capture that must not touch
  * the allocator, tight serving loops, harnesses that forbid hipMalloc).  No

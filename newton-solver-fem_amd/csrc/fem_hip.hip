@@ -534,7 +534,9 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
 // ---------------------------------------------------------------------------
 // k_rhs: step right-hand side (STEP) or plain force evaluation (EVAL).
 // ---------------------------------------------------------------------------
-enum { MODE_EVAL = 0, MODE_STEP = 1 };
+// MODE_LOAD: the right-hand side of a step whose h-scaled force sum the
+// caller provides in `ext` (implicit_integration): ext - predictor terms.
+enum { MODE_EVAL = 0, MODE_STEP = 1, MODE_LOAD = 2 };
 
 template <int G, int EL>
 __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restrict__ x, const float* __restrict__ v,
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
         fh = fh + hex_node_force(a, code >> 3, code & 7);
       }
     }
-    if (mode == MODE_STEP) {
+    if (mode != MODE_EVAL) {
       const v3 vi = ld3(v, i);
       const int k1 = a.offsets[i + 1];
       for (int k = a.offsets[i] + lane; k < k1; k += G) {
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
     ftri = {group_sum<G>(ftri.x), group_sum<G>(ftri.y), group_sum<G>(ftri.z)};
     srest = {group_sum<G>(srest.x), group_sum<G>(srest.y), group_sum<G>(srest.z)};
   }
-  if (mode == MODE_STEP) corr = {group_sum<G>(corr.x), group_sum<G>(corr.y), group_sum<G>(corr.z)};
+  if (mode != MODE_EVAL) corr = {group_sum<G>(corr.x), group_sum<G>(corr.y), group_sum<G>(corr.z)};
   float mval = 0.f;
   if (active && lane == 0) {
     const bool on = (a.flags[i] & 1) != 0;
@@ -676,12 +678,20 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(RowArgs a, const float* __restric
       if (EL & 4) f = f + srest;
       if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
       mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
+    } else if (mode == MODE_LOAD) {
+      // implicit_integration solver.py:1254-1307: the caller's h-scaled force
+      // sum, minus the elastic predictor, plus the spring-damping restore, with
+      // fixed rows projected out (FixedParticles.project_rhs).
+      f = ld3(ext, i) - corr;
+      if (EL & 4) f = f + srest;
+      if (a.row_fixed[i]) f = {0.f, 0.f, 0.f};
+      mval = fmaxf(fabsf(f.x), fmaxf(fabsf(f.y), fabsf(f.z)));
     } else {
       f = fsp + ft + fh + ftri + fpc + fg + fsc + fgr + fdm;
     }
     st3(out, i, f);
   }
-  if (mode == MODE_STEP) {
+  if (mode != MODE_EVAL) {
     // maximum_rhs_component kernels.py:1779-1782, per instance, spread over
     // RHS_MAX_SLOTS words so that no address sees more than a few atomics.
     if (a.row_inst == nullptr) {
@@ -3578,6 +3588,25 @@ int fem_step_rhs(fem_system* s, float* q_in, float* qd_in, float* f_out, const f
   if (int rc = soft_contact(s, q_in, qd_in, st)) return rc;
   return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, (const float*)q_in, (const float*)qd_in, act, ext,
                                (const float*)s->b_vals, f_out, s->d_rhs_max, terms, (int)MODE_STEP, dt, st);
+}
+
+// Right-hand side from a caller-provided h-scaled force sum, in place.
+int fem_integrate_rhs(fem_system* s, const float* qd_in, float* f, float dt, void* stream) {
+  if (!s || !qd_in || !f) return fail(FEM_ERR_ARG, "null argument");
+  if (!s->b_vals) return fail(FEM_ERR_STATE, "fem_bind must be called first");
+  if (!(dt > 0.f) || !std::isfinite(dt)) return fail(FEM_ERR_ARG, "dt must be finite and positive");
+  if (s->P == 0) return FEM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  {
+    int n = std::max(s->ninst * RHS_MAX_SLOTS, 1);
+    hipLaunchKernelGGL(k_prepare, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const int*)nullptr, 0,
+                       (const float*)nullptr, (float*)nullptr, (float*)nullptr, s->d_rhs_max, s->ninst);
+    HIP_TRY(hipGetLastError());
+  }
+  RowArgs ra = s->row_args();
+  // No force term is evaluated: positions are not read (qd stands in for x).
+  return dispatch_g<LaunchRhs>(s->elem_g, s->full(), ra, qd_in, qd_in, (const float*)nullptr, (const float*)f,
+                               (const float*)s->b_vals, f, s->d_rhs_max, 0, (int)MODE_LOAD, dt, st);
 }
 
 }  // extern "C"

@@ -754,6 +754,39 @@ class SolverFEM:
             self.last_cg_iters = min(self.linear_solver_maxiter, -(-it // n) * n)
         return state_out
 
+    def implicit_integration(self, model, state_in: State, state_out: State, dt: float) -> None:
+        """Solve ``A dv = h f - h^2 K v`` and integrate ``v``, ``x`` (reference
+        solver.py:1254-1351), the second half of :meth:`step`.
+
+        ``state_in.particle_f`` must hold the force sum already multiplied by
+        ``dt`` (external loads included); it is overwritten with the projected
+        right-hand side.  The input state's fixed nodes are not reset (the
+        reference's step does that before the forces); ``state_out`` receives
+        the integrated state with fixed nodes at their anchors.
+        """
+        P = self.model.particle_count
+        if not (math.isfinite(dt) and dt > 0):
+            raise ValueError("dt must be finite and positive")
+        for name, t in (("state_in.particle_q", state_in.particle_q), ("state_in.particle_qd", state_in.particle_qd),
+                        ("state_in.particle_f", state_in.particle_f), ("state_out.particle_q", state_out.particle_q),
+                        ("state_out.particle_qd", state_out.particle_qd)):
+            _check_vec3(name, t, P, self.device)
+        self._sync_params()
+        self._stepped = True
+        st = _stream(self.device)
+        _lib.check(self._lib.fem_integrate_rhs(self._sys, state_in.particle_qd.data_ptr(),
+                                               state_in.particle_f.data_ptr(), float(dt), st), "fem_integrate_rhs")
+        _lib.check(
+            self._lib.fem_step_solve(self._sys, state_in.particle_q.data_ptr(), state_in.particle_qd.data_ptr(),
+                                     state_in.particle_f.data_ptr(), state_out.particle_q.data_ptr(),
+                                     state_out.particle_qd.data_ptr(), float(dt), st),
+            "fem_step_solve",
+        )
+        if self.linear_solver_check_every > 0:
+            it = int(self.solve_stats()["iterations"][0])
+            n = int(self.linear_solver_check_every)
+            self.last_cg_iters = min(self.linear_solver_maxiter, -(-it // n) * n)
+
     def prepare_step(self, state_in: State, control: Control | None, contacts, dt: float, *,
                      external_forces: torch.Tensor | None = None):
         """Prepare reusable affine mechanics without integrating physical state

@@ -207,3 +207,48 @@ def test_spring_bend_torque_matches_oracle():
                                                                         np_(st.particle_qd))) < 1e-5
     with pytest.raises(ValueError, match="spring_rest_direction shape"):
         solver_for(model, dt=dt, torque_stiffness=kt, spring_rest_direction=rest[:-1])
+
+
+def test_implicit_integration_equals_step_and_oracle():
+    """SolverFEM.implicit_integration (reference solver.py:1254-1351) given
+    state_in.particle_f = h (sum of the force terms + external load) gives the
+    step's state, and the oracle's; the mixed model exercises the elastic
+    predictor, the spring-damping restore and the fixed-row projection."""
+    print("case implicit integration", flush=True)
+    model = on_device(mixed_model())
+    dt = 0.002
+    kw = dict(linear_solver_maxiter=80, linear_solver_tolerance=1e-7, fixed_particle_indices=(0, 1))
+    s = solver_for(model, dt=dt, **kw)
+    ref = oracle_for(model, dt=dt, fixed=(0, 1), linear_solver_tolerance=1e-12, linear_solver_maxiter=200)
+    q = model.host["particle_q"].astype(np.float64).copy()
+    q[2] += (0.002, -0.001, 0.001)
+    q[6] += (-0.001, 0.002, -0.003)
+    v = np.linspace(-0.02, 0.03, 24).reshape(8, 3)
+    v[:2] = 0
+    ext = torch.tensor(np.linspace(-0.01, 0.02, 24, dtype=np.float32).reshape(8, 3), device=DEV)
+    a, b, c = model.state(), model.state(), model.state()
+    for st in (a, c):
+        st.particle_q.copy_(torch.tensor(q, dtype=torch.float32))
+        st.particle_qd.copy_(torch.tensor(v, dtype=torch.float32))
+    s.refresh_tangent(a.particle_q)
+    # h * (every force term + external load), then the second half of the step
+    total = (s.eval_spring_forces(model, c) + s.eval_tetrahedral_forces(model, None, c)
+             + s.eval_hexahedral_forces(model, None, c) + s.eval_triangle_forces(model, None, c)
+             + s.eval_particle_ground_contact_forces(model, None, c) + s.eval_gravity_forces(model, c)
+             + s.eval_linear_damping_forces(model, c) + ext)
+    c.particle_f.copy_(total * dt)
+    s.dv.zero_()
+    s.implicit_integration(model, c, b, dt)
+    torch.cuda.synchronize()
+    q_ii, v_ii = np_(b.particle_q), np_(b.particle_qd)
+    s.dv.zero_()
+    s.step(a, b, None, None, dt, external_forces=ext)
+    torch.cuda.synchronize()
+    assert rel(v_ii, np_(b.particle_qd)) < 2e-5
+    assert rel(q_ii, np_(b.particle_q)) < 1e-6
+    ref.refresh_tangent(q)
+    out = ref.step(q, v, external_forces=np_(ext))
+    assert rel(v_ii, out["qd"]) < 2e-5
+    assert rel(q_ii, out["q"]) < 1e-6
+    np.testing.assert_array_equal(q_ii[:2], model.host["particle_q"][:2])  # fixed nodes at their anchors
+    np.testing.assert_array_equal(v_ii[:2], 0.0)
