@@ -61,6 +61,17 @@ def hex_reference_data(rest: np.ndarray, hexes: np.ndarray):
     return np.linalg.inv(jac), np.abs(det)
 
 
+def _rigid(xform):
+    """Rotation matrix and translation of a ``(px, py, pz, qx, qy, qz, qw)`` transform."""
+    v = np.asarray(xform, dtype=np.float64).reshape(7)
+    p = v[:3]
+    x, y, z, w = v[3:] / np.linalg.norm(v[3:])
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    return R, p
+
+
 @dataclass
 class LinearFEMData:
     """Hex8 reference geometry and materials stored under ``model.fem``."""
@@ -366,6 +377,41 @@ class ModelBuilder:
         self._tet_act.append(act)
         return volume
 
+    def add_builder(self, builder: "ModelBuilder", xform=None, label_prefix=None) -> None:
+        """Append another builder's particles and elements (Newton's
+        ``ModelBuilder.add_builder``).  ``xform`` is a rigid transform
+        ``(px, py, pz, qx, qy, qz, qw)`` (a ``wp.transform``'s layout) applied to
+        the copied positions and, by its rotation, velocities; element rest
+        data (tet and triangle poses, spring rest lengths) are invariant under
+        it and copied as they are.  ``label_prefix`` is accepted for Newton
+        compatibility (this builder keeps no labels)."""
+        offset = self.particle_count
+        q = np.asarray(builder._q, dtype=np.float64).reshape(-1, 3)
+        qd = np.asarray(builder._qd, dtype=np.float64).reshape(-1, 3)
+        if xform is not None:
+            R, p = _rigid(xform)
+            q = q @ R.T + p
+            qd = qd @ R.T
+        self._q.extend(q)
+        self._qd.extend(qd)
+        self._m.extend(builder._m)
+        self._r.extend(builder._r)
+        self._flags.extend(builder._flags)
+        for idx, pose, mat, act in zip(builder._tet_idx, builder._tet_pose, builder._tet_mat, builder._tet_act):
+            self._tet_idx.append((np.asarray(idx) + offset).astype(np.int32))
+            self._tet_pose.append(np.array(pose))
+            self._tet_mat.append(np.array(mat))
+            self._tet_act.append(np.array(act))
+        self._spring_idx.extend((i + offset, j + offset) for i, j in builder._spring_idx)
+        self._spring_rest.extend(builder._spring_rest)
+        self._spring_ke.extend(builder._spring_ke)
+        self._spring_kd.extend(builder._spring_kd)
+        self._spring_control.extend(builder._spring_control)
+        self._tri_idx.extend((i + offset, j + offset, k + offset) for i, j, k in builder._tri_idx)
+        self._tri_pose.extend(np.array(x) for x in builder._tri_pose)
+        self._tri_mat.extend(builder._tri_mat)
+        self._tri_area.extend(builder._tri_area)
+
     def _host_arrays(self) -> dict:
         p = len(self._m)
         mass = np.asarray(self._m, dtype=np.float64)
@@ -459,6 +505,24 @@ class FEMBuilder(ModelBuilder):
                     pairs.append(key)
         if pairs:
             self.add_springs(pairs, float(k_mu) * 0.5, float(k_damp) * 0.5, 0.0)
+
+    def add_builder(self, builder, xform=None, label_prefix=None) -> None:
+        """Copy another builder and remap its Hex8 indices (reference
+        builder.py:36-49): a builder with Hex8 elements may be placed by a
+        translation only, as Hex8 rest quadrature is taken from the final
+        positions without rotating the elements' frames."""
+        indices = getattr(builder, "fem_hex_indices", [])
+        if indices and xform is not None:
+            R, _ = _rigid(xform)
+            if not np.allclose(R, np.eye(3), atol=1e-6):
+                raise ValueError(
+                    "Newton 1.7 particle replication supports translation only; rotate Hex8 rest vertices before adding them"
+                )
+        offset = self.particle_count
+        super().add_builder(builder, xform=xform, label_prefix=label_prefix)
+        for idx, mat in zip(indices, getattr(builder, "fem_hex_materials", [])):
+            self.fem_hex_indices.append((np.asarray(idx) + offset).astype(np.int32))
+            self.fem_hex_materials.append(np.array(mat))
 
     def add_soft_hex_mesh(self, hex_indices, k_mu=1.0e3, k_lambda=1.0e3, k_damp=0.0):
         indices = np.asarray(hex_indices, dtype=np.int32)
