@@ -257,6 +257,12 @@ class ModelBuilder:
         return list(self._q)
 
     @property
+    def particle_mass(self):
+        """The builder's particle masses, as Newton's mutable list (assigning an
+        entry, e.g. 0 for a support, changes the particle's mass)."""
+        return self._m
+
+    @property
     def tet_count(self) -> int:
         return sum(len(a) for a in self._tet_idx)
 
@@ -460,7 +466,7 @@ class FEMBuilder(ModelBuilder):
         self.fem_hex_materials = []
 
     def add_soft_mesh(self, pos, rot, scale, vel, vertices, indices, density, k_mu, k_lambda, k_damp, tri_ke=None,
-                      tri_ka=None, tri_kd=None, tri_drag=None, tri_lift=None):
+                      tri_ka=None, tri_kd=None, tri_drag=None, tri_lift=None, add_surface_mesh_edges=True):
         """Tet4 mesh with lumped masses, open-face triangles and a unique-edge spring network.
 
         Particles at ``rot * (scale * v) + pos`` (``rot`` a quaternion x, y, z, w)
@@ -469,6 +475,9 @@ class FEMBuilder(ModelBuilder):
         membrane (Newton's ``add_soft_mesh``).  Each unique edge of a tet with
         positive rest volume receives one spring of stiffness ``k_mu / 2`` and
         damping ``k_damp / 2`` (reference builder.py:51-79).
+        ``add_surface_mesh_edges`` is accepted for Newton compatibility: the
+        bending edges it adds there are no force term of this solver (nor of
+        the reference's), so none are stored.
         """
         verts = np.asarray(vertices, dtype=np.float64).reshape(-1, 3) * float(scale)
         x, y, z, w = (float(c) for c in np.asarray(rot, dtype=np.float64).reshape(4))
