@@ -249,6 +249,44 @@ def test_all_fixed_boundary_restored():
     np.testing.assert_array_equal(np_(s.velocity_increment), np.zeros((4, 3)))
 
 
+def test_fixed_capture_uses_live_forces_and_exact_anchors():
+    """Reference test_fixed_particles.py:92-120: a recorded tangent refresh +
+    step replays live external forces against the reduced dense solve of the
+    free nodes, with the fixed nodes exactly at their anchors."""
+    print("case fixed capture", flush=True)
+    model = tet_pair(device=DEV)
+    s = solver_for(model, dt=0.002, fixed_particle_indices=(0, 1), linear_solver_maxiter=64)
+    cur, nxt = model.state(), model.state()
+    forces = torch.zeros(4, 3, dtype=torch.float32, device=DEV)
+    s.refresh_tangent(cur.particle_q)
+    matrix = s.A_bsr.dense()
+    elastic = np_(s.eval_tetrahedral_forces(model, None, cur))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        s.step(cur, nxt, None, None, 0.002, external_forces=forces)
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        s.refresh_tangent(cur.particle_q)
+        s.step(cur, nxt, None, None, 0.002, external_forces=forces)
+    rest = model.host["particle_q"].astype(np.float32)
+    for value in (0.02, -0.03, 0.0):
+        force = np.zeros((4, 3), dtype=np.float32)
+        force[2, 2] = value
+        forces.copy_(torch.tensor(force))
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(np_(nxt.particle_q)[:2], rest[:2])
+        np.testing.assert_array_equal(np_(nxt.particle_qd)[:2], np.zeros((2, 3)))
+        expected = np.zeros(12)
+        rhs = 0.002 * (elastic + force).ravel()
+        expected[6:] = np.linalg.solve(matrix[6:, 6:], rhs[6:])
+        np.testing.assert_allclose(np_(nxt.particle_qd).ravel(), expected, rtol=5e-4, atol=3e-8)
+        if value:
+            assert np_(nxt.particle_qd)[2, 2] * value > 0
+
+
 def test_true_residual_within_tolerance():
     """||A dv - b|| / ||b|| <= tol for the production 1e-2 solve (test_solver.py:403-405)."""
     print("case residual", flush=True)
