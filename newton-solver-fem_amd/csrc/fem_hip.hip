@@ -400,14 +400,29 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
   int* sptr = reinterpret_cast<int*>(st_h + ST_HEX * (size_t)nh);     // [ke-kb+1] slot -> first contribution
   uint16_t* codes = reinterpret_cast<uint16_t*>(sptr + (ke - kb + 1));  // [nc]
   uint8_t* srow = reinterpret_cast<uint8_t*>(codes + nc);             // [ke-kb] slot -> local row
-  if (threadIdx.x <= nr) roff[threadIdx.x] = a.offsets[r0 + threadIdx.x] - kb;
+  // The first incidence of every thread is fetched ahead of the row set-up
+  // (its code, then its nodes and element data), so that the element pass's
+  // chain of dependent loads overlaps the set-up's loads.
+  const bool first = (int)threadIdx.x < nt;
+  const int code_f = first ? a.tinc[tb + (int)threadIdx.x] : 0;
+  // Row metadata: loads with clamped indices (no divergent branch before
+  // they are issued), stores for the group's rows only.
+  const int rl = min((int)threadIdx.x, nr), rr = min((int)threadIdx.x, nr - 1);
+  const int off_l = a.offsets[r0 + rl], off_n = a.offsets[r0 + rr + 1];
+  const int tptr = a.tinc_ptr[r0 + rr], hptr = a.hinc_ptr[r0 + rr], drel = a.diag_rel[r0 + rr];
+  const float mrow = a.mass[r0 + rr];
+  const int e_f = code_f >> 2;
+  const int4 nd_f = first ? *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e_f) : make_int4(0, 0, 0, 0);
+  const v3 zf = {0.f, 0.f, 0.f};
+  const v3 x0_f = first ? ld3(x, nd_f.x) : zf, x1_f = first ? ld3(x, nd_f.y) : zf;
+  const v3 x2_f = first ? ld3(x, nd_f.z) : zf, x3_f = first ? ld3(x, nd_f.w) : zf;
+  if (threadIdx.x <= nr) roff[threadIdx.x] = off_l - kb;
   if (threadIdx.x < nr) {
-    const int i = r0 + threadIdx.x;
-    rtb[threadIdx.x] = a.tinc_ptr[i] - tb;
-    rhb[threadIdx.x] = a.hinc_ptr[i] - hb;
-    rdiag[threadIdx.x] = a.diag_rel[i];
-    rmass[threadIdx.x] = a.mass[i];
-    const int s0 = a.offsets[i] - kb, s1 = a.offsets[i + 1] - kb;
+    rtb[threadIdx.x] = tptr - tb;
+    rhb[threadIdx.x] = hptr - hb;
+    rdiag[threadIdx.x] = drel;
+    rmass[threadIdx.x] = mrow;
+    const int s0 = off_l - kb, s1 = off_n - kb;
     for (int k = s0; k < s1; ++k) srow[k] = (uint8_t)threadIdx.x;
   }
   // Phase 2's slot pointers and contribution codes: the first ASM_PF per
@@ -423,11 +438,16 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(RowArgs a, const int* __rest
     pf_code[u] = j < nc ? a.contrib[cb + j] : (uint16_t)0;
   }
   for (int t = threadIdx.x; t < nt; t += BLOCK) {
-    const int code = a.tinc[tb + t];
+    const bool pre = t == (int)threadIdx.x;
+    const int code = pre ? code_f : a.tinc[tb + t];
     const int e = code >> 2, la = code & 3;
-    const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
     TetGeo g;
-    tet_geo(ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w), a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+    if (pre) {
+      tet_geo(x0_f, x1_f, x2_f, x3_f, a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+    } else {
+      const int4 nd = *reinterpret_cast<const int4*>(a.tet_idx + 4 * (size_t)e);
+      tet_geo(ld3(x, nd.x), ld3(x, nd.y), ld3(x, nd.z), ld3(x, nd.w), a.tet_pose + 9 * (size_t)e, a.mn, a.mx, g);
+    }
     float K[4][9];
     tet_tangent_row(g, a.tet_mat[3 * e], a.tet_mat[3 * e + 1], h2, la, K);
     float* o = st_t + ST_TET * t;
