@@ -2775,9 +2775,16 @@ struct fem_system {
     // most one 512-thread workgroup per CU for large ones (2 rows per thread
     // of the 256-thread kernel); the resident kernel also needs enough chunks
     // for every chunk's blocks to fit on chip.  cg_wg_request overrides.
+    // A single resident system takes about 160 rows per workgroup: below one
+    // workgroup per CU the iteration is bound by each chunk's serial block
+    // products, not by the barrier (measured on an 18,785-node Hex8 mesh:
+    // 13.8 us per iteration at 48 chunks, 11.4 us at 96-128, 12.0 us at
+    // 256).  Batches keep one chunk per instance where that suffices, so an
+    // instance is split as a looped one-workgroup solve of it would be.
     const long res_blocks = (long)RES_NT * (RES_JR + RES_JL + RES_JS);  // blocks one resident chunk holds
+    const long rows_per_wg = (res && ninst == 1) ? 160 : 512;
     // (The 256-thread streamed kernel holds two workgroups per CU.)
-    long natural = (nt == 512 || res) ? std::min<long>(P / 512, ncu) : std::min<long>(P / 512, 2L * ncu);
+    long natural = (nt == 512 || res) ? std::min<long>(P / rows_per_wg, ncu) : std::min<long>(P / 512, 2L * ncu);
     if (res)
       natural = std::min<long>(ncu, std::max<long>(natural, (long)std::ceil((double)koff[P] / (0.95 * res_blocks))));
     long target = cg_wg_request > 0 ? cg_wg_request : std::max<long>(need, natural);

@@ -601,7 +601,7 @@ class SolverFEM:
         dv0 = self.dv.clone()
         s_in, s_out = self.model.state(), self.model.state()
         if cg_workgroups is None:
-            base = max(1, P // 512)
+            base = max(1, P // 160)
             cg_workgroups = sorted({0, 128, 256, 384, 512, base})
 
         def timed() -> float:
