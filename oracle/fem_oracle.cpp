@@ -16,6 +16,14 @@
 //
 // Parity pinning of each piece is done by tests/test_oracle_*.py (finite
 // differences of an independent energy, dense solves, closed forms, numpy SVD).
+// Parity unpinned (Warp and Newton cannot be imported to compare with): the
+// bitwise conventions of Warp's svd3 for degenerate F (only the clamp result
+// is pinned, which does not depend on them for distinct or clamped-equal
+// singular values); the exact iterate sequence of Warp's cg and its fp32 dot
+// order; the host-check loop count for check_every > 0 (restated as rounding up
+// to the next check); wp.inverse rounding; the eps32^(9/4) residual floor
+// constant; the blocks prune_numerical_zeros would drop when an element block
+// sums to exactly zero (every valid triplet is kept here).
 
 #include <algorithm>
 #include <cmath>

@@ -10,6 +10,12 @@ matrices, with every update masked by the latest residual test
 rr > atol^2 (atol = max(tol ||b||, eps32^(9/4))), so that running the whole
 budget gives the result of stopping at convergence.  Only ``tests/`` import
 this module, as the checker.
+
+Parity unpinned: Warp's exact bicgstab / cr / gmres iterate sequences and
+their breakdown guards (warp-lang cannot be imported here); these
+restatements are pinned through the minimisation properties of the cr and
+gmres iterates, convergence to dense solves and the point-Jacobi kinds
+(tests/test_oracle_krylov.py).
 """
 
 from __future__ import annotations
