@@ -218,7 +218,9 @@ def rollout(solver, q: torch.Tensor, qd: torch.Tensor, steps: int, *, activation
     (steps, P, 3).  ``checkpoint_every=k > 0`` stores only every k-th state
     for the backward pass and recomputes each k-step segment from it
     (torch.utils.checkpoint), trading one extra forward for O(steps / k)
-    memory instead of O(steps).  ``after_step(q, qd) -> (q, qd)`` is applied
+    memory instead of O(steps).  The last segment is recorded directly: the
+    backward pass would recompute it first, so keeping its states raises no
+    memory peak and saves that segment's recomputation.  ``after_step(q, qd) -> (q, qd)`` is applied
     after every substep (differentiably), e.g. the support projection the
     reference uses for zero-mass pinned nodes (tests/diffsim_optimize.py:29-45).
     """
@@ -252,5 +254,8 @@ def rollout(solver, q: torch.Tensor, qd: torch.Tensor, steps: int, *, activation
         extra.append(x if isinstance(x, torch.Tensor) else None)
     for k0 in range(0, steps, checkpoint_every):
         k1 = min(steps, k0 + checkpoint_every)
-        q, qd = ckpt.checkpoint(run, q, qd, k0, k1, *extra, use_reentrant=False)
+        if k1 == steps:
+            q, qd = run(q, qd, k0, k1, *extra)
+        else:
+            q, qd = ckpt.checkpoint(run, q, qd, k0, k1, *extra, use_reentrant=False)
     return q, qd
