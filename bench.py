@@ -745,7 +745,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding is a design only "
-                                   "(row slabs with one halo exchange and two scalar all-reduces per CG iteration)",
+                                   "(row slabs with one halo exchange and one 64-byte all-reduce of the partial sums per CG iteration)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
